@@ -1,0 +1,158 @@
+/*
+ * oracle.h -- CPU restatement of RESPACK's cRPA polarisation path
+ * (src/chiqw) and the Wannier-basis integrals (src/calc_int).
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this library, and only as the checker.  The product
+ * (respackdev_amd, librespack_gpu.so) never links, loads or calls it.
+ *
+ * Conventions: 0-based indices; complex numbers are C99 double complex (two
+ * adjacent doubles, the same bytes as Fortran complex(8)); arrays that the
+ * reference stores column-major keep that element order (first index fastest).
+ * Each function cites the reference routine (file:line under src/) it restates.
+ */
+#ifndef RESPACK_ORACLE_H
+#define RESPACK_ORACLE_H
+#include <complex.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double _Complex ro_z;
+
+/* chiqw/est_nwx2.f90:1-13 */
+int ro_algn235(int n);
+/* chiqw/gen_grid.f90:1-39 */
+void ro_gen_grid(int nt, double emax, ro_z *em);
+/* chiqw/chiqw_sub.F:773-812; kg is kg[3*i+d]; returns NG (capped writes at ntg) */
+int ro_make_kg0(int ntg, const double *b1, const double *b2, const double *b3,
+                double gcut, double q1, double q2, double q3, int32_t *kg);
+/* chiqw/chiqw_sub.F:668-716 and :718-771 */
+void ro_kcheck(double *k, int32_t *rw);
+void ro_kcheck_trs(double *k, int32_t *rw);
+/* chiqw/chiqw_sub.F:76-95 */
+void ro_search_list(const double *sqi, double *sq);
+/* chiqw/chiqw_sub.F:1-75; returns ikq (0-based), -1 if not found */
+int ro_search_kq(int ntk, const double *sk0, double q1, double q2, double q3,
+                 int ik, int32_t *shift_g);
+/* chiqw/est_NTK.f90:1-39 */
+void ro_est_nkbi(int n, const double *sk, int32_t *nkb);
+/* chiqw/chiqw.F:437-480 (unfolding; also est_NTK.f90:42-98 for the count).
+ * rg[9*iop + i + 3*j] = rg(i+1,j+1,iop+1).  Outputs sized 2*nk_irr*nsym;
+ * returns NTK. numirr/numrot are 0-based. */
+int ro_unfold_k(int nk_irr, int nsym, const double *ski, const int32_t *rg,
+                double *sk0, int32_t *numirr, int32_t *numrot, int32_t *trs,
+                int32_t *rw);
+/* chiqw/chiqw_sub.F:324-384; idx[ix + n1*(iy + n2*iz)] = 0-based k */
+void ro_make_index_kpt(int ntk, int n1, int n2, int n3, const double *sk0,
+                       int32_t *idx);
+/* chiqw/chiqw_sub.F:570-613; eig[ib + ntb*ik] */
+void ro_judge_fermi(int nk, int ntb, const double *eig, double ef, int32_t *win,
+                    int32_t *tindx, double *eave, double *bmax, double *bmin);
+/* chiqw/chiqw_sub.F:264-322; pack is (2l1+1)(2l2+1)(2l3+1), 1-based G index
+ * values as in the reference (0 = absent); rginv[i + 3*j]. */
+void ro_make_c0(int itrs, int ng, const int32_t *kg, const int32_t *rw,
+                const double *rginv, const int32_t *pg, int nnp, int l1, int l2,
+                int l3, const int32_t *pack, const ro_z *occ, ro_z *c0);
+
+/* m_tetrahedron_20170325.F:505-584 (libtetrabz corner table); idx[4*t + v] is
+ * the 0-based k index of corner v of tetrahedron t (6 per k-cube). */
+void ro_ttrhdrn_mkidx(int n1, int n2, int n3, const double *b1, const double *b2,
+                      const double *b3, int32_t *idx);
+/* m_tetrahedron_20170325.F:99-141; xo[ie + ne*ik] is ACCUMULATED. */
+void ro_ttrhdrn_chi0(double dmna, double dmnr, int n1, int n2, int n3,
+                     const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
+                     double f2, double delt, double dsgn, int ne, const ro_z *em,
+                     ro_z *xo);
+/* m_tetrahedron_20170325.F:63-76 (ne = 1, em = 0); xo accumulated */
+void ro_ttrhdrn_simple(double dmna, double dmnr, int n1, int n2, int n3,
+                       const int32_t *idx, const ro_z *fk, const ro_z *gk, ro_z *xo);
+/* m_tetrahedron_20170325.F:209-302 for one tetrahedron: fk[4], gk[4] in,
+ * xo[ie + ne*v] accumulated (v = 0..3). Exposed for unit tests. */
+void ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk,
+                     int ne, const ro_z *em, ro_z *xo);
+
+/* Brute-force 3D DFT on an nx*ny*nz grid (x fastest).  dir = -1 matches
+ * fft3_fw (e^{-i}, scaled 1/N), dir = +1 matches fft3_bw (e^{+i}, unscaled),
+ * m_fft3d_20150826.f90:1598-1720. */
+void ro_dft3(int nx, int ny, int nz, int dir, const ro_z *in, ro_z *out);
+
+/* calc_InterStateMatrix, chiqw/chiqw_sub.F:97-205, restated in G space:
+ * M(G) = sum_{G1 in kg_k} C_k(G1) conj(grid_kq[wrap(G1 + G + shift)]), where
+ * grid_kq holds C_kq scattered at wrap(G2).  Same aliasing as the FFT route. */
+void ro_pairdens(int ng_k, const int32_t *kg_k, const ro_z *c_k, int ng_kq,
+                 const int32_t *kg_kq, const ro_z *c_kq, int nwx, int nwy, int nwz,
+                 int ng_eps, const int32_t *lg, const int32_t *shift_g, ro_z *m);
+
+/* calc_VMab (chiqw_sub.F:207-232) and calc_VMaa (:234-262); vm[3] */
+void ro_vmab(int ng, const int32_t *kg, const ro_z *c_k, const ro_z *c_kq,
+             const double *b1, const double *b2, const double *b3, ro_z *vm);
+void ro_vmaa(int ng, const int32_t *kg, const ro_z *c_k, const ro_z *c_kq,
+             const double *b1, const double *b2, const double *b3, const double *skt,
+             ro_z *vm);
+
+/* In-place inverse of a complex n*n matrix (column-major) by LU with partial
+ * pivoting, the algorithm of LAPACK zgetrf+zgetri called at invZGE.F90:1-23.
+ * Returns 0, or k+1 if U(k,k) is exactly zero (as zgetrf's info). */
+int ro_zinv(int n, ro_z *a);
+
+/* calc_eps_rpa (chiqw_sub.F:386-457): eps = delta - 2*chi*4pi/(|q+G||q+G'|),
+ * inverted.  chi (n*n, column-major) is read; epsinv written. */
+int ro_eps_rpa(int ng, const int32_t *lg, const double *q, const double *b1,
+               const double *b2, const double *b3, const ro_z *chi, ro_z *epsinv);
+/* calc_eps_rpa_q_0 (chiqw_sub.F:459-568) with head/wings and Drude term. */
+int ro_eps_rpa_q0(int ng, const int32_t *lg, const double *b1, const double *b2,
+                  const double *b3, int g0, double wd, double delt, ro_z w,
+                  const ro_z *chi, ro_z *epsinv);
+
+/*
+ * One q != 0 of chiqw (chiqw.F:1257-1476), in the reference's loop order:
+ * OpenMP over band pairs, then k; per pair the tetrahedron weights for
+ * dsgn = +1 and -1, then pc0d(ig,jg,ie) += x(ik,ie) conj(M(ik,ig)) M(ik,jg)
+ * / NTK * (1 - P_occ P_vir).  chi is divided by the cell volume at the end.
+ *
+ * Wavefunctions are given already unfolded to the full mesh (make_C0 applied):
+ * c_occ[(ib*ntk + ik)*ntg + g], c_vir likewise; ng0[ik], kg0[(ik*ntg+g)*3+d].
+ * e_occ[ib*ntk + ik], e_vir likewise (eigenvalues of numirr(ik)).
+ * p_occ[ib*ntk + ik], p_vir likewise (cRPA projector weights; zeros = fRPA).
+ * pair_occ/pair_vir: the band pairs (0-based into the occ/vir arrays).
+ * ikq[ik], shift[3*ik]: k+q map from search_kq.
+ * chi[(ie*ng + jg)*ng + ig] = chiqw(ig,jg,ie).
+ * If m_out != NULL it receives M for pair 0: m_out[ik*ng + ig].
+ * If x_out != NULL it receives x for pair 0: x_out[ik*nen + ie].
+ * nthreads <= 0 uses the OpenMP default.  Returns 0.
+ */
+int ro_chiqw_q(int ntk, int ntg, const int32_t *ng0, const int32_t *kg0,
+               int nocc, const ro_z *c_occ, int nvir, const ro_z *c_vir,
+               const double *e_occ, const double *e_vir, const ro_z *p_occ,
+               const ro_z *p_vir, int npair, const int32_t *pair_occ,
+               const int32_t *pair_vir, const int32_t *ikq, const int32_t *shift,
+               int nwx, int nwy, int nwz, int ng, const int32_t *lg, int nkb1,
+               int nkb2, int nkb3, const int32_t *imt, const int32_t *index_kpt,
+               double ef, double delt, double dmna, double dmnr, int nen,
+               const ro_z *em, double volume, ro_z *chi, ro_z *m_out, ro_z *x_out,
+               int nthreads);
+
+/*
+ * The q = 0 branch (chiqw.F:1557-1985) for gapped, non-intraband pairs: three
+ * accumulators (q -> 0 along b1, b2, b3) whose G = 0 entry of M is replaced by
+ * -i <vir|p|occ>/(e_vir - e_occ) (calc_VMab; zero when |de| <= delt).
+ * chi3[((ix*nen + ie)*ng + jg)*ng + ig].  g0 = 0-based index of G = 0.
+ */
+int ro_chiqw_q0(int ntk, int ntg, const int32_t *ng0, const int32_t *kg0,
+                int nocc, const ro_z *c_occ, int nvir, const ro_z *c_vir,
+                const double *e_occ, const double *e_vir, const ro_z *p_occ,
+                const ro_z *p_vir, int npair, const int32_t *pair_occ,
+                const int32_t *pair_vir, int nwx, int nwy, int nwz, int ng,
+                const int32_t *lg, int g0, const double *b1, const double *b2,
+                const double *b3, int nkb1, int nkb2, int nkb3, const int32_t *imt,
+                const int32_t *index_kpt, double ef, double delt, double dmna,
+                double dmnr, int nen, const ro_z *em, double volume, ro_z *chi3,
+                int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

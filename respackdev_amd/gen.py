@@ -1,0 +1,199 @@
+"""Seeded synthetic inputs in RESPACK's ``dir-wfn`` / ``dir-wan`` file formats.
+
+The files are what ``util/qe2respack/qe2respack.f90:118-194`` and
+``util/xtapp2respack/wfn2respack.F90:103-199`` write, so ``calc_chiqw`` (the
+reference's or this package's) reads them unchanged:
+
+* ``dat.bandcalc``   Ecut_psi (Ry), E_F (Ha), E_tot
+* ``dat.lattice``    a1, a2, a3 (bohr)
+* ``dat.symmetry``   nsym, nnp, then per op 9 ints rg and 3 ints pg
+* ``dat.sample-k``   Nk_irr, then k (fractional)
+* ``dat.nkm``        plane waves per k
+* ``dat.kg``         per k: NG, then G triplets
+* ``dat.eigenvalue`` NTB, then Nk_irr*NTB eigenvalues (Ha), k outer
+* ``dat.wfn``        Fortran unformatted: ncomp=1, then per k per band one
+                     record of NG complex(8)
+
+System: simple cubic cell (edge ``a`` bohr), identity symmetry only, a
+Gamma-centred ``nk^3`` mesh folded to (-1/2, 1/2] of which only the
+time-reversal-irreducible half is listed (the reference regenerates -k as the
+conjugate, ``chiqw.F:458-473``).  Coefficients are complex Gaussians scaled by
+exp(-|k+G|^2/8) and orthonormalised per k; eigenvalues are uniform in
+[-10, 0] eV (occupied) and [1, 40] eV (empty), sorted per k; E_F = 0.5 eV.
+Random numbers come from numpy's counter-based Philox generator.
+"""
+from __future__ import annotations
+
+import os
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+AU_EV = 27.21151  # chiqw/config.h:19
+
+
+@dataclass
+class System:
+    a: float
+    nk: int
+    nocc: int
+    nvir: int
+    ecut_psi: float
+    seed: int
+    ef: float                      # Ha
+    lattice: np.ndarray            # (3,3) rows a1,a2,a3
+    bvec: np.ndarray               # (3,3) rows b1,b2,b3
+    ski: np.ndarray                # (nk_irr,3)
+    kg: list = field(default_factory=list)     # per k: (NG,3) int32
+    coef: list = field(default_factory=list)   # per k: (NTB,NG) complex128
+    eig: np.ndarray | None = None  # (nk_irr, NTB) Ha
+
+    @property
+    def ntb(self) -> int:
+        return self.nocc + self.nvir
+
+    @property
+    def volume(self) -> float:
+        return float(abs(np.linalg.det(self.lattice)))
+
+
+def _mesh_order(n: int) -> list[int]:
+    """0, 1, .., floor(n/2), -1, .., so that +k precedes -k."""
+    pos = list(range(0, n // 2 + 1))
+    neg = [-i for i in range(1, n - len(pos) + 1)]
+    return pos + neg
+
+
+def _fold(x: float) -> float:
+    # k = n/N folded into (-1/2, 1/2]
+    while x > 0.5 + 1e-9:
+        x -= 1.0
+    while x <= -0.5 + 1e-9:
+        x += 1.0
+    return x
+
+
+def irreducible_kmesh(nk: int) -> np.ndarray:
+    order = _mesh_order(nk)
+    ks: list[tuple[float, float, float]] = []
+    seen: set[tuple[int, int, int]] = set()
+    for n1 in order:
+        for n2 in order:
+            for n3 in order:
+                k = (_fold(n1 / nk), _fold(n2 / nk), _fold(n3 / nk))
+                key = tuple(int(round(v * nk)) for v in k)
+                mk = tuple(int(round(_fold(-v) * nk)) for v in k)
+                if key in seen or mk in seen:
+                    continue
+                seen.add(key)
+                ks.append(k)
+    return np.array(ks, dtype=np.float64)
+
+
+def _sphere(kf: np.ndarray, bvec: np.ndarray, ecut: float, gmax: int) -> tuple[np.ndarray, np.ndarray]:
+    r = np.arange(-gmax, gmax + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)  # g1 outer, g3 inner
+    kc = (kf[None, :] + g) @ bvec
+    e = np.einsum("ij,ij->i", kc, kc)
+    sel = e <= ecut
+    return g[sel].astype(np.int32), e[sel]
+
+
+def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
+                ecut_psi: float = 13.9, seed: int = 1000, ef_ev: float = 0.5) -> System:
+    lat = np.eye(3) * a
+    bvec = 2.0 * np.pi * np.linalg.inv(lat).T  # rows b1, b2, b3
+    ski = irreducible_kmesh(nk)
+    gmax = int(np.ceil(np.sqrt(ecut_psi) / (2 * np.pi / a))) + 2
+    # keep Ecut away from every |k+G|^2 so the reference's re-derived cutoff
+    # (max|k+G|^2 + 1e-8, chiqw.F:349) selects the same spheres
+    allk = np.array([[_fold(n1 / nk), _fold(n2 / nk), _fold(n3 / nk)]
+                     for n1 in range(nk) for n2 in range(nk) for n3 in range(nk)])
+    r = np.arange(-gmax, gmax + 1)
+    g = np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    ek = np.concatenate([np.einsum("ij,ij->i", (k + g) @ bvec, (k + g) @ bvec) for k in allk])
+    for _ in range(100):
+        if np.all(np.abs(ek - ecut_psi) > 1e-5):
+            break
+        ecut_psi += 3e-5
+    rng = np.random.Generator(np.random.Philox(seed))
+    sysm = System(a=a, nk=nk, nocc=nocc, nvir=nvir, ecut_psi=ecut_psi, seed=seed,
+                  ef=ef_ev / AU_EV, lattice=lat, bvec=bvec, ski=ski)
+    ntb = nocc + nvir
+    eig = np.empty((len(ski), ntb))
+    for ik, kf in enumerate(ski):
+        kg, e2 = _sphere(kf, bvec, ecut_psi, gmax)
+        ng = len(kg)
+        if ng < ntb:
+            raise ValueError(f"{ng} plane waves cannot hold {ntb} bands")
+        z = rng.standard_normal((ng, ntb)) + 1j * rng.standard_normal((ng, ntb))
+        z *= np.exp(-e2 / 8.0)[:, None]
+        q, _ = np.linalg.qr(z)
+        sysm.kg.append(kg)
+        sysm.coef.append(np.ascontiguousarray(q.T))
+        eo = np.sort(rng.uniform(-10.0, 0.0, nocc))
+        ev = np.sort(rng.uniform(1.0, 40.0, nvir))
+        eig[ik] = np.concatenate([eo, ev]) / AU_EV
+    sysm.eig = eig
+    return sysm
+
+
+def _fortran_record(fh, payload: bytes) -> None:
+    n = struct.pack("<i", len(payload))
+    fh.write(n)
+    fh.write(payload)
+    fh.write(n)
+
+
+def write_dir_wfn(sysm: System, root: str) -> str:
+    d = os.path.join(root, "dir-wfn")
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "dat.bandcalc"), "w") as f:
+        f.write(f" {sysm.ecut_psi:.16E}\n {sysm.ef:.16E}\n {0.0:.16E}\n")
+    with open(os.path.join(d, "dat.lattice"), "w") as f:
+        for row in sysm.lattice:
+            f.write(" " + " ".join(f"{v:.16E}" for v in row) + "\n")
+    with open(os.path.join(d, "dat.symmetry"), "w") as f:
+        f.write(" 1\n 1\n")
+        f.write("    1    0    0    0    1    0    0    0    1\n")
+        f.write("    0    0    0\n")
+    with open(os.path.join(d, "dat.sample-k"), "w") as f:
+        f.write(f" {len(sysm.ski)}\n")
+        for k in sysm.ski:
+            f.write(" " + " ".join(f"{v:.16E}" for v in k) + "\n")
+    with open(os.path.join(d, "dat.nkm"), "w") as f:
+        for kg in sysm.kg:
+            f.write(f" {len(kg)}\n")
+    with open(os.path.join(d, "dat.kg"), "w") as f:
+        for kg in sysm.kg:
+            f.write(f" {len(kg)}\n")
+            for g in kg:
+                f.write(f" {g[0]:6d} {g[1]:6d} {g[2]:6d}\n")
+    with open(os.path.join(d, "dat.eigenvalue"), "w") as f:
+        f.write(f" {sysm.ntb}\n")
+        for row in sysm.eig:
+            for v in row:
+                f.write(f" {v:.16E}\n")
+    with open(os.path.join(d, "dat.wfn"), "wb") as f:
+        _fortran_record(f, struct.pack("<i", 1))
+        for c in sysm.coef:
+            for band in c:
+                _fortran_record(f, np.ascontiguousarray(band, dtype=np.complex128).tobytes())
+    return d
+
+
+# Named workloads (sizes follow the reference's own derivation rules: the FFT
+# grid comes from algn235(int(2 sqrt(Ecut)/d)+1), the frequency grid from
+# gen_grid with E_max = band spread).
+WORKLOADS = {
+    # 4^3 k, 8 occ + 24 empty, ~300 PW (9^3 grid), NG ~100, q = (0,0,1/4)
+    "c1_plain_fp64": dict(a=7.0, nk=4, nocc=8, nvir=24, ecut_psi=13.9, seed=1000,
+                          ecut_eps=6.9, nen=16, q=(0.0, 0.0, 0.25)),
+    # 6^3 k, 20 occ + 180 empty, ~1000 PW (15^3 grid), NG ~600, 32 frequencies
+    "c2_precision": dict(a=7.0, nk=6, nocc=20, nvir=180, ecut_psi=30.97, seed=1001,
+                         ecut_eps=22.2, nen=32, q=(0.0, 0.0, 1.0 / 6.0)),
+    # small parity case used by tests
+    "tiny": dict(a=7.0, nk=2, nocc=2, nvir=3, ecut_psi=4.5, seed=7,
+                 ecut_eps=2.0, nen=14, q=(0.0, 0.0, 0.5)),
+}

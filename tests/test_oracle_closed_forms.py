@@ -1,0 +1,101 @@
+"""Pins the oracle where the reference cannot be compiled here (its LAPACK
+inversion and its MPI driver with the inline chi accumulation, chiqw.F:1397-1410)
+through closed forms, library routines and invariants."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import ptr
+
+AU = 27.21151
+B7 = np.eye(3) * (2 * np.pi / 7.0)
+
+
+def test_zinv_matches_numpy_inverse():
+    rng = np.random.default_rng(0)
+    for n in (1, 2, 7, 40, 101):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) + 3 * np.eye(n)
+        np.testing.assert_allclose(oracle.zinv(a), np.linalg.inv(a), rtol=1e-10, atol=1e-12)
+
+
+def test_zinv_pivots_and_detects_singularity():
+    a = np.array([[0, 1], [1, 0]], np.complex128)  # needs a row swap
+    np.testing.assert_allclose(oracle.zinv(a), a)
+    with pytest.raises(np.linalg.LinAlgError):
+        oracle.zinv(np.zeros((3, 3), np.complex128))
+
+
+def test_eps_rpa_formula_and_inverse():
+    rng = np.random.default_rng(1)
+    lg = oracle.make_kg0(B7, 3.0, (0, 0, 0.25))
+    n = len(lg)
+    chi = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 1e-3
+    q = np.array([0, 0, 0.25])
+    out = np.zeros((n, n), np.complex128, order="F")
+    chif = np.asfortranarray(chi)
+    b = [np.ascontiguousarray(B7[i]) for i in range(3)]
+    assert oracle.lib().ro_eps_rpa(n, ptr(lg), ptr(q), ptr(b[0]), ptr(b[1]), ptr(b[2]),
+                                   chif.ctypes.data, out.ctypes.data) == 0
+    qg = np.linalg.norm((q + lg) @ B7, axis=1)
+    eps = np.eye(n) - 2 * chi * 4 * np.pi / np.outer(qg, qg)
+    np.testing.assert_allclose(out, np.linalg.inv(eps), rtol=1e-11, atol=1e-13)
+
+
+def test_flat_band_tetrahedron_closed_form():
+    # all four vertices degenerate -> x(k,w) = 1/(w - D + i d) - 1/(w + D - i d)
+    nkb = (4, 4, 4)
+    idx = oracle.mkidx(nkb, B7)
+    em = oracle.gen_grid(16, 50 / AU)
+    e1 = np.full(64, -3.0 / AU)
+    e2 = np.full(64, 5.0 / AU)
+    d = 0.1 / AU
+    x = oracle.ttrhdrn_chi0(nkb, idx, e1, e2, 0.5 / AU, d, 1.0, em)
+    x = oracle.ttrhdrn_chi0(nkb, idx, e1, e2, 0.5 / AU, d, -1.0, em, xo=x)
+    dl = (5.0 + 3.0) / AU
+    want = 1.0 / (em - dl + 1j * d) - 1.0 / (em + dl - 1j * d)
+    np.testing.assert_allclose(x, np.broadcast_to(want, x.shape), rtol=1e-12)
+
+
+def test_linear_band_tetrahedron_against_quadrature():
+    # non-degenerate vertices: the tetrahedron integral of the linearly
+    # interpolated 1/(w + g(r)), checked against brute-force quadrature on one
+    # tetrahedron (closed forms vs numerical integration)
+    rng = np.random.default_rng(4)
+    g = -(rng.uniform(0.1, 0.6, 4)) + 0.08j
+    em = np.array([0.0, 0.2, 0.35, 0.9], np.complex128)
+    fk = np.ones(4, np.complex128)
+    xo = np.zeros((4, len(em)), np.complex128)
+    oracle.lib().ro_ttrhdrn_calc(1e-3, 1e-3, ptr(fk), ptr(g), len(em), ptr(em), ptr(xo))
+    # quadrature over the unit simplex (volume 1/6): weight of vertex v is its
+    # barycentric coordinate; result scaled by 6 * (1/24 per-vertex normalisation)
+    n = 2000000
+    u = rng.random((n, 3))
+    u.sort(axis=1)
+    bary = np.stack([u[:, 0], u[:, 1] - u[:, 0], u[:, 2] - u[:, 1], 1 - u[:, 2]], 1)
+    gg = bary @ g
+    for ie, w in enumerate(em):
+        f = 1.0 / (w + gg)
+        q = (bary * f[:, None]).mean(0) / 6.0  # integral over volume-1/6 simplex / (1/6) * (1/6)
+        np.testing.assert_allclose(xo[:, ie], q, rtol=1.5e-3)
+
+
+def test_pairdens_single_plane_wave_closed_form():
+    # psi_k = e^{i(k+G1)r}, psi_kq = e^{i(k+q+G2)r}: M(G) = delta(G2 = G1 + G + dG)
+    kk = (0.0, 0.0, 0.25)
+    kg = oracle.make_kg0(B7, 4.0, kk)
+    lg = oracle.make_kg0(B7, 2.0, (0, 0, 0.25))
+    nw = oracle.lib().ro_algn235(int(2 * np.sqrt(4.0) / B7[0, 0]) + 1)
+    i1 = 3
+    ck = np.zeros(len(kg), np.complex128); ck[i1] = 1.0
+    for j in (1, 5):
+        target = kg[i1] + lg[j] + np.array([0, 0, 1])
+        hit = np.where((kg == target).all(1))[0]
+        if len(hit) == 0:
+            continue
+        ckq = np.zeros(len(kg), np.complex128); ckq[hit[0]] = 1.0
+        m = np.zeros(len(lg), np.complex128)
+        sg = np.array([0, 0, 1], np.int32)
+        oracle.lib().ro_pairdens(len(kg), ptr(kg), ptr(ck), len(kg), ptr(kg), ptr(ckq),
+                                 nw, nw, nw, len(lg), ptr(lg), ptr(sg), ptr(m))
+        want = np.zeros(len(lg)); want[j] = 1.0
+        np.testing.assert_allclose(m, want, atol=1e-7)
