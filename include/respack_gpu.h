@@ -1,0 +1,134 @@
+/*
+ * respack_gpu.h -- C ABI of librespack_gpu.so, the MI355X (gfx950) hot path of
+ * RESPACK's cRPA polarisation (src/chiqw) and Wannier-basis interaction
+ * integrals (src/calc_int).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a non-zero status otherwise;
+ *     rp_last_error() then returns the message (thread-local).  There is no
+ *     silent fallback: a missing device or an unsupported size is an error.
+ *   - Arrays are plain host pointers.  "complex" arrays are interleaved
+ *     (re, im) doubles -- the bytes of Fortran complex(8) / C double _Complex.
+ *     Index order is given per argument; "Fortran order (i,j,k)" means i fastest.
+ *   - G vectors and k/q points use the reference's conventions: integer
+ *     triplets g(3), fractional coordinates in units of the reciprocal vectors.
+ *   - The library owns all device memory behind the handle; no host pointer is
+ *     kept after a call returns.  One handle = one HIP device + one stream;
+ *     calls on one handle must not overlap.
+ */
+#ifndef RESPACK_GPU_H
+#define RESPACK_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rp_ctx* rp_handle;
+
+/* Precision of the chi contraction (the rest of the path is FP64). */
+enum {
+  RP_PREC_FP64 = 0,   /* FP64 MFMA, FP64 accumulation                            */
+  RP_PREC_FP32 = 1,   /* FP32 MFMA, FP32 accumulation per K-chunk, FP64 merge    */
+  RP_PREC_BF16X3 = 2  /* split-BF16 3-term products on MFMA, FP32 accumulation    */
+};
+
+/* Lifecycle.  Replaces the reference's MPI_INIT/OpenMP setup
+ * (chiqw.F:8-12); device = HIP ordinal. */
+int rp_init(int32_t device, rp_handle* out);
+int rp_finalize(rp_handle h);
+const char* rp_last_error(void);
+
+/* ------------------------------------------------------------------------
+ * Setup helpers (host code; restate the reference routines cited).
+ * ------------------------------------------------------------------------ */
+/* est_nwx2.f90:1-13 */
+int32_t rp_algn235(int32_t n);
+/* gen_grid.f90:1-39; em = nt complex */
+int rp_gen_grid(int32_t nt, double emax, double* em);
+/* chiqw_sub.F:773-812; b = b1,b2,b3 (9 doubles); kg = 3*ntg ints; *ng = count
+ * (kg written only while count <= ntg) */
+int rp_make_kg0(int32_t ntg, const double* b, double gcut, const double* q,
+                int32_t* kg, int32_t* ng);
+/* m_tetrahedron_20170325.F:505-584; idx = 4*6*n1*n2*n3 zero-based grid
+ * indices (ix + n1*(iy + n2*iz)) */
+int rp_ttrhdrn_mkidx(int32_t n1, int32_t n2, int32_t n3, const double* b, int32_t* idx);
+
+/* ------------------------------------------------------------------------
+ * chiqw drop-in (replaces PROGRAM DIELECTRIC's hot path, chiqw.F:1250-1555
+ * and the q = 0 branch chiqw.F:1557-2085).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  int32_t n_calc_band;          /* N_CALC_BAND (0 -> NTB), m_rdinput.f90:54     */
+  int32_t num_freq_grid;        /* Num_freq_grid (70)                           */
+  int32_t flg_crpa;             /* 0 fRPA, 1 band-cRPA                          */
+  int32_t precision;            /* RP_PREC_* for the contraction                */
+  double ecut_for_eps;          /* Ry; 0 -> Ecut_for_psi/10 (chiqw.F:350-352)   */
+  double shift_ef;              /* eV (chiqw.F:67)                              */
+  double max_excitation_energy; /* eV (200)                                     */
+  double green_func_delt;       /* eV (0.1)                                     */
+  double ttrhdrn_dmna;          /* used in Hartree, as the reference does       */
+  double ttrhdrn_dmnr;
+} rp_chiqw_params;
+
+/* Opens a chiqw system from the contents of dir-wfn (chiqw.F:88-329):
+ *   lattice   9 doubles: a1, a2, a3 (bohr)
+ *   rg        9*nsym ints, Fortran order rg(i,j,iop); pg 3*nsym ints
+ *   ski       3*nk_irr doubles; ngi nk_irr ints
+ *   kgi       3*ntg*nk_irr ints, Fortran order KGI(d,ig,ik)
+ *   eig       ntb*nk_irr doubles (Ha), Fortran order E_EIGI(ib,ik)
+ *   wfn       complex, Fortran order C(ig,ib,ik), ntg*ntb*nk_irr
+ *   ef        Fermi energy (Ha) from dat.bandcalc
+ * Unfolds the k mesh (symmetry + time reversal), classifies the bands,
+ * builds the frequency grid and uploads the band grids u_nk(r) to HBM. */
+int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
+                  const int32_t* rg, const int32_t* pg, int32_t nk_irr,
+                  const double* ski, const int32_t* ngi, int32_t ntg,
+                  const int32_t* kgi, int32_t ntb, const double* eig,
+                  const double* wfn, double ef, const rp_chiqw_params* prm);
+
+/* Band-cRPA projector (chiqw.F:870-934): per full-mesh k (NTK of them, in the
+ * order rp_chiqw_info reports) the window offset ns[k] and width nb[k]; umat
+ * complex, Fortran order UNT(jb, jw, k) with jb < mb (the max of nb). */
+int rp_chiqw_set_crpa(rp_handle h, int32_t ntk, const int32_t* ns, const int32_t* nb,
+                      int32_t mb, int32_t nwf, const double* umat);
+
+/* Sizes of the opened system.  iout (16 ints): NTK, nkb1, nkb2, nkb3, nwx,
+ * nwy, nwz, nen, N_occ, N_vir, N_docc, NTB, Nk_irr, ntg, 0, 0.
+ * dout (8 doubles): Ecut_for_psi (Ry), Ecut_for_eps (Ry), volume, emax (Ha),
+ * E_F (Ha, shifted), delt (Ha), 0, 0.  em: nen complex (may be NULL). */
+int rp_chiqw_info(rp_handle h, int32_t* iout, double* dout, double* em);
+/* Full-mesh k points (3*NTK doubles, SK0 order) and their irreducible index
+ * (0-based) and time-reversal flag (+1/-1). */
+int rp_chiqw_kmesh(rp_handle h, double* sk0, int32_t* numirr, int32_t* trs);
+
+/* Per-q stages.  rp_chiqw_prepare_q sets q (fractional, folded as the
+ * reference's search_list) and returns NG (size of the chi G sphere), and the
+ * sphere itself in lg (3*NG ints, may be NULL; size it with a first call). */
+int rp_chiqw_prepare_q(rp_handle h, const double* q, int32_t* ng, int32_t* lg, int32_t lg_cap);
+/* Pair densities M_(pair,k)(G) of calc_InterStateMatrix (chiqw_sub.F:97-205)
+ * for every pair and k into HBM.  m_out (optional, may be NULL): complex64
+ * (interleaved floats) [K][NG] with K = pair*NTK + k; at q = 0, [3][K][NG]
+ * (the G = 0 entry replaced per direction as chiqw.F:1787-1805). */
+int rp_pairdens_batch(rp_handle h, float* m_out);
+/* Tetrahedron weights (ttrhdrn_chi0 for dsgn = +1 and -1, chiqw.F:1350-1395)
+ * times (1 - P_occ P_vir)/NTK into HBM.  d_out (optional): complex [nen][K]. */
+int rp_tetra_weights_batch(rp_handle h, double* d_out);
+/* chi(G,G',w) = sum_K conj(M_K(G)) d_K(w) M_K(G') / volume (chiqw.F:1397-1476)
+ * in the given RP_PREC_*.  chi_out (optional): complex, Fortran order
+ * chiqw(ig,jg,ie) (times 3 directions at q = 0). */
+int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out);
+/* Dyson step calc_eps_rpa / calc_eps_rpa_q_0 (chiqw_sub.F:386-568) on the GPU
+ * in FP64: eps^-1, Fortran order (ig,jg,ie) (times 3 at q = 0). */
+int rp_dyson(rp_handle h, double* epsinv_out);
+/* All four stages for one q.  timings (optional, 6 doubles, seconds): pair
+ * densities, weights, contraction, Dyson, total device time, host prepare. */
+int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_out,
+               double* chi_out, double* timings);
+/* Device-resident number of pairs (after the Max_excitation_energy filter). */
+int rp_chiqw_npair(rp_handle h, int32_t* npair);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,183 @@
+"""The oracle's chiqw run for one q (TEST INFRASTRUCTURE, the checker).
+
+Builds every setup array with the oracle's own restated routines (unfolding,
+G spheres, band classification, frequency grid: chiqw.F:88-1218), then calls
+ro_chiqw_q / ro_chiqw_q0 and ro_eps_rpa / ro_eps_rpa_q0.  Takes any object with
+the attributes of respackdev_amd.chiqw.WfnData (the dir-wfn contents).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import lib, ptr
+
+AU = 27.21151
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def setup(w, ecut_for_eps=0.0, num_freq_grid=70, n_calc_band=0, shift_ef=0.0,
+          max_excitation_energy=200.0, green_func_delt=0.1, dmna=1e-3, dmnr=1e-3):
+    L = lib()
+    a = np.asarray(w.lattice, np.float64)
+    b1 = np.cross(a[1], a[2]); vol = float(a[0] @ b1)
+    b = np.stack([b1, np.cross(a[2], a[0]), np.cross(a[0], a[1])]) * (2 * np.acos(-1.0) / vol)
+    b = np.ascontiguousarray(b)
+    nk = len(w.ski)
+    ntg = w.kgi.shape[1]
+    ecut_psi = 0.0
+    for ik in range(nk):
+        g = w.kgi[ik, : w.ngi[ik]]
+        kc = (w.ski[ik] + g) @ b
+        ecut_psi = max(ecut_psi, float(np.max(np.einsum("ij,ij->i", kc, kc))))
+    ecut_psi += 1e-8
+    ecut_eps = ecut_psi / 10.0 if ecut_for_eps == 0.0 else ecut_for_eps
+    qwf = 2.0 * np.sqrt(ecut_psi)
+    nw = []
+    for d in range(3):
+        h = a[d] / np.linalg.norm(a[d])
+        nw.append(L.ro_algn235(int(qwf / abs(b[d] @ h)) + 1))
+    nsym = len(w.rg)
+    n = 2 * nk * nsym
+    sk0 = np.zeros((n, 3)); numirr = np.zeros(n, np.int32); numrot = np.zeros(n, np.int32)
+    trs = np.zeros(n, np.int32); rw = np.zeros((n, 3), np.int32)
+    ski = np.ascontiguousarray(w.ski, np.float64)
+    rg = _i32(w.rg)
+    ntk = L.ro_unfold_k(nk, nsym, ptr(ski), ptr(rg), ptr(sk0), ptr(numirr), ptr(numrot),
+                        ptr(trs), ptr(rw))
+    sk0 = np.ascontiguousarray(sk0[:ntk]); numirr = numirr[:ntk]; numrot = numrot[:ntk]
+    trs = trs[:ntk]; rw = np.ascontiguousarray(rw[:ntk])
+    nkb = np.zeros(3, np.int32)
+    L.ro_est_nkbi(ntk, ptr(sk0), ptr(nkb))
+    index_kpt = np.full(int(np.prod(nkb)), -1, np.int32)
+    L.ro_make_index_kpt(ntk, int(nkb[0]), int(nkb[1]), int(nkb[2]), ptr(sk0), ptr(index_kpt))
+    imt = np.zeros((6 * ntk, 4), np.int32)
+    L.ro_ttrhdrn_mkidx(int(nkb[0]), int(nkb[1]), int(nkb[2]), ptr(b[0]), ptr(b[1]), ptr(b[2]), ptr(imt))
+    # G lists per full k and unfolded wavefunctions (make_C0)
+    ntb = w.eig.shape[1]
+    nb = ntb if n_calc_band == 0 else n_calc_band
+    ng0 = np.zeros(ntk, np.int32)
+    kg0 = np.zeros((ntk, ntg, 3), np.int32)
+    Ls = [int(np.abs(w.kgi[:, :, d]).max()) + 1 for d in range(3)]
+    coef = np.zeros((nb, ntk, ntg), np.complex128)
+    tmp = np.zeros((200000, 3), np.int32)
+    for k in range(ntk):
+        ik, op = int(numirr[k]), int(numrot[k])
+        r = np.asarray(w.rg[op]).reshape(3, 3).T  # r[i,j] = rg(i,j)
+        kk = r @ w.ski[ik] + rw[k]
+        m = L.ro_make_kg0(200000, ptr(b[0]), ptr(b[1]), ptr(b[2]), ecut_psi, kk[0], kk[1], kk[2], ptr(tmp))
+        assert m == w.ngi[ik], "NG_for_psi should be NGI(ik)"
+        ng0[k] = m
+        kg0[k, :m] = tmp[:m] * (1 if trs[k] == 1 else -1)
+        pack = np.zeros((2 * Ls[2] + 1, 2 * Ls[1] + 1, 2 * Ls[0] + 1), np.int32)
+        gi = w.kgi[ik, : w.ngi[ik]]
+        pack[gi[:, 2] + Ls[2], gi[:, 1] + Ls[1], gi[:, 0] + Ls[0]] = np.arange(1, len(gi) + 1)
+        rinv = np.ascontiguousarray(np.linalg.inv(r).T.reshape(-1))  # rginv(i,j) at [i + 3j]
+        kgk = np.ascontiguousarray(kg0[k, :m])
+        rwk = np.ascontiguousarray(rw[k])
+        pgk = _i32(w.pg[op])
+        for ib in range(nb):
+            occ = np.ascontiguousarray(w.wfn[ik, ib], np.complex128)
+            out = np.zeros(m, np.complex128)
+            L.ro_make_c0(int(trs[k]), m, ptr(kgk), ptr(rwk), ptr(rinv), ptr(pgk), int(w.nnp),
+                         Ls[0], Ls[1], Ls[2], ptr(pack), ptr(occ), ptr(out))
+            coef[ib, k, :m] = out
+    # band classification
+    ef = w.ef + shift_ef / AU
+    eig = np.ascontiguousarray(w.eig, np.float64)
+    win = np.zeros(ntb, np.int32); tindx = np.zeros(ntb, np.int32)
+    eave = np.zeros(ntb); bmax = np.zeros(ntb); bmin = np.zeros(ntb)
+    L.ro_judge_fermi(nk, ntb, ptr(eig), ef, ptr(win), ptr(tindx), ptr(eave), ptr(bmax), ptr(bmin))
+    n_docc = int(np.sum(win[:nb] == 0)); n_pocc = int(np.sum(win[:nb] == 1))
+    n_uocc = int(np.sum(win[:nb] == 2))
+    emax = abs(bmax[:nb].max() - bmin[:nb].min())
+    em = np.zeros(num_freq_grid, np.complex128)
+    L.ro_gen_grid(num_freq_grid, emax, ptr(em))
+    eband = np.ascontiguousarray(w.eig[numirr, :nb].T)  # [band][k]
+    return dict(b=b, volume=vol, ecut_psi=ecut_psi, ecut_eps=ecut_eps, nw=nw, ntk=ntk, sk0=sk0,
+                numirr=numirr, trs=trs, nkb=nkb, index_kpt=index_kpt, imt=imt, ng0=ng0, kg0=kg0,
+                coef=coef, win=win, tindx=tindx, eave=eave, n_docc=n_docc, n_pocc=n_pocc,
+                n_occ=n_docc + n_pocc, n_vir=n_uocc + n_pocc, em=em, ef=ef, eband=eband,
+                delt=green_func_delt / AU, delta_ex=max_excitation_energy / AU, dmna=dmna,
+                dmnr=dmnr, ntg=ntg)
+
+
+def run_q(w, q, nthreads=0, prob=None, **kw):
+    """Returns chi [dir][ie][jg][ig], epsinv (same), and pair-0 M and x."""
+    L = lib()
+    s = setup(w, **kw)
+    b = s["b"]
+    ntk, ntg = s["ntk"], s["ntg"]
+    sq = np.zeros(3)
+    qin = np.ascontiguousarray(q, np.float64)
+    L.ro_search_list(ptr(qin), ptr(sq))
+    tmp = np.zeros((200000, 3), np.int32)
+    ng = L.ro_make_kg0(200000, ptr(b[0]), ptr(b[1]), ptr(b[2]), s["ecut_eps"], sq[0], sq[1], sq[2], ptr(tmp))
+    lg = np.ascontiguousarray(tmp[:ng])
+    q0 = bool(np.all(sq == 0.0))
+    ikq = np.zeros(ntk, np.int32)
+    shift = np.zeros((ntk, 3), np.int32)
+    for k in range(ntk):
+        if q0:
+            ikq[k] = k
+            continue
+        sg = np.zeros(3, np.int32)
+        ikq[k] = L.ro_search_kq(ntk, ptr(s["sk0"]), sq[0], sq[1], sq[2], k, ptr(sg))
+        shift[k] = sg
+    pocc, pvir = [], []
+    for jb in range(s["n_occ"]):
+        for ib in range(s["n_vir"]):
+            vb = s["n_docc"] + ib
+            if abs(s["eave"][vb] - s["eave"][jb]) >= s["delta_ex"]:
+                continue
+            if q0 and s["tindx"][vb] == s["tindx"][jb] and s["tindx"][jb] != 0:
+                continue
+            pocc.append(jb)
+            pvir.append(vb)
+    pocc = _i32(pocc); pvir = _i32(pvir)
+    npair = len(pocc)
+    nb = s["coef"].shape[0]
+    coef = np.ascontiguousarray(s["coef"])
+    eband = s["eband"]
+    pband = np.zeros((nb, ntk), np.complex128) if prob is None else np.ascontiguousarray(prob)
+    nen = len(s["em"])
+    nkb = [int(x) for x in s["nkb"]]
+    em = s["em"]
+    m0 = np.zeros((ntk, ng), np.complex128)
+    x0 = np.zeros((ntk, nen), np.complex128)
+    if not q0:
+        chi = np.zeros((1, nen, ng, ng), np.complex128)
+        L.ro_chiqw_q(ntk, ntg, ptr(s["ng0"]), ptr(s["kg0"]), nb, ptr(coef), nb, ptr(coef),
+                     ptr(eband), ptr(eband), ptr(pband), ptr(pband), npair, ptr(pocc), ptr(pvir),
+                     ptr(ikq), ptr(shift), s["nw"][0], s["nw"][1], s["nw"][2], ng, ptr(lg),
+                     nkb[0], nkb[1], nkb[2], ptr(s["imt"]), ptr(s["index_kpt"]), s["ef"], s["delt"],
+                     s["dmna"], s["dmnr"], nen, ptr(em), s["volume"], ptr(chi), ptr(m0), ptr(x0),
+                     nthreads)
+    else:
+        g0 = int(np.where((lg == 0).all(1))[0][0])
+        chi = np.zeros((3, nen, ng, ng), np.complex128)
+        L.ro_chiqw_q0(ntk, ntg, ptr(s["ng0"]), ptr(s["kg0"]), nb, ptr(coef), nb, ptr(coef),
+                      ptr(eband), ptr(eband), ptr(pband), ptr(pband), npair, ptr(pocc), ptr(pvir),
+                      s["nw"][0], s["nw"][1], s["nw"][2], ng, ptr(lg), g0, ptr(b[0]), ptr(b[1]),
+                      ptr(b[2]), nkb[0], nkb[1], nkb[2], ptr(s["imt"]), ptr(s["index_kpt"]),
+                      s["ef"], s["delt"], s["dmna"], s["dmnr"], nen, ptr(em), s["volume"],
+                      ptr(chi), nthreads)
+    eps = np.zeros_like(chi)
+    for ix in range(chi.shape[0]):
+        for ie in range(nen):
+            c = np.ascontiguousarray(chi[ix, ie])  # [jg][ig] == Fortran (ig,jg)
+            e = np.zeros_like(c)
+            if not q0:
+                rc = L.ro_eps_rpa(ng, ptr(lg), ptr(sq), ptr(b[0]), ptr(b[1]), ptr(b[2]), ptr(c), ptr(e))
+            else:
+                w2 = (C.c_double * 2)(em[ie].real, em[ie].imag)
+                rc = L.ro_eps_rpa_q0(ng, ptr(lg), ptr(b[0]), ptr(b[1]), ptr(b[2]), g0, 0.0,
+                                     s["delt"], w2, ptr(c), ptr(e))
+            assert rc == 0
+            eps[ix, ie] = e
+    return dict(chi=chi, epsinv=eps, m0=m0, x0=x0, lg=lg, em=em, npair=npair, pocc=pocc,
+                pvir=pvir, ikq=ikq, shift=shift, setup=s)

@@ -1,0 +1,856 @@
+// C ABI of librespack_gpu: host-side setup of the chiqw path and the per-q
+// orchestration of the gfx950 kernels.  Declarations and argument layouts are
+// in include/respack_gpu.h.
+//
+// The setup restates the reference's host logic (k-mesh unfolding, G spheres,
+// band classification, frequency grid) so that every index and ordering the
+// output depends on is the reference's; all per-q arithmetic of the hot path
+// (pair densities, tetrahedron weights, contraction, Dyson) runs on the GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rp_common.h"
+#include "rp_kernels.h"
+
+using namespace rp;
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+constexpr double kAu = 27.21151;  // chiqw/config.h:19
+
+// ---------------------------------------------------------------- setup math
+int algn235(int n) {  // est_nwx2.f90:1-42
+  for (;; ++n) {
+    int i = n;
+    while (i != 1) {
+      if (i % 2 == 0) i /= 2;
+      else if (i % 3 == 0) i /= 3;
+      else if (i % 5 == 0) i /= 5;
+      else break;
+    }
+    if (i == 1) return n;
+  }
+}
+
+void gen_grid(int nt, double e, double* em) {  // gen_grid.f90:1-39 (em interleaved)
+  const double dlt = 0.05 / kAu, ratio = 0.1, eps = 1.0e-6;
+  if (nt == 1) { em[0] = em[1] = 0.0; return; }
+  int m = (int)(ratio * (double)nt), n = nt - m;
+  double alp = e / dlt, x = 1.5, dl = 1.0;
+  while (dl > eps) {
+    double f = std::pow(x, n - 1) - alp * x + alp - 1.0;
+    double g = (double)(n - 1) * std::pow(x, n - 2) - alp;
+    double x1 = x - f / g;
+    dl = std::fabs(x1 - x);
+    x = x1;
+  }
+  double a = dlt / (x - 1.0), b = std::log(x);
+  for (int i = 1; i <= n; ++i) { em[2 * (i - 1)] = a * std::exp(b * (double)(i - 1)) - a; em[2 * (i - 1) + 1] = 0; }
+  double c = e * std::exp(-std::log(2.0) * (double)n / (double)m), d = std::log(2.0) / (double)m;
+  for (int i = n + 1; i <= n + m; ++i) { em[2 * (i - 1)] = c * std::exp(d * (double)i); em[2 * (i - 1) + 1] = 0; }
+}
+
+// make_KG0 (chiqw_sub.F:773-812): g1 outer, g3 inner over [-100,100]^3; the
+// loop is clipped to a box that contains the sphere, which keeps the order.
+std::vector<int> make_kg0(const double b[3][3], const double a[3][3], double gcut, const double* q) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    double an = std::sqrt(a[d][0] * a[d][0] + a[d][1] * a[d][1] + a[d][2] * a[d][2]);
+    int r = (int)std::ceil(std::sqrt(std::max(gcut, 0.0)) * an / (2.0 * M_PI) + std::fabs(q[d])) + 2;
+    lo[d] = std::max(-100, -r);
+    hi[d] = std::min(100, r);
+  }
+  std::vector<int> kg;
+  for (int i1 = lo[0]; i1 <= hi[0]; ++i1)
+    for (int i2 = lo[1]; i2 <= hi[1]; ++i2)
+      for (int i3 = lo[2]; i3 <= hi[2]; ++i3) {
+        double v[3];
+        for (int d = 0; d < 3; ++d)
+          v[d] = (q[0] + (double)i1) * b[0][d] + (q[1] + (double)i2) * b[1][d] + (q[2] + (double)i3) * b[2][d];
+        double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        if (s <= gcut) { kg.push_back(i1); kg.push_back(i2); kg.push_back(i3); }
+      }
+  return kg;
+}
+
+void kcheck(double* k, int* rw) {  // chiqw_sub.F:668-716
+  const double d = 1.0e-6;
+  for (int i = 0; i < 3; ++i) {
+    if (k[i] > 1.5 + d) { k[i] -= 2.0; rw[i] = -2; }
+    if (k[i] > 0.5 + d) { k[i] -= 1.0; rw[i] = -1; }
+    if (k[i] <= -1.5 + d) { k[i] += 2.0; rw[i] = 2; }
+    if (k[i] <= -0.5 + d) { k[i] += 1.0; rw[i] = 1; }
+  }
+}
+void kcheck_trs(double* k, int* rw) {  // chiqw_sub.F:718-771
+  const double d = -1.0e-6;
+  for (int i = 0; i < 3; ++i) {
+    if (k[i] >= 1.5 + d) { k[i] -= 2.0; rw[i] = -2; }
+    if (k[i] >= 0.5 + d) { k[i] -= 1.0; rw[i] = -1; }
+    if (k[i] < -1.5 + d) { k[i] += 2.0; rw[i] = 2; }
+    if (k[i] < -0.5 + d) { k[i] += 1.0; rw[i] = 1; }
+  }
+}
+void search_list(const double* in, double* sq) {  // chiqw_sub.F:76-95
+  const double d = 1.0e-6;
+  for (int i = 0; i < 3; ++i) sq[i] = in[i];
+  for (int i = 0; i < 2; ++i) {
+    if (sq[i] > 1.5 + d) sq[i] -= 2.0;
+    if (sq[i] > 0.5 + d) sq[i] -= 1.0;
+    if (sq[i] <= -1.5 + d) sq[i] += 2.0;
+    if (sq[i] <= -0.5 + d) sq[i] += 1.0;
+  }
+  if (sq[2] > 1.5 + d) sq[2] -= 2.0;
+  if (sq[2] > 0.5 + d) sq[2] -= 1.0;
+  if (sq[2] <= -0.5 + d) sq[2] += 1.0;
+  if (sq[2] <= -1.5 + d) sq[2] += 2.0;
+}
+int search_kq(int ntk, const double* sk0, const double* q, int ik, int* sg) {  // chiqw_sub.F:1-75
+  const double d = 1.0e-6;
+  double s[3] = {sk0[3 * ik] + q[0], sk0[3 * ik + 1] + q[1], sk0[3 * ik + 2] + q[2]};
+  sg[0] = sg[1] = sg[2] = 0;
+  for (int i = 0; i < 3; ++i) {
+    if (s[i] > 1.5 + d) { s[i] -= 2.0; sg[i] = 2; }
+    if (s[i] > 0.5 + d) { s[i] -= 1.0; sg[i] = 1; }
+    if (s[i] <= -1.5 + d) { s[i] += 2.0; sg[i] = -2; }
+    if (s[i] <= -0.5 + d) { s[i] += 1.0; sg[i] = -1; }
+  }
+  int ikq = -1;
+  for (int jk = 0; jk < ntk; ++jk)
+    if (std::fabs(sk0[3 * jk] - s[0]) < 1e-6 && std::fabs(sk0[3 * jk + 1] - s[1]) < 1e-6 &&
+        std::fabs(sk0[3 * jk + 2] - s[2]) < 1e-6)
+      ikq = jk;
+  return ikq;
+}
+
+void mkidx(int n1, int n2, int n3, const double* b, int* idx) {
+  // libtetrabz corner table (m_tetrahedron_20170325.F:523-584)
+  int ng[3] = {n1, n2, n3};
+  double bv[3][3], b3v[4][3], l[4];
+  for (int d = 0; d < 3; ++d) { bv[0][d] = b[d] / n1; bv[1][d] = b[3 + d] / n2; bv[2][d] = b[6 + d] / n3; }
+  for (int d = 0; d < 3; ++d) {
+    b3v[0][d] = -bv[0][d] + bv[1][d] + bv[2][d];
+    b3v[1][d] = bv[0][d] - bv[1][d] + bv[2][d];
+    b3v[2][d] = bv[0][d] + bv[1][d] - bv[2][d];
+    b3v[3][d] = bv[0][d] + bv[1][d] + bv[2][d];
+  }
+  int itype = 0;
+  for (int i = 0; i < 4; ++i) {
+    l[i] = b3v[i][0] * b3v[i][0] + b3v[i][1] * b3v[i][1] + b3v[i][2] * b3v[i][2];
+    if (l[i] < l[itype]) itype = i;
+  }
+  int v0[4] = {0, 0, 0, 0}, dv[4][4] = {{0}};
+  for (int i = 0; i < 4; ++i) dv[i][i] = 1;
+  v0[itype] = 1;
+  dv[itype][itype] = -1;
+  int vv[6][4][3], it = 0;
+  for (int i1 = 0; i1 < 3; ++i1)
+    for (int i2 = 0; i2 < 3; ++i2) {
+      if (i2 == i1) continue;
+      for (int i3 = 0; i3 < 3; ++i3) {
+        if (i3 == i1 || i3 == i2) continue;
+        for (int d = 0; d < 3; ++d) {
+          vv[it][0][d] = v0[d];
+          vv[it][1][d] = vv[it][0][d] + dv[i1][d];
+          vv[it][2][d] = vv[it][1][d] + dv[i2][d];
+          vv[it][3][d] = vv[it][2][d] + dv[i3][d];
+        }
+        ++it;
+      }
+    }
+  int nt = 0;
+  for (int k3 = 0; k3 < n3; ++k3)
+    for (int k2 = 0; k2 < n2; ++k2)
+      for (int k1 = 0; k1 < n1; ++k1)
+        for (it = 0; it < 6; ++it, ++nt)
+          for (int ii = 0; ii < 4; ++ii) {
+            int kv[3] = {k1 + vv[it][ii][0], k2 + vv[it][ii][1], k3 + vv[it][ii][2]};
+            for (int d = 0; d < 3; ++d) kv[d] = ((kv[d] % ng[d]) + ng[d]) % ng[d];
+            idx[4 * nt + ii] = kv[0] + n1 * kv[1] + n1 * n2 * kv[2];
+          }
+}
+
+inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+struct rp_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  // system
+  bool open = false;
+  double a[3][3], b[3][3], volume = 0;
+  int nk_irr = 0, ntg = 0, ntb = 0, nsym = 0, nnp = 1;
+  double ef = 0, delt = 0, dmna = 0, dmnr = 0, delta_ex = 0, ecut_psi = 0, ecut_eps = 0, emax = 0;
+  int nw[3] = {0, 0, 0};
+  int ntk = 0, nkb[3] = {0, 0, 0};
+  std::vector<double> sk0;
+  std::vector<int> numirr, numrot, trs, rwv, index_kpt, imt, ng0, kg0;
+  int n_calc = 0, n_docc = 0, n_pocc = 0, n_uocc = 0, n_occ = 0, n_vir = 0, nen = 0, flg_crpa = 0;
+  std::vector<int> win, tindx;
+  std::vector<double> eave, em;
+  std::vector<double> eband;   // [band][k]
+  std::vector<double> pband;   // complex [band][k]
+  // device: system
+  DevBuf<double2> d_coef;
+  DevBuf<int> d_kg0, d_ng0, d_imt, d_index;
+  DevBuf<float2> d_ur;
+  DevBuf<double> d_eband, d_bvec;
+  DevBuf<double2> d_pband, d_em;
+  // per q
+  bool qready = false, q0 = false, have_A = false, have_w = false, have_chi = false;
+  double q[3] = {0, 0, 0};
+  int ng = 0, g0 = -1, npair = 0;
+  std::vector<int> lg, ikq, shift, pocc, pvir;
+  std::vector<double> qn;
+  size_t lda = 0, kpad = 0;
+  double prep_seconds = 0;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv;
+  DevBuf<float2> d_A, d_A3;
+  DevBuf<double2> d_w, d_part, d_chi, d_eps, d_rowc, d_colc;
+  DevBuf<double> d_qn, d_wd;
+
+  ~rp_ctx() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int nbatch() const { return q0 ? 3 * nen : nen; }
+};
+
+extern "C" {
+
+const char* rp_last_error(void) { return g_err.c_str(); }
+
+int rp_init(int32_t device, rp_handle* out) {
+  return guarded([&] {
+    RP_CHECK(out, "rp_init: null output");
+    int n = 0;
+    RP_HIP(hipGetDeviceCount(&n));
+    RP_CHECK(n > 0, "rp_init: no HIP device visible");
+    RP_CHECK(device >= 0 && device < n, "rp_init: device ordinal out of range");
+    RP_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    RP_HIP(hipGetDeviceProperties(&prop, device));
+    RP_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,
+             std::string("rp_init: built for gfx950, device is ") + prop.gcnArchName);
+    auto* h = new rp_ctx();
+    h->device = device;
+    RP_HIP(hipStreamCreate(&h->stream));
+    for (auto& e : h->ev) RP_HIP(hipEventCreate(&e));
+    h->d_status.reserve(1);
+    *out = h;
+  });
+}
+
+int rp_finalize(rp_handle h) {
+  return guarded([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    delete h;
+  });
+}
+
+int32_t rp_algn235(int32_t n) { return algn235(n); }
+
+int rp_gen_grid(int32_t nt, double emax, double* em) {
+  return guarded([&] {
+    RP_CHECK(nt >= 1 && em, "rp_gen_grid: bad arguments");
+    gen_grid(nt, emax, em);
+  });
+}
+
+int rp_make_kg0(int32_t ntg, const double* b, double gcut, const double* q, int32_t* kg, int32_t* ng) {
+  return guarded([&] {
+    double bb[3][3], aa[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) bb[i][d] = b[3 * i + d];
+    // real-space vectors for the bounding box: a_i = 2pi (b^-1)^T rows
+    double det = bb[0][0] * (bb[1][1] * bb[2][2] - bb[1][2] * bb[2][1]) -
+                 bb[0][1] * (bb[1][0] * bb[2][2] - bb[1][2] * bb[2][0]) +
+                 bb[0][2] * (bb[1][0] * bb[2][1] - bb[1][1] * bb[2][0]);
+    RP_CHECK(std::fabs(det) > 0, "rp_make_kg0: singular reciprocal basis");
+    double c[3][3];
+    for (int i = 0; i < 3; ++i) {
+      int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+      for (int d = 0; d < 3; ++d) {
+        int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        c[i][d] = bb[i1][d1] * bb[i2][d2] - bb[i1][d2] * bb[i2][d1];
+      }
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) aa[i][d] = 2.0 * M_PI * c[i][d] / det;
+    std::vector<int> v = make_kg0(bb, aa, gcut, q);
+    *ng = (int32_t)(v.size() / 3);
+    if (kg) std::memcpy(kg, v.data(), sizeof(int) * std::min<size_t>(v.size(), (size_t)3 * ntg));
+  });
+}
+
+int rp_ttrhdrn_mkidx(int32_t n1, int32_t n2, int32_t n3, const double* b, int32_t* idx) {
+  return guarded([&] { mkidx(n1, n2, n3, b, idx); });
+}
+
+int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp, const int32_t* rg,
+                  const int32_t* pg, int32_t nk_irr, const double* ski, const int32_t* ngi,
+                  int32_t ntg, const int32_t* kgi, int32_t ntb, const double* eig,
+                  const double* wfn, double ef, const rp_chiqw_params* prm) {
+  return guarded([&] {
+    RP_CHECK(h && lattice && rg && pg && ski && ngi && kgi && eig && wfn && prm,
+             "rp_chiqw_open: null argument");
+    RP_HIP(hipSetDevice(h->device));
+    h->open = false;
+    h->qready = false;
+    // ---- lattice (chiqw.F:145-156)
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) h->a[i][d] = lattice[3 * i + d];
+    auto cross = [](const double* x, const double* y, double* z) {
+      z[0] = x[1] * y[2] - x[2] * y[1];
+      z[1] = x[2] * y[0] - x[0] * y[2];
+      z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    const double tpi = 2.0 * std::acos(-1.0);
+    cross(h->a[1], h->a[2], h->b[0]);
+    h->volume = h->a[0][0] * h->b[0][0] + h->a[0][1] * h->b[0][1] + h->a[0][2] * h->b[0][2];
+    cross(h->a[2], h->a[0], h->b[1]);
+    cross(h->a[0], h->a[1], h->b[2]);
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) h->b[i][d] = h->b[i][d] * tpi / h->volume;
+    // ---- parameters (m_rdinput.f90:52-128, chiqw.F:67-69)
+    h->nk_irr = nk_irr;
+    h->ntg = ntg;
+    h->ntb = ntb;
+    h->nsym = nsym;
+    h->nnp = nnp;
+    h->ef = ef + prm->shift_ef / kAu;
+    h->delt = prm->green_func_delt / kAu;
+    h->delta_ex = prm->max_excitation_energy / kAu;
+    h->dmna = prm->ttrhdrn_dmna;
+    h->dmnr = prm->ttrhdrn_dmnr;
+    h->nen = prm->num_freq_grid;
+    h->flg_crpa = prm->flg_crpa;
+    RP_CHECK(h->nen >= 1, "Num_freq_grid must be >= 1");
+    // ---- Ecut_for_psi from the largest |k+G|^2 (chiqw.F:313-352)
+    double emx = 0;
+    for (int ik = 0; ik < nk_irr; ++ik)
+      for (int ig = 0; ig < ngi[ik]; ++ig) {
+        const int* g = kgi + 3 * ((size_t)ik * ntg + ig);
+        double v[3];
+        for (int d = 0; d < 3; ++d)
+          v[d] = (ski[3 * ik] + (double)g[0]) * h->b[0][d] + (ski[3 * ik + 1] + (double)g[1]) * h->b[1][d] +
+                 (ski[3 * ik + 2] + (double)g[2]) * h->b[2][d];
+        emx = std::max(emx, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      }
+    h->ecut_psi = emx + 1.0e-8;
+    h->ecut_eps = prm->ecut_for_eps == 0.0 ? h->ecut_psi / 10.0 : prm->ecut_for_eps;
+    // ---- packing of the irreducible G lists (chiqw.F:360-378)
+    int L[3] = {0, 0, 0};
+    for (size_t i = 0; i < (size_t)3 * ntg * nk_irr; ++i) L[i % 3] = std::max(L[i % 3], std::abs(kgi[i]));
+    for (int d = 0; d < 3; ++d) L[d] += 1;
+    const int P1 = 2 * L[0] + 1, P2 = 2 * L[1] + 1, P3 = 2 * L[2] + 1;
+    std::vector<int> pack((size_t)P1 * P2 * P3 * nk_irr, -1);
+    for (int ik = 0; ik < nk_irr; ++ik)
+      for (int ig = 0; ig < ngi[ik]; ++ig) {
+        const int* g = kgi + 3 * ((size_t)ik * ntg + ig);
+        pack[(size_t)ik * P1 * P2 * P3 + (g[0] + L[0]) + P1 * ((g[1] + L[1]) + (size_t)P2 * (g[2] + L[2]))] = ig;
+      }
+    // ---- FFT grid (chiqw.F:389-401)
+    const double qwf = 2.0 * std::sqrt(h->ecut_psi);
+    for (int d = 0; d < 3; ++d) {
+      double an = std::sqrt(h->a[d][0] * h->a[d][0] + h->a[d][1] * h->a[d][1] + h->a[d][2] * h->a[d][2]);
+      double dd = std::fabs((h->b[d][0] * h->a[d][0] + h->b[d][1] * h->a[d][1] + h->b[d][2] * h->a[d][2]) / an);
+      h->nw[d] = algn235((int)(qwf / dd) + 1);
+      RP_CHECK(dft_size_supported(h->nw[d]),
+               "FFT grid dimension " + std::to_string(h->nw[d]) + " is not supported by the LDS transform");
+    }
+    RP_CHECK((long)h->nw[0] * h->nw[1] * h->nw[2] <= max_grid_points(),
+             "FFT grid too large for the LDS-resident transform");
+    // ---- symmetry (rginv by the adjugate: integer matrices of det +-1)
+    std::vector<double> rginv(9 * (size_t)nsym);
+    for (int op = 0; op < nsym; ++op) {
+      const int* r = rg + 9 * op;  // r[i + 3j] = rg(i+1, j+1)
+      auto R = [&](int i, int j) { return (double)r[i + 3 * j]; };
+      double det = R(0, 0) * (R(1, 1) * R(2, 2) - R(1, 2) * R(2, 1)) -
+                   R(0, 1) * (R(1, 0) * R(2, 2) - R(1, 2) * R(2, 0)) +
+                   R(0, 2) * (R(1, 0) * R(2, 1) - R(1, 1) * R(2, 0));
+      RP_CHECK(det != 0.0, "singular symmetry operation");
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          int j1 = (j + 1) % 3, j2 = (j + 2) % 3, i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+          // inverse(i,j) = cofactor(j,i)/det
+          rginv[9 * op + i + 3 * j] = (R(j1, i1) * R(j2, i2) - R(j1, i2) * R(j2, i1)) / det;
+        }
+    }
+    // ---- unfold the mesh (chiqw.F:437-480)
+    h->sk0.clear(); h->numirr.clear(); h->numrot.clear(); h->trs.clear(); h->rwv.clear();
+    for (int ik = 0; ik < nk_irr; ++ik)
+      for (int op = 0; op < nsym; ++op) {
+        const int* r = rg + 9 * op;
+        for (int pass = 0; pass < 2; ++pass) {
+          double k[3];
+          int w[3] = {0, 0, 0};
+          for (int i = 0; i < 3; ++i)
+            k[i] = (double)r[i] * ski[3 * ik] + (double)r[i + 3] * ski[3 * ik + 1] + (double)r[i + 6] * ski[3 * ik + 2];
+          double sgn = 1.0;
+          if (pass == 0) kcheck(k, w);
+          else { kcheck_trs(k, w); sgn = -1.0; }
+          bool found = false;
+          for (size_t j = 0; j < h->numirr.size(); ++j)
+            if (std::fabs(h->sk0[3 * j] - sgn * k[0]) < 1e-4 && std::fabs(h->sk0[3 * j + 1] - sgn * k[1]) < 1e-4 &&
+                std::fabs(h->sk0[3 * j + 2] - sgn * k[2]) < 1e-4) { found = true; break; }
+          if (found) continue;
+          for (int i = 0; i < 3; ++i) { h->sk0.push_back(sgn * k[i]); h->rwv.push_back(w[i]); }
+          h->numirr.push_back(ik);
+          h->numrot.push_back(op);
+          h->trs.push_back(pass == 0 ? 1 : -1);
+        }
+      }
+    h->ntk = (int)h->numirr.size();
+    const int ntk = h->ntk;
+    // ---- mesh size (est_NTK.f90:1-39) and the 3D index (chiqw_sub.F:324-384)
+    for (int d = 0; d < 3; ++d) {
+      double x = 1.0;
+      for (int i = 0; i < ntk; ++i) {
+        double v = std::fabs(h->sk0[3 * i + d]);
+        if (v < 1.0e-7) continue;
+        if (v < x) x = v;
+      }
+      h->nkb[d] = (int)std::lround(1.0 / x);
+    }
+    const int n1 = h->nkb[0], n2 = h->nkb[1], n3 = h->nkb[2];
+    RP_CHECK(n1 * n2 * n3 == ntk, "k mesh is not a full Gamma-centred nkb1 x nkb2 x nkb3 mesh");
+    h->index_kpt.assign((size_t)ntk, -1);
+    for (int ik = 0; ik < ntk; ++ik) {
+      double x = h->sk0[3 * ik] * n1, y = h->sk0[3 * ik + 1] * n2, z = h->sk0[3 * ik + 2] * n3;
+      long ix, iy, iz;
+      if (ntk % 2 != 0) {
+        ix = std::lround(x + (n1 - 1.0) / 2.0) + 1; iy = std::lround(y + (n2 - 1.0) / 2.0) + 1;
+        iz = std::lround(z + (n3 - 1.0) / 2.0) + 1;
+      } else {
+        ix = std::lround(x + n1 / 2.0); iy = std::lround(y + n2 / 2.0); iz = std::lround(z + n3 / 2.0);
+      }
+      RP_CHECK(ix >= 1 && ix <= n1 && iy >= 1 && iy <= n2 && iz >= 1 && iz <= n3, "k point off the mesh");
+      h->index_kpt[(ix - 1) + n1 * ((iy - 1) + (size_t)n2 * (iz - 1))] = ik;
+    }
+    for (int v : h->index_kpt) RP_CHECK(v >= 0, "k mesh has holes");
+    h->imt.assign((size_t)24 * ntk, 0);
+    double bflat[9];
+    for (int i = 0; i < 3; ++i)
+      for (int d = 0; d < 3; ++d) bflat[3 * i + d] = h->b[i][d];
+    mkidx(n1, n2, n3, bflat, h->imt.data());
+    RP_CHECK(tetra_lds_bytes(ntk, h->nen) <= 160 * 1024,
+             "k mesh x frequencies too large for the LDS accumulator of the tetrahedron kernel");
+    // ---- G lists of the full mesh (chiqw.F:509-545)
+    h->ng0.assign(ntk, 0);
+    h->kg0.assign((size_t)3 * ntg * ntk, 0);
+    for (int jk = 0; jk < ntk; ++jk) {
+      const int ik = h->numirr[jk], op = h->numrot[jk];
+      const int* r = rg + 9 * op;
+      double k[3];
+      for (int i = 0; i < 3; ++i)
+        k[i] = (double)r[i] * ski[3 * ik] + (double)r[i + 3] * ski[3 * ik + 1] + (double)r[i + 6] * ski[3 * ik + 2] +
+               (double)h->rwv[3 * jk + i];
+      std::vector<int> kg = make_kg0(h->b, h->a, h->ecut_psi, k);
+      int n = (int)(kg.size() / 3);
+      RP_CHECK(n == ngi[ik], "NG_for_psi should be NGI(ik) (chiqw.F:521-524)");
+      h->ng0[jk] = n;
+      const double s = h->trs[jk] == 1 ? 1 : -1;
+      for (int i = 0; i < 3 * n; ++i) h->kg0[(size_t)3 * ntg * jk + i] = (int)(s * kg[i]);
+    }
+    // ---- band classification (chiqw_sub.F:570-613; chiqw.F:941-967)
+    h->n_calc = prm->n_calc_band == 0 ? ntb : prm->n_calc_band;
+    RP_CHECK(h->n_calc >= 1 && h->n_calc <= ntb, "N_CALC_BAND out of range");
+    h->win.assign(ntb, 0);
+    h->tindx.assign(ntb, 0);
+    h->eave.assign(ntb, 0);
+    std::vector<double> bmax(ntb), bmin(ntb);
+    for (int ib = 0; ib < ntb; ++ib) {
+      double top = 0, btm = 0;
+      for (int ik = 0; ik < nk_irr; ++ik) {
+        double e = eig[ib + (size_t)ntb * ik];
+        if (ik == 0) top = btm = e;
+        if (e < btm) btm = e;
+        if (e > top) top = e;
+      }
+      int fs = 0;
+      if (btm > h->ef) fs = 2;
+      if (btm <= h->ef && top >= h->ef) { fs = 1; h->tindx[ib] = ib + 1; }
+      if (top < h->ef) fs = 0;
+      h->win[ib] = fs;
+      h->eave[ib] = (top + btm) / 2.0;
+      bmax[ib] = top;
+      bmin[ib] = btm;
+    }
+    h->n_docc = h->n_pocc = h->n_uocc = 0;
+    for (int ib = 0; ib < h->n_calc; ++ib) {
+      if (h->win[ib] == 0) ++h->n_docc;
+      if (h->win[ib] == 1) ++h->n_pocc;
+      if (h->win[ib] == 2) ++h->n_uocc;
+    }
+    h->n_occ = h->n_docc + h->n_pocc;
+    h->n_vir = h->n_uocc + h->n_pocc;
+    // ---- frequency grid (chiqw.F:1184-1192)
+    double mx = -1e300, mn = 1e300;
+    for (int ib = 0; ib < h->n_calc; ++ib) { mx = std::max(mx, bmax[ib]); mn = std::min(mn, bmin[ib]); }
+    h->emax = std::fabs(mx - mn);
+    h->em.assign(2 * (size_t)h->nen, 0.0);
+    gen_grid(h->nen, h->emax, h->em.data());
+    for (double v : h->em)
+      RP_CHECK(std::isfinite(v), "the reference's gen_grid gives a non-finite frequency grid for this "
+                                 "Num_freq_grid and band spread (choose another Num_freq_grid)");
+    // ---- energies and (zero) projector weights per band and full-mesh k
+    const int nb = h->n_calc;
+    h->eband.assign((size_t)nb * ntk, 0.0);
+    for (int ib = 0; ib < nb; ++ib)
+      for (int k = 0; k < ntk; ++k) h->eband[(size_t)ib * ntk + k] = eig[ib + (size_t)ntb * h->numirr[k]];
+    h->pband.assign(2 * (size_t)nb * ntk, 0.0);
+    // ---- unfold the wavefunctions (make_C0, chiqw_sub.F:264-322) and upload
+    std::vector<double2> coef((size_t)nb * ntk * ntg, make_double2(0, 0));
+    for (int k = 0; k < ntk; ++k) {
+      const int ik = h->numirr[k], op = h->numrot[k], itrs = h->trs[k];
+      const double* ri = &rginv[9 * (size_t)op];
+      const int* pgo = pg + 3 * op;
+      const double sgn = itrs == 1 ? 1.0 : -1.0;
+      for (int ig = 0; ig < h->ng0[k]; ++ig) {
+        const int* g = &h->kg0[(size_t)3 * ntg * k + 3 * ig];
+        int i1 = (int)sgn * g[0], j1 = (int)sgn * g[1], k1 = (int)sgn * g[2];
+        int i2 = i1 + h->rwv[3 * k], j2 = j1 + h->rwv[3 * k + 1], k2 = k1 + h->rwv[3 * k + 2];
+        int i3 = (int)ri[0] * i2 + (int)ri[3] * j2 + (int)ri[6] * k2;
+        int j3 = (int)ri[1] * i2 + (int)ri[4] * j2 + (int)ri[7] * k2;
+        int k3 = (int)ri[2] * i2 + (int)ri[5] * j2 + (int)ri[8] * k2;
+        RP_CHECK(std::abs(i3) <= L[0] && std::abs(j3) <= L[1] && std::abs(k3) <= L[2],
+                 "rotated G vector outside the packing box");
+        int jg = pack[(size_t)ik * P1 * P2 * P3 + (i3 + L[0]) + P1 * ((j3 + L[1]) + (size_t)P2 * (k3 + L[2]))];
+        RP_CHECK(jg >= 0, "rotated G vector missing from the irreducible G list");
+        double ph = tpi * ((double)i1 * pgo[0] + (double)j1 * pgo[1] + (double)k1 * pgo[2]);
+        double cr = std::cos(ph / (double)nnp), ci = -std::sin(ph / (double)nnp);
+        for (int ib = 0; ib < nb; ++ib) {
+          const double* z = wfn + 2 * ((size_t)jg + (size_t)ntg * (ib + (size_t)ntb * ik));
+          double vr = z[0] * cr - z[1] * ci, vi = z[0] * ci + z[1] * cr;
+          coef[((size_t)ib * ntk + k) * ntg + ig] = make_double2(vr, itrs == 1 ? vi : -vi);
+        }
+      }
+    }
+    hipStream_t s = h->stream;
+    h->d_coef.upload(coef, s);
+    h->d_kg0.upload(h->kg0, s);
+    h->d_ng0.upload(h->ng0, s);
+    const size_t nr = (size_t)h->nw[0] * h->nw[1] * h->nw[2];
+    h->d_ur.reserve((size_t)nb * ntk * nr);
+    launch_band_grids(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, ntk, ntg, nb, h->nw[0], h->nw[1], h->nw[2],
+                      h->d_ur.p, s);
+    h->d_eband.upload(h->eband, s);
+    h->d_pband.upload((const double2*)h->pband.data(), (size_t)nb * ntk, s);
+    h->d_imt.upload(h->imt, s);
+    h->d_index.upload(h->index_kpt, s);
+    h->d_em.upload((const double2*)h->em.data(), (size_t)h->nen, s);
+    h->d_bvec.upload(bflat, 9, s);
+    RP_HIP(hipStreamSynchronize(s));
+    h->open = true;
+  });
+}
+
+int rp_chiqw_set_crpa(rp_handle h, int32_t ntk, const int32_t* ns, const int32_t* nb, int32_t mb,
+                      int32_t nwf, const double* umat) {
+  return guarded([&] {
+    RP_CHECK(h && h->open, "rp_chiqw_set_crpa: system not open");
+    RP_CHECK(ntk == h->ntk, "rp_chiqw_set_crpa: NTK mismatch");
+    // prob(Ns(k)+ib, k) = sum_w |U(ib,w,k)|^2  (chiqw.F:905-915)
+    std::fill(h->pband.begin(), h->pband.end(), 0.0);
+    for (int k = 0; k < ntk; ++k)
+      for (int ib = 0; ib < nb[k]; ++ib) {
+        double sr = 0, si = 0;
+        for (int w = 0; w < nwf; ++w) {
+          const double* u = umat + 2 * ((size_t)ib + (size_t)mb * (w + (size_t)nwf * k));
+          // conj(U) U
+          sr += u[0] * u[0] + u[1] * u[1];
+          si += u[0] * u[1] - u[1] * u[0];
+        }
+        int band = ns[k] + ib;
+        if (band >= 0 && band < h->n_calc) {
+          h->pband[2 * ((size_t)band * ntk + k)] = sr;
+          h->pband[2 * ((size_t)band * ntk + k) + 1] = si;
+        }
+      }
+    h->d_pband.upload((const double2*)h->pband.data(), (size_t)h->n_calc * ntk, h->stream);
+    RP_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int rp_chiqw_info(rp_handle h, int32_t* iout, double* dout, double* em) {
+  return guarded([&] {
+    RP_CHECK(h && h->open, "rp_chiqw_info: system not open");
+    if (iout) {
+      int v[16] = {h->ntk, h->nkb[0], h->nkb[1], h->nkb[2], h->nw[0], h->nw[1], h->nw[2], h->nen,
+                   h->n_occ, h->n_vir, h->n_docc, h->ntb, h->nk_irr, h->ntg, 0, 0};
+      std::memcpy(iout, v, sizeof v);
+    }
+    if (dout) {
+      double v[8] = {h->ecut_psi, h->ecut_eps, h->volume, h->emax, h->ef, h->delt, 0, 0};
+      std::memcpy(dout, v, sizeof v);
+    }
+    if (em) std::memcpy(em, h->em.data(), sizeof(double) * h->em.size());
+  });
+}
+
+int rp_chiqw_kmesh(rp_handle h, double* sk0, int32_t* numirr, int32_t* trs) {
+  return guarded([&] {
+    RP_CHECK(h && h->open, "rp_chiqw_kmesh: system not open");
+    if (sk0) std::memcpy(sk0, h->sk0.data(), sizeof(double) * h->sk0.size());
+    if (numirr) std::memcpy(numirr, h->numirr.data(), sizeof(int) * h->numirr.size());
+    if (trs) std::memcpy(trs, h->trs.data(), sizeof(int) * h->trs.size());
+  });
+}
+
+int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgout, int32_t lg_cap) {
+  return guarded([&] {
+    RP_CHECK(h && h->open, "rp_chiqw_prepare_q: system not open");
+    auto t0 = std::chrono::steady_clock::now();
+    double q[3];
+    search_list(qin, q);  // chiqw.F:727-729
+    const bool same = h->qready && q[0] == h->q[0] && q[1] == h->q[1] && q[2] == h->q[2];
+    if (!same) {
+      h->qready = h->have_A = h->have_w = h->have_chi = false;
+      std::memcpy(h->q, q, sizeof q);
+      h->q0 = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;  // chiqw.F:1257
+      h->lg = make_kg0(h->b, h->a, h->ecut_eps, q);
+      h->ng = (int)(h->lg.size() / 3);
+      RP_CHECK(h->ng > 0, "empty chi G sphere");
+      RP_CHECK(h->ng <= h->ntg, "NG_for_eps exceeds NTG (LG0 is sized NTG in the reference)");
+      h->g0 = -1;
+      for (int i = 0; i < h->ng; ++i)
+        if (h->lg[3 * i] == 0 && h->lg[3 * i + 1] == 0 && h->lg[3 * i + 2] == 0) h->g0 = i;
+      const int ntk = h->ntk;
+      h->ikq.assign(ntk, 0);
+      h->shift.assign(3 * (size_t)ntk, 0);
+      for (int k = 0; k < ntk; ++k) {
+        if (h->q0) { h->ikq[k] = k; continue; }
+        int sg[3];
+        int kq = search_kq(ntk, h->sk0.data(), q, k, sg);
+        RP_CHECK(kq >= 0, "k+q is not on the k mesh (q must be a mesh point)");
+        h->ikq[k] = kq;
+        for (int d = 0; d < 3; ++d) h->shift[3 * k + d] = sg[d];
+      }
+      // pair list (chiqw.F:1267-1295; q = 0: chiqw.F:1582-1619)
+      h->pocc.clear();
+      h->pvir.clear();
+      int nintra = 0;
+      for (int jb = 0; jb < h->n_occ; ++jb)
+        for (int ib = 0; ib < h->n_vir; ++ib) {
+          const int vb = h->n_docc + ib;
+          if (std::fabs(h->eave[vb] - h->eave[jb]) >= h->delta_ex) continue;
+          if (h->q0) {
+            const int f1 = h->tindx[vb], f2 = h->tindx[jb];
+            if (f1 == f2 && f1 != 0) { ++nintra; continue; }
+          }
+          h->pocc.push_back(jb);
+          h->pvir.push_back(vb);
+        }
+      RP_CHECK(!(h->q0 && nintra > 0 && h->flg_crpa == 0),
+               "q = 0 with partially occupied bands (intraband Drude term) is not supported");
+      h->npair = (int)h->pocc.size();
+      RP_CHECK(h->npair > 0, "no band pairs");
+      // |q+G| for the Dyson step (chiqw_sub.F:417-432)
+      h->qn.assign(h->ng, 0.0);
+      for (int i = 0; i < h->ng; ++i) {
+        double v[3];
+        for (int d = 0; d < 3; ++d)
+          v[d] = (q[0] + (double)h->lg[3 * i]) * h->b[0][d] + (q[1] + (double)h->lg[3 * i + 1]) * h->b[1][d] +
+                 (q[2] + (double)h->lg[3 * i + 2]) * h->b[2][d];
+        h->qn[i] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      }
+      h->lda = roundup((size_t)h->ng, (size_t)chi_tile());
+      const size_t K = (size_t)h->npair * ntk;
+      h->kpad = roundup(K, (size_t)chi_kchunk());
+      hipStream_t s = h->stream;
+      h->d_lg.upload(h->lg, s);
+      h->d_ikq.upload(h->ikq, s);
+      h->d_shift.upload(h->shift, s);
+      h->d_pocc.upload(h->pocc, s);
+      h->d_pvir.upload(h->pvir, s);
+      h->d_qn.upload(h->qn, s);
+      RP_HIP(hipStreamSynchronize(s));
+      h->qready = true;
+    }
+    h->prep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (ng) *ng = h->ng;
+    if (lgout) std::memcpy(lgout, h->lg.data(), sizeof(int) * std::min<size_t>(h->lg.size(), (size_t)3 * lg_cap));
+  });
+}
+
+int rp_chiqw_npair(rp_handle h, int32_t* npair) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready, "no q prepared");
+    *npair = h->npair;
+  });
+}
+
+static void check_status(rp_handle h, const char* what) {
+  int st = 0;
+  RP_HIP(hipMemcpyAsync(&st, h->d_status.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  RP_HIP(hipStreamSynchronize(h->stream));
+  if (st & 1) throw Error(std::string(what) + ": too many sub-tetrahedra in a Fermi-surface cut");
+  if (st & 2) throw Error(std::string(what) + ": bad tetrahedron degeneracy class (errexec)");
+  if (st & 4) throw Error(std::string(what) + ": singular dielectric matrix (zero pivot)");
+}
+
+int rp_pairdens_batch(rp_handle h, float* m_out) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready, "rp_pairdens_batch: no q prepared");
+    RP_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const size_t K = (size_t)h->npair * h->ntk;
+    h->d_A.reserve(h->kpad * h->lda);
+    if (h->kpad > K)
+      RP_HIP(hipMemsetAsync(h->d_A.p + K * h->lda, 0, (h->kpad - K) * h->lda * sizeof(float2), s));
+    launch_pair_densities(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
+                          h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_A.p, s);
+    if (h->q0) {
+      h->d_A3.reserve(3 * h->kpad * h->lda);
+      for (int ix = 0; ix < 3; ++ix)
+        if (h->kpad > K)
+          RP_HIP(hipMemsetAsync(h->d_A3.p + (ix * h->kpad + K) * h->lda, 0,
+                                (h->kpad - K) * h->lda * sizeof(float2), s));
+      // three copies laid out [ix][kpad][lda]
+      launch_q0_heads(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_pocc.p, h->d_pvir.p,
+                      h->d_eband.p, h->d_bvec.p, h->delt, h->g0, h->ng, h->lda, h->kpad, (int)K,
+                      h->d_A.p, h->d_A3.p, s);
+    }
+    if (m_out) {
+      const int nc = h->q0 ? 3 : 1;
+      for (int ix = 0; ix < nc; ++ix) {
+        const float2* src = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+        RP_HIP(hipMemcpy2DAsync(m_out + 2 * (size_t)ix * K * h->ng, sizeof(float2) * h->ng, src,
+                                sizeof(float2) * h->lda, sizeof(float2) * h->ng, K, hipMemcpyDeviceToHost, s));
+      }
+    }
+    RP_HIP(hipStreamSynchronize(s));
+    h->have_A = true;
+  });
+}
+
+int rp_tetra_weights_batch(rp_handle h, double* d_out) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready, "rp_tetra_weights_batch: no q prepared");
+    RP_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const size_t K = (size_t)h->npair * h->ntk;
+    h->d_w.reserve((size_t)h->nen * h->kpad);
+    RP_HIP(hipMemsetAsync(h->d_w.p, 0, sizeof(double2) * h->nen * h->kpad, s));
+    RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
+    launch_tetra_weights(h->ntk, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p, h->d_pocc.p,
+                         h->d_pvir.p, h->d_ikq.p, h->d_eband.p, h->d_pband.p, h->ef, h->delt, h->dmna,
+                         h->dmnr, h->nen, h->d_em.p, h->kpad, h->npair, h->d_status.p, h->d_w.p, s);
+    check_status(h, "tetrahedron weights");
+    if (d_out)
+      RP_HIP(hipMemcpy2DAsync(d_out, sizeof(double2) * K, h->d_w.p, sizeof(double2) * h->kpad,
+                              sizeof(double2) * K, h->nen, hipMemcpyDeviceToHost, s));
+    RP_HIP(hipStreamSynchronize(s));
+    h->have_w = true;
+  });
+}
+
+int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready && h->have_A && h->have_w, "rp_chi_accumulate: pair densities/weights missing");
+    RP_CHECK(precision >= RP_PREC_FP64 && precision <= RP_PREC_BF16X3, "unknown precision");
+    RP_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const int ng = h->ng, nen = h->nen, nc = h->q0 ? 3 : 1;
+    const int ntile = (ng + chi_tile() - 1) / chi_tile();
+    const size_t kchunks = h->kpad / chi_kchunk();
+    long wgs = (long)ntile * ntile * nen;
+    int nsplit = (int)std::min<long>((long)kchunks, std::max<long>(1, (2048 + wgs - 1) / wgs));
+    const size_t per = (size_t)ng * ng * nen;
+    h->d_part.reserve((size_t)nsplit * per);
+    h->d_chi.reserve((size_t)nc * per);
+    for (int ix = 0; ix < nc; ++ix) {
+      const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+      launch_chi_gemm(precision, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, nsplit, h->d_part.p, s);
+      launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
+    }
+    if (chi_out)
+      RP_HIP(hipMemcpyAsync(chi_out, h->d_chi.p, sizeof(double2) * nc * per, hipMemcpyDeviceToHost, s));
+    RP_HIP(hipStreamSynchronize(s));
+    h->have_chi = true;
+  });
+}
+
+int rp_dyson(rp_handle h, double* epsinv_out) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready && h->have_chi, "rp_dyson: chi missing");
+    RP_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const int n = h->ng, nb = h->nbatch();
+    const size_t per = (size_t)n * n;
+    h->d_eps.reserve((size_t)nb * per);
+    h->d_rowc.reserve((size_t)nb * n);
+    h->d_colc.reserve((size_t)nb * n);
+    h->d_ipiv.reserve((size_t)nb * n);
+    h->d_wd.reserve(3);
+    const double wd0[3] = {0, 0, 0};  // no intraband pairs (cRPA, or gapped fRPA): wpl = 0
+    RP_HIP(hipMemcpyAsync(h->d_wd.p, wd0, sizeof wd0, hipMemcpyHostToDevice, s));
+    RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
+    launch_eps_build(h->d_chi.p, h->d_qn.p, n, nb, h->nen, h->q0 ? 1 : 0, h->g0, h->d_wd.p, h->d_em.p,
+                     h->delt, h->d_eps.p, s);
+    launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_rowc.p, h->d_colc.p, h->d_status.p, s);
+    check_status(h, "Dyson step");
+    if (epsinv_out)
+      RP_HIP(hipMemcpyAsync(epsinv_out, h->d_eps.p, sizeof(double2) * nb * per, hipMemcpyDeviceToHost, s));
+    RP_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_out, double* chi_out,
+               double* timings) {
+  int ng = 0;
+  if (rp_chiqw_prepare_q(h, q, &ng, nullptr, 0)) return 1;
+  return guarded([&] {
+    hipStream_t s = h->stream;
+    RP_HIP(hipEventRecord(h->ev[0], s));
+    if (rp_pairdens_batch(h, nullptr)) throw Error(g_err);
+    RP_HIP(hipEventRecord(h->ev[1], s));
+    if (rp_tetra_weights_batch(h, nullptr)) throw Error(g_err);
+    RP_HIP(hipEventRecord(h->ev[2], s));
+    if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
+    RP_HIP(hipEventRecord(h->ev[3], s));
+    if (rp_dyson(h, epsinv_out)) throw Error(g_err);
+    RP_HIP(hipEventRecord(h->ev[4], s));
+    RP_HIP(hipEventSynchronize(h->ev[4]));
+    if (timings) {
+      float ms;
+      for (int i = 0; i < 4; ++i) {
+        RP_HIP(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
+        timings[i] = ms * 1e-3;
+      }
+      RP_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[4]));
+      timings[4] = ms * 1e-3;
+      timings[5] = h->prep_seconds;
+    }
+  });
+}
+
+}  // extern "C"

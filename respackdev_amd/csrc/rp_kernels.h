@@ -1,0 +1,47 @@
+// Host-side launchers of the gfx950 kernels (internal to librespack_gpu).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "../../include/respack_gpu.h"
+
+namespace rp {
+
+// rp_pairdens.hip
+bool dft_size_supported(int n);
+int max_grid_points();
+void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                       int nband, int nx, int ny, int nz, float2* ur, hipStream_t s);
+void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
+                           const int* ikq, const int* shift, int nx, int ny, int nz,
+                           const int* lg, int ng, size_t lda, int nK, float2* A, hipStream_t s);
+void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                     const int* pocc, const int* pvir, const double* eband, const double* bvec,
+                     double delt, int g0, int ng, size_t lda, size_t rows, int nK, const float2* A,
+                     float2* A3, hipStream_t s);
+
+// rp_tetra.hip
+size_t tetra_lds_bytes(int nk3, int nen);
+void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const int* index_kpt,
+                          const int* pocc, const int* pvir, const int* ikq, const double* eband,
+                          const double2* pband, double ef, double delt, double dmna, double dmnr,
+                          int nen, const double2* em, size_t ldd, int npair, int* status,
+                          double2* dout, hipStream_t s);
+
+// rp_chi_gemm.hip
+int chi_tile();
+int chi_kchunk();
+void launch_chi_gemm(int prec, const float2* A, size_t lda, const double2* dw, size_t ldd,
+                     size_t kpad, int ng, int nen, int nsplit, double2* partial, hipStream_t s);
+void launch_chi_reduce(const double2* partial, size_t n, int nsplit, double inv_vol, double2* chi,
+                       hipStream_t s);
+
+// rp_dyson.hip
+void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
+                      int g0, const double* wd, const double2* em, double delt, double2* eps,
+                      hipStream_t s);
+void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, double2* rowc, double2* colc,
+                         int* status, hipStream_t s);
+
+}  // namespace rp

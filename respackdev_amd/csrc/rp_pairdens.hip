@@ -1,0 +1,286 @@
+// Pair densities of calc_InterStateMatrix (chiqw_sub.F:97-205) on gfx950.
+//
+// Stage 1 (once per system): every band at every full-mesh k is scattered onto
+// the nwx*nwy*nwz grid and transformed with e^{+i} (fft3_bw, unscaled); the
+// result is rounded to single precision exactly as the reference's CMPLX does
+// (chiqw_sub.F:143,162) and kept in HBM as complex64: ur[band][k][r].
+//
+// Stage 2 (per q): for every (pair, k) the product conj(u_kq(r) e^{-i 2pi
+// r.dG/N}) u_k(r) is formed in FP64, transformed with e^{-i} and 1/N
+// (fft3_fw), read at -G for the chi sphere and rounded to single precision
+// (chiqw_sub.F:192-202): A[K][lda] complex64, K = pair*NTK + k.
+//
+// Both transforms are LDS-resident: the whole FP64 grid (<= 10240 points)
+// lives in one workgroup's LDS and each axis is done as independent
+// per-thread line DFTs (the axis lengths are the reference's 2,3,5-smooth
+// sizes, <= 32), so HBM sees one read of the inputs and one write of the
+// outputs per transform.
+#include "rp_common.h"
+#include "rp_kernels.h"
+
+namespace rp {
+
+// Largest grid the LDS-resident transform holds (FP64 grid plus twiddles
+// within the 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
+constexpr int kMaxGrid = 10000;
+
+// Output k of a length-N DFT of v with twiddles w[m] = exp(-2 pi i m / N);
+// sign > 0 uses the conjugate twiddles (e^{+i}).
+template <int N>
+__device__ inline void dft_line(double2* base, int stride, int sign, const double2* tw) {
+  double2 v[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = base[j * stride];
+  double2 out[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      double2 w = tw[(j * k) % N];
+      double wi = sign > 0 ? -w.y : w.y;
+      re += v[j].x * w.x - v[j].y * wi;
+      im += v[j].x * wi + v[j].y * w.x;
+    }
+    out[k] = make_double2(re, im);
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) base[k * stride] = out[k];
+}
+
+// All lines of one axis.  Line l (of nlines) starts at line_base(l).
+template <int N>
+__device__ void dft_axis_n(double2* g, int axis, int nx, int ny, int nz, int sign,
+                           const double2* tw) {
+  int nlines, stride;
+  if (axis == 0) { nlines = ny * nz; stride = 1; }
+  else if (axis == 1) { nlines = nx * nz; stride = nx; }
+  else { nlines = nx * ny; stride = nx * ny; }
+  for (int l = threadIdx.x; l < nlines; l += blockDim.x) {
+    int base;
+    if (axis == 0) base = nx * l;
+    else if (axis == 1) base = (l % nx) + nx * ny * (l / nx);
+    else base = l;
+    dft_line<N>(g + base, stride, sign, tw);
+  }
+}
+
+__device__ void dft_axis(double2* g, int axis, int n, int nx, int ny, int nz, int sign,
+                         const double2* tw) {
+  switch (n) {
+#define RP_CASE(N) \
+  case N: dft_axis_n<N>(g, axis, nx, ny, nz, sign, tw); break;
+    RP_CASE(1) RP_CASE(2) RP_CASE(3) RP_CASE(4) RP_CASE(5) RP_CASE(6) RP_CASE(8)
+    RP_CASE(9) RP_CASE(10) RP_CASE(12) RP_CASE(15) RP_CASE(16) RP_CASE(18) RP_CASE(20)
+    RP_CASE(24) RP_CASE(25) RP_CASE(27) RP_CASE(30) RP_CASE(32)
+#undef RP_CASE
+    default: break;  // rejected on the host before launch
+  }
+}
+
+bool dft_size_supported(int n) {
+  switch (n) {
+    case 1: case 2: case 3: case 4: case 5: case 6: case 8: case 9: case 10: case 12:
+    case 15: case 16: case 18: case 20: case 24: case 25: case 27: case 30: case 32:
+      return true;
+    default:
+      return false;
+  }
+}
+int max_grid_points() { return kMaxGrid; }
+
+__device__ inline void fill_twiddles(double2* tw, int n) {
+  for (int m = threadIdx.x; m < n; m += blockDim.x) {
+    double s, c;
+    sincospi(2.0 * (double)m / (double)n, &s, &c);
+    tw[m] = make_double2(c, -s);
+  }
+}
+
+__device__ inline void dft3(double2* g, int nx, int ny, int nz, int sign, const double2* twx,
+                            const double2* twy, const double2* twz) {
+  // z, then y, then x (the order of fft3_fw/fft3_bw, m_fft3d_20150826.f90:1598-1720)
+  dft_axis(g, 2, nz, nx, ny, nz, sign, twz);
+  __syncthreads();
+  dft_axis(g, 1, ny, nx, ny, nz, sign, twy);
+  __syncthreads();
+  dft_axis(g, 0, nx, nx, ny, nz, sign, twx);
+  __syncthreads();
+}
+
+__device__ inline int wrap(int g, int n) { return ((g % n) + n) % n; }
+
+// grid: (ntk, nband).  coef[band][k][ntg] complex128; kg[k][ntg][3]; ng0[k].
+__global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ coef,
+                                                    const int* __restrict__ kg,
+                                                    const int* __restrict__ ng0, int ntk,
+                                                    int ntg, int nx, int ny, int nz,
+                                                    float2* __restrict__ ur) {
+  extern __shared__ double2 lds[];
+  double2* g = lds;
+  const int nr = nx * ny * nz;
+  double2* twx = lds + nr;
+  double2* twy = twx + nx;
+  double2* twz = twy + ny;
+  const int k = blockIdx.x, b = blockIdx.y;
+  fill_twiddles(twx, nx);
+  fill_twiddles(twy, ny);
+  fill_twiddles(twz, nz);
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) g[r] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const double2* c = coef + ((size_t)b * ntk + k) * ntg;
+  const int* kk = kg + (size_t)k * ntg * 3;
+  const int n = ng0[k];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int r = wrap(kk[3 * i], nx) + nx * (wrap(kk[3 * i + 1], ny) + ny * wrap(kk[3 * i + 2], nz));
+    g[r] = c[i];
+  }
+  __syncthreads();
+  dft3(g, nx, ny, nz, +1, twx, twy, twz);
+  float2* out = ur + ((size_t)b * ntk + k) * nr;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x)
+    out[r] = make_float2((float)g[r].x, (float)g[r].y);  // CMPLX(): single precision
+}
+
+// grid: K = npair*ntk blocks (flattened).  u_occ = ur of band pocc[p] at k,
+// u_vir = ur of band pvir[p] at ikq[k]; bands are rows of ur (occ and vir
+// both index the same band array).
+__global__ void __launch_bounds__(256) k_pair_densities(
+    const float2* __restrict__ ur, int ntk, const int* __restrict__ pocc,
+    const int* __restrict__ pvir, const int* __restrict__ ikq, const int* __restrict__ shift,
+    int nx, int ny, int nz, const int* __restrict__ lg, int ng, size_t lda, int k_offset,
+    float2* __restrict__ A) {
+  extern __shared__ double2 lds[];
+  double2* g = lds;
+  const int nr = nx * ny * nz;
+  double2* twx = lds + nr;
+  double2* twy = twx + nx;
+  double2* twz = twy + ny;
+  const int K = blockIdx.x + k_offset;
+  const int p = K / ntk, k = K % ntk;
+  const int kq = ikq[k];
+  fill_twiddles(twx, nx);
+  fill_twiddles(twy, ny);
+  fill_twiddles(twz, nz);
+  const float2* uk = ur + ((size_t)pocc[p] * ntk + k) * nr;
+  const float2* ukq = ur + ((size_t)pvir[p] * ntk + kq) * nr;
+  const double d1 = shift[3 * k], d2 = shift[3 * k + 1], d3 = shift[3 * k + 2];
+  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+    int r1 = r % nx, r2 = (r / nx) % ny, r3 = r / (nx * ny);
+    double ph = tpi * ((double)r1 * d1 / (double)nx + (double)r2 * d2 / (double)ny +
+                       (double)r3 * d3 / (double)nz);
+    double s, c;
+    sincos(ph, &s, &c);
+    float2 a = uk[r], b = ukq[r];
+    // u_kq * exp(-i ph)
+    double bx = (double)b.x * c + (double)b.y * s;
+    double by = (double)b.y * c - (double)b.x * s;
+    // conj(u_kq') * u_k
+    double ax = a.x, ay = a.y;
+    g[r] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
+  }
+  __syncthreads();
+  dft3(g, nx, ny, nz, -1, twx, twy, twz);
+  const double scl = 1.0 / (double)nr;
+  float2* out = A + (size_t)K * lda;
+  for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) {
+    if (i < ng) {
+      int r = wrap(-lg[3 * i], nx) + nx * (wrap(-lg[3 * i + 1], ny) + ny * wrap(-lg[3 * i + 2], nz));
+      double2 v = g[r];
+      out[i] = make_float2((float)(v.x * scl), (float)(v.y * scl));
+    } else {
+      out[i] = make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// q = 0: the G = 0 entry per direction ix is -i<vir|p|occ>/de (calc_VMab,
+// chiqw_sub.F:207-232, and chiqw.F:1787-1805), zero when |de| <= delt.
+// Writes three copies A3[ix][K][lda] from A[K][lda] with that entry replaced.
+// grid: K blocks, 64 threads.  coef as in k_band_grids.
+__global__ void __launch_bounds__(64) k_q0_heads(
+    const double2* __restrict__ coef, const int* __restrict__ kg, const int* __restrict__ ng0,
+    int ntk, int ntg, const int* __restrict__ pocc, const int* __restrict__ pvir,
+    const double* __restrict__ eband, const double* bvec, double delt, int g0, int ng,
+    size_t lda, size_t rows, const float2* __restrict__ A, float2* __restrict__ A3) {
+  const int K = blockIdx.x;
+  const int p = K / ntk, k = K % ntk;
+  const int jo = pocc[p], iv = pvir[p];
+  const double2* ck = coef + ((size_t)jo * ntk + k) * ntg;
+  const double2* ckq = coef + ((size_t)iv * ntk + k) * ntg;
+  const int* kk = kg + (size_t)k * ntg * 3;
+  double s[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  const int n = ng0[k];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double2 a = ck[i], b = ckq[i];
+    // conj(b) * a
+    double pr = b.x * a.x + b.y * a.y, pi = b.x * a.y - b.y * a.x;
+    for (int d = 0; d < 3; ++d) {
+      double gv = (double)kk[3 * i] * bvec[d] + (double)kk[3 * i + 1] * bvec[3 + d] +
+                  (double)kk[3 * i + 2] * bvec[6 + d];
+      s[d][0] += gv * pr;
+      s[d][1] += gv * pi;
+    }
+  }
+  __shared__ double red[6][64];
+  for (int d = 0; d < 3; ++d) { red[2 * d][threadIdx.x] = s[d][0]; red[2 * d + 1][threadIdx.x] = s[d][1]; }
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int c = 0; c < 6; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + w];
+    __syncthreads();
+  }
+  double de = eband[(size_t)iv * ntk + k] - eband[(size_t)jo * ntk + k];
+  if (fabs(de) < delt) de = delt;
+  const float2* src = A + (size_t)K * lda;
+  for (int ix = 0; ix < 3; ++ix) {
+    float2* dst = A3 + ((size_t)ix * rows + K) * lda;
+    for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+    if (threadIdx.x == 0 && g0 >= 0 && g0 < ng) {
+      // vm = i*S ; vm = -i*(vm - 0)/de = S/de
+      double vr = red[2 * ix][0] / de, vi = red[2 * ix + 1][0] / de;
+      // m_tmp is complex(8) in the reference (no single rounding of this entry)
+      dst[g0] = fabs(de) <= delt ? make_float2(0.f, 0.f) : make_float2((float)vr, (float)vi);
+    }
+  }
+}
+
+void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                       int nband, int nx, int ny, int nz, float2* ur, hipStream_t s) {
+  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
+  RP_HIP(hipFuncSetAttribute((const void*)k_band_grids, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds));
+  dim3 grid(ntk, nband);
+  hipLaunchKernelGGL(k_band_grids, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, ur);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
+                           const int* ikq, const int* shift, int nx, int ny, int nz,
+                           const int* lg, int ng, size_t lda, int nK, float2* A,
+                           hipStream_t s) {
+  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
+  RP_HIP(hipFuncSetAttribute((const void*)k_pair_densities,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  // grid.x limited to 2^31-1; chunk for safety
+  const int chunk = 1 << 30;
+  for (int off = 0; off < nK; off += chunk) {
+    int n = nK - off < chunk ? nK - off : chunk;
+    hipLaunchKernelGGL(k_pair_densities, dim3(n), dim3(256), lds, s, ur, ntk, pocc, pvir, ikq,
+                       shift, nx, ny, nz, lg, ng, lda, off, A);
+    RP_HIP(hipGetLastError());
+  }
+}
+
+void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                     const int* pocc, const int* pvir, const double* eband, const double* bvec,
+                     double delt, int g0, int ng, size_t lda, size_t rows, int nK, const float2* A,
+                     float2* A3, hipStream_t s) {
+  hipLaunchKernelGGL(k_q0_heads, dim3(nK), dim3(64), 0, s, coef, kg, ng0, ntk, ntg, pocc, pvir,
+                     eband, bvec, delt, g0, ng, lda, rows, A, A3);
+  RP_HIP(hipGetLastError());
+}
+
+}  // namespace rp
