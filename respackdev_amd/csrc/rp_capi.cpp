@@ -512,6 +512,9 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     }
     h->n_occ = h->n_docc + h->n_pocc;
     h->n_vir = h->n_uocc + h->n_pocc;
+    RP_CHECK(h->n_pocc == 0,
+             "partially occupied bands (tetrahedra cut by the Fermi level) are not handled by "
+             "the GPU tetrahedron kernel");
     // ---- frequency grid (chiqw.F:1184-1192)
     double mx = -1e300, mn = 1e300;
     for (int ib = 0; ib < h->n_calc; ++ib) { mx = std::max(mx, bmax[ib]); mn = std::min(mn, bmin[ib]); }

@@ -234,12 +234,15 @@ __global__ void __launch_bounds__(256) k_chi_gemm(const float2* __restrict__ A, 
     }
   }
 
-  // epilogue: lane holds C[4*(lane>>4)+v][lane&15] of each 16x16 block
+  // epilogue.  Result layout of a 16x16 block (checked on gfx950): the FP32
+  // and BF16 MFMAs leave C[4*(lane>>4)+v][lane&15] in element v of a lane, the
+  // FP64 MFMA leaves C[(lane>>4)+4*v][lane&15].
   double2* o = out + ((size_t)z * nen + w) * (size_t)ng * ng;
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 4; ++v) {
-        const int gi = g0 + wm * 32 + a * 16 + 4 * q + v;
+        const int row = P == RP_PREC_FP64 ? q + 4 * v : 4 * q + v;
+        const int gi = g0 + wm * 32 + a * 16 + row;
         const int hj = h0 + wn * 32 + b * 16 + li;
         if (gi < ng && hj < ng)
           o[(size_t)hj * ng + gi] = make_double2(accd_re[a][b][v], accd_im[a][b][v]);

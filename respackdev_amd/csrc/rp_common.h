@@ -84,6 +84,6 @@ __host__ __device__ inline zd operator/(zd a, zd b) {
 }
 __host__ __device__ inline zd operator/(double s, zd b) { return zd{s, 0.0} / b; }
 __host__ __device__ inline double absz(zd a) { return hypot(a.x, a.y); }
-__device__ inline zd logz(zd a) { return {log(hypot(a.x, a.y)), atan2(a.y, a.x)}; }
+__device__ __forceinline__ zd logz(zd a) { return {log(hypot(a.x, a.y)), atan2(a.y, a.x)}; }
 
 }  // namespace rp
