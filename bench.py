@@ -1,0 +1,182 @@
+"""chi(q,w) seconds per q-point on one MI355X: split-BF16 mixed path vs plain FP64.
+
+Workload (the precision configuration): simple-cubic a = 7.0 bohr, 6x6x6
+Gamma-centred k mesh (216 k), 20 occupied + 180 empty bands (3600 pairs),
+~1000 plane waves per k (reference FFT grid 15^3), chi cutoff 22.2 Ry
+(~600 G vectors), 32 frequencies from the reference's gen_grid, q = (0,0,1/6).
+Seeded synthetic wavefunctions (see respackdev_amd/gen.py).
+
+A step is one q-point of the hot path with the wavefunctions resident in HBM:
+pair densities, tetrahedron weights, the chi contraction and the Dyson step
+(eps^-1 for all 32 frequencies), i.e. everything chiqw does per q.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is not supported (one GPU).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "chi(q,w) s per q-point: mixed/batched vs plain FP64 GPU at <=1e-5 rel err"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}  # dense MFMA, MI355X
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(w, wl, npair_total: int, sample_pairs: int, threads: int) -> dict:
+    """The oracle's FP64 OpenMP restatement (same loop order as chiqw.F:1319-1410)
+    on the first `sample_pairs` band pairs, scaled to the full pair count."""
+    import ctypes as C
+    from oracle import lib, ptr, pipeline
+    s = pipeline.setup(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+    L = lib()
+    b = s["b"]
+    sq = np.zeros(3)
+    qin = np.ascontiguousarray(wl["q"], np.float64)
+    L.ro_search_list(ptr(qin), ptr(sq))
+    tmp = np.zeros((200000, 3), np.int32)
+    ng = L.ro_make_kg0(200000, ptr(b[0]), ptr(b[1]), ptr(b[2]), s["ecut_eps"], sq[0], sq[1], sq[2], ptr(tmp))
+    lg = np.ascontiguousarray(tmp[:ng])
+    ntk, ntg = s["ntk"], s["ntg"]
+    ikq = np.zeros(ntk, np.int32)
+    shift = np.zeros((ntk, 3), np.int32)
+    for k in range(ntk):
+        sg = np.zeros(3, np.int32)
+        ikq[k] = L.ro_search_kq(ntk, ptr(s["sk0"]), sq[0], sq[1], sq[2], k, ptr(sg))
+        shift[k] = sg
+    pocc = np.array([j for j in range(s["n_occ"]) for i in range(s["n_vir"])][:sample_pairs], np.int32)
+    pvir = np.array([s["n_docc"] + i for j in range(s["n_occ"]) for i in range(s["n_vir"])][:sample_pairs], np.int32)
+    nb = s["coef"].shape[0]
+    coef = np.ascontiguousarray(s["coef"])
+    pband = np.zeros((nb, ntk), np.complex128)
+    nen = len(s["em"])
+    chi = np.zeros((nen, ng, ng), np.complex128)
+    nkb = [int(x) for x in s["nkb"]]
+    t0 = time.perf_counter()
+    L.ro_chiqw_q(ntk, ntg, ptr(s["ng0"]), ptr(s["kg0"]), nb, ptr(coef), nb, ptr(coef),
+                 ptr(s["eband"]), ptr(s["eband"]), ptr(pband), ptr(pband), len(pocc), ptr(pocc),
+                 ptr(pvir), ptr(ikq), ptr(shift), s["nw"][0], s["nw"][1], s["nw"][2], ng, ptr(lg),
+                 nkb[0], nkb[1], nkb[2], ptr(s["imt"]), ptr(s["index_kpt"]), s["ef"], s["delt"],
+                 s["dmna"], s["dmnr"], nen, ptr(s["em"]), s["volume"], ptr(chi), None, None,
+                 threads)
+    dt = time.perf_counter() - t0
+    return {"value": dt * npair_total / len(pocc), "unit": "s/q-point", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(pocc)} of {npair_total} band pairs x {ntk} k x {nen} w at NG={ng} "
+                      f"({dt:.1f} s measured, scaled linearly in pairs)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2_precision")
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--cpu-pairs", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="run a single precision (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(q-points would shard with no communication; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    from respackdev_amd import gen
+    from respackdev_amd.chiqw import Chiqw, from_system
+
+    wl = gen.WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    w = from_system(s)
+    log(f"system generated in {time.perf_counter() - t0:.1f} s: {len(w.ski)} irreducible k, "
+        f"{w.ntb} bands, NG_psi {w.ngi.min()}-{w.ngi.max()}")
+    c = Chiqw(0)
+    t0 = time.perf_counter()
+    info = c.open(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+    log(f"opened in {time.perf_counter() - t0:.1f} s: NTK {info['ntk']} grid {info['nw']} "
+        f"nen {info['nen']}")
+    lg = c.prepare_q(wl["q"])
+    ng, npair, ntk, nen = len(lg), c.npair, info["ntk"], info["nen"]
+    K = npair * ntk
+    flops = 8.0 * ng * ng * nen * K
+    log(f"NG {ng}, pairs {npair}, K {K}, contraction {flops / 1e12:.1f} TFLOP per q")
+
+    precs = [args.only] if args.only else ["fp64", args.mixed]
+    results = {}
+    for prec in precs:
+        for i in range(args.warmup):
+            r = c.run_q(wl["q"], prec, want_chi=False)
+            log(f"{prec} warmup {i}: {r['timings']['device_total']:.3f} s")
+        times, stages = [], []
+        t_all = time.perf_counter()
+        for i in range(args.steps):
+            r = c.run_q(wl["q"], prec, want_chi=(i == args.steps - 1))
+            times.append(r["timings"]["device_total"])
+            stages.append(r["timings"])
+            log(f"{prec} step {i}: {r['timings']}")
+        wall = (time.perf_counter() - t_all) / args.steps
+        results[prec] = dict(device=float(np.mean(times)), wall=wall, chi=r["chi"], eps=r["epsinv"],
+                             stages={k: float(np.mean([st[k] for st in stages])) for k in stages[0]})
+    c.close()
+
+    main_prec = precs[-1]
+    res = results[main_prec]
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out = {
+        "metric": METRIC,
+        "value": res["wall"],
+        "unit": "s/q-point",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["wall"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": f"{main_prec} contraction (fp32 accumulation, fp64 merge); fp64 pair densities, "
+                 f"tetrahedron weights and Dyson" if main_prec != "fp64" else "fp64",
+        "data": "synthetic (seeded Gaussian plane-wave coefficients, Gram-Schmidt per k)",
+        "config": {"workload": f"{args.workload}: a=7.0 bohr sc, {info['nkb'][0]}^3 k (NTK {ntk}), "
+                               f"{info['n_occ']} occ + {info['n_vir']} empty bands, grid "
+                               f"{info['nw'][0]}^3, NG {ng}, nen {nen}, q={tuple(wl['q'])}",
+                   "ntk": ntk, "npair": npair, "ng": ng, "nen": nen, "K": K},
+    }
+    if "fp64" in results and main_prec != "fp64":
+        out["fp64_s_per_q"] = results["fp64"]["wall"]
+        out["speedup_vs_fp64"] = results["fp64"]["wall"] / res["wall"]
+        out["chi_rel_err_vs_fp64"] = rel(res["chi"], results["fp64"]["chi"])
+        out["epsinv_rel_err_vs_fp64"] = rel(res["eps"], results["fp64"]["eps"])
+    out["stage_s"] = {p: results[p]["stages"] for p in results}
+    t_gemm = res["stages"]["contraction"]
+    achieved = flops / t_gemm
+    rf = {"kernel": f"k_chi_gemm<{main_prec}>", "bound": "mfma", "achieved": achieved / 1e12,
+          "peak": PEAK[main_prec] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[main_prec],
+          "traffic": None, "algorithmic_flops_per_launch": flops}
+    if main_prec == "bf16x3":
+        rf["executed_frac"] = 3.0 * achieved / PEAK[main_prec]
+    out["roofline"] = rf
+    if "fp64" in results:
+        a64 = flops / results["fp64"]["stages"]["contraction"]
+        out["roofline_fp64"] = {"achieved": a64 / 1e12, "peak": PEAK["fp64"] / 1e12,
+                                "unit": "TFLOP/s", "frac": a64 / PEAK["fp64"]}
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"cpu baseline: oracle on {args.cpu_pairs} pairs with {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(w, wl, npair, args.cpu_pairs, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

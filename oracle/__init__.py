@@ -63,7 +63,7 @@ def lib():
         L.ro_eps_rpa.restype = I32
         L.ro_eps_rpa.argtypes = [I32, P, P, P, P, P, P, P]
         L.ro_eps_rpa_q0.restype = I32
-        L.ro_eps_rpa_q0.argtypes = [I32, P, P, P, P, I32, D, D, C.c_double * 2, P, P]
+        L.ro_eps_rpa_q0.argtypes = [I32, P, P, P, P, I32, D, D, D, P, P]
         L.ro_kcheck.argtypes = [P, P]
         L.ro_kcheck_trs.argtypes = [P, P]
         L.ro_search_list.argtypes = [P, P]

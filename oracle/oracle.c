@@ -775,12 +775,12 @@ int ro_eps_rpa(int ng, const int32_t *lg, const double *q, const double *b1,
 }
 
 int ro_eps_rpa_q0(int ng, const int32_t *lg, const double *b1, const double *b2,
-                  const double *b3, int g0, double wd, double delt, ro_z wc,
+                  const double *b3, int g0, double wd, double delt, double wr,
                   const ro_z *chi, ro_z *eps) {
   /* chiqw_sub.F:459-568 */
   const double tpi = 2.0 * acos(-1.0);
   const double q[3] = {0, 0, 0};
-  double w = creal(wc);
+  double w = wr;
   if (w == 0.0) w = 1.0e-5;
   double *qn = malloc(sizeof(double) * ng);
   for (int i = 0; i < ng; ++i) qn[i] = qg_norm(lg + 3 * i, q, b1, b2, b3);

@@ -102,9 +102,10 @@ int ro_zinv(int n, ro_z *a);
  * inverted.  chi (n*n, column-major) is read; epsinv written. */
 int ro_eps_rpa(int ng, const int32_t *lg, const double *q, const double *b1,
                const double *b2, const double *b3, const ro_z *chi, ro_z *epsinv);
-/* calc_eps_rpa_q_0 (chiqw_sub.F:459-568) with head/wings and Drude term. */
+/* calc_eps_rpa_q_0 (chiqw_sub.F:459-568) with head/wings and Drude term;
+ * w = Re of the frequency (the routine uses dble(wcmplx) only). */
 int ro_eps_rpa_q0(int ng, const int32_t *lg, const double *b1, const double *b2,
-                  const double *b3, int g0, double wd, double delt, ro_z w,
+                  const double *b3, int g0, double wd, double delt, double w,
                   const ro_z *chi, ro_z *epsinv);
 
 /*

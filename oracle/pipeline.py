@@ -169,14 +169,13 @@ def run_q(w, q, nthreads=0, prob=None, **kw):
     eps = np.zeros_like(chi)
     for ix in range(chi.shape[0]):
         for ie in range(nen):
-            c = np.ascontiguousarray(chi[ix, ie])  # [jg][ig] == Fortran (ig,jg)
+            c = np.array(chi[ix, ie], order="C")  # copy; [jg][ig] == Fortran (ig,jg)
             e = np.zeros_like(c)
             if not q0:
                 rc = L.ro_eps_rpa(ng, ptr(lg), ptr(sq), ptr(b[0]), ptr(b[1]), ptr(b[2]), ptr(c), ptr(e))
             else:
-                w2 = (C.c_double * 2)(em[ie].real, em[ie].imag)
                 rc = L.ro_eps_rpa_q0(ng, ptr(lg), ptr(b[0]), ptr(b[1]), ptr(b[2]), g0, 0.0,
-                                     s["delt"], w2, ptr(c), ptr(e))
+                                     s["delt"], float(em[ie].real), ptr(c), ptr(e))
             assert rc == 0
             eps[ix, ie] = e
     return dict(chi=chi, epsinv=eps, m0=m0, x0=x0, lg=lg, em=em, npair=npair, pocc=pocc,
