@@ -198,6 +198,7 @@ inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
 // ------------------------------------------------------------------ context
 struct rp_ctx {
   int device = 0;
+  int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
   // system
@@ -230,6 +231,8 @@ struct rp_ctx {
   double prep_seconds = 0;
   DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv;
   DevBuf<float2> d_A, d_A3;
+  DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand
+  DevBuf<float2> d_wf;         // single-precision copy of the weights
   DevBuf<double2> d_w, d_part, d_chi, d_eps, d_rowc, d_colc;
   DevBuf<double> d_qn, d_wd;
 
@@ -259,6 +262,7 @@ int rp_init(int32_t device, rp_handle* out) {
              std::string("rp_init: built for gfx950, device is ") + prop.gcnArchName);
     auto* h = new rp_ctx();
     h->device = device;
+    h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     RP_HIP(hipStreamCreate(&h->stream));
     for (auto& e : h->ev) RP_HIP(hipEventCreate(&e));
     h->d_status.reserve(1);
@@ -685,7 +689,7 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
                  (q[2] + (double)h->lg[3 * i + 2]) * h->b[2][d];
         h->qn[i] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       }
-      h->lda = roundup((size_t)h->ng, (size_t)chi_tile());
+      h->lda = roundup((size_t)h->ng, std::max((size_t)chi_tile(), bf16x3_lda_multiple()));
       const size_t K = (size_t)h->npair * ntk;
       h->kpad = roundup(K, (size_t)chi_kchunk());
       hipStream_t s = h->stream;
@@ -783,17 +787,43 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const int ng = h->ng, nen = h->nen, nc = h->q0 ? 3 : 1;
-    const int ntile = (ng + chi_tile() - 1) / chi_tile();
-    const size_t kchunks = h->kpad / chi_kchunk();
-    long wgs = (long)ntile * ntile * nen;
-    int nsplit = (int)std::min<long>((long)kchunks, std::max<long>(1, (2048 + wgs - 1) / wgs));
     const size_t per = (size_t)ng * ng * nen;
-    h->d_part.reserve((size_t)nsplit * per);
     h->d_chi.reserve((size_t)nc * per);
-    for (int ix = 0; ix < nc; ++ix) {
-      const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
-      launch_chi_gemm(precision, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, nsplit, h->d_part.p, s);
-      launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
+    if (precision == RP_PREC_BF16X3) {
+      // 128 x 128 tiles, one workgroup per CU at a time: split K (1..8 ways)
+      // to minimise the rounds of workgroups per unit of work (tail effect)
+      const int ntile = (ng + 127) / 128;
+      const size_t nchunks = h->kpad / 32;
+      const long wgs = (long)ntile * ntile * nen;
+      int nsplit = 1;
+      double best = 1e300;
+      for (int n = 1; n <= 8 && (size_t)n <= nchunks; ++n) {
+        const double rounds = (double)((wgs * n + h->ncu - 1) / h->ncu) / n;
+        if (rounds < best - 1e-9) { best = rounds; nsplit = n; }
+      }
+      h->d_part.reserve((size_t)nsplit * per);
+      h->d_Ao.reserve(h->kpad * h->lda * 8);
+      h->d_wf.reserve((size_t)nen * h->kpad);
+      launch_weights_to_f32(h->d_w.p, (size_t)nen * h->kpad, h->d_wf.p, s);
+      for (int ix = 0; ix < nc; ++ix) {
+        const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+        launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
+        RP_HIP(hipMemsetAsync(h->d_part.p, 0, sizeof(double2) * nsplit * per, s));
+        launch_chi_bf16x3(h->d_Ao.p, A, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit,
+                          h->d_part.p, s);
+        launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
+      }
+    } else {
+      const int ntile = (ng + chi_tile() - 1) / chi_tile();
+      const size_t kchunks = h->kpad / chi_kchunk();
+      long wgs = (long)ntile * ntile * nen;
+      int nsplit = (int)std::min<long>((long)kchunks, std::max<long>(1, (2048 + wgs - 1) / wgs));
+      h->d_part.reserve((size_t)nsplit * per);
+      for (int ix = 0; ix < nc; ++ix) {
+        const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+        launch_chi_gemm(precision, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, nsplit, h->d_part.p, s);
+        launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
+      }
     }
     if (chi_out)
       RP_HIP(hipMemcpyAsync(chi_out, h->d_chi.p, sizeof(double2) * nc * per, hipMemcpyDeviceToHost, s));

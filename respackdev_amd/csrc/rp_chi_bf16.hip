@@ -1,0 +1,278 @@
+// Split-BF16 chi contraction for gfx950 (the mixed-precision path).
+//
+// chi_w(G,G') = sum_K conj(a_K(G)) d_w(K) a_K(G'), the rank-K update of
+// chiqw.F:1397-1410, on v_mfma_f32_16x16x32_bf16 with every real operand split
+// into two bf16 terms x = hi + lo and the products hi*hi + hi*lo + lo*hi
+// (3 MFMAs per real product, FP32 accumulation).
+//
+// Operand format: a "packed octet" holds 8 consecutive K of one G as
+// 32 bf16 = [Re hi x8][Re lo x8][Im hi x8][Im lo x8] (64 bytes), exactly the
+// eight k-slots one MFMA lane supplies.  The left operand (conj(a), shared by
+// every frequency) is split once per q by k_presplit into octets in HBM; the
+// right operand d_w(K) a_K(G') is formed and split per workgroup while a chunk
+// is staged, once for all four waves.
+//
+// Workgroup: 512 threads = 2x4 waves (two per SIMD), tile 128 G x 128 G', one
+// frequency, one K range; each wave owns 4x2 blocks of 16x16.  Per chunk of 32
+// complex K the left tile (32 KB of octets) and the converted right tile
+// (32 KB) sit in LDS with rows padded by 16 bytes (conflict-free 16-byte
+// reads), double buffered: the next chunk's global data is loaded into
+// registers during the MFMAs and converted/stored after them, with one barrier
+// per chunk.
+// Every kFlush chunks the FP32 accumulators are added into the FP64 output
+// tile in HBM, which the workgroup owns exclusively (FP64 accumulation across
+// K blocks of 1024, FP32 within).
+#include <hip/hip_bfloat16.h>
+
+#include <algorithm>
+
+#include "rp_common.h"
+#include "rp_kernels.h"
+
+namespace rp {
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+
+namespace {
+constexpr int kTM = 128, kTN = 128;   // tile (G x G')
+constexpr int kKC = 32;               // complex K per chunk
+constexpr int kK8 = kKC / 8;          // octets per row per chunk
+constexpr int kRowBytes = kK8 * 64 + 16;  // padded LDS row (bytes)
+constexpr int kFlushChunks = 32;      // FP32 accumulation span: 1024 K
+}  // namespace
+
+__device__ __forceinline__ unsigned short bf16_bits(float x) {
+  // round to nearest even (the conversion __bf16(x) performs)
+  __bf16 h = (__bf16)x;
+  return __builtin_bit_cast(unsigned short, h);
+}
+__device__ __forceinline__ float bf16_to_f(unsigned short b) {
+  return __builtin_bit_cast(float, (unsigned int)b << 16);
+}
+
+// Splits 8 complex values into one packed octet (4 x uint4).
+__device__ __forceinline__ void make_octet(const float2* v, u4v* o) {
+  unsigned short rh[8], rl[8], ih[8], il[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    rh[e] = bf16_bits(v[e].x);
+    rl[e] = bf16_bits(v[e].x - bf16_to_f(rh[e]));
+    ih[e] = bf16_bits(v[e].y);
+    il[e] = bf16_bits(v[e].y - bf16_to_f(ih[e]));
+  }
+#pragma unroll
+  for (int w = 0; w < 4; ++w) {
+    o[0][w] = (unsigned)rh[2 * w] | ((unsigned)rh[2 * w + 1] << 16);
+    o[1][w] = (unsigned)rl[2 * w] | ((unsigned)rl[2 * w + 1] << 16);
+    o[2][w] = (unsigned)ih[2 * w] | ((unsigned)ih[2 * w + 1] << 16);
+    o[3][w] = (unsigned)il[2 * w] | ((unsigned)il[2 * w + 1] << 16);
+  }
+}
+
+// A[K][lda] complex64 -> octets Ao[K/8][lda] (4 x uint4 each).  One thread per
+// (octet row, G).
+__global__ void __launch_bounds__(256) k_presplit(const float2* __restrict__ A, size_t lda,
+                                                  size_t k8n, u4v* __restrict__ Ao) {
+  const size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (idx >= k8n * lda) return;
+  const size_t k8 = idx / lda, g = idx % lda;
+  float2 v[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) v[e] = A[(k8 * 8 + e) * lda + g];
+  u4v o[4];
+  make_octet(v, o);
+  u4v* dst = Ao + idx * 4;
+#pragma unroll
+  for (int w = 0; w < 4; ++w) dst[w] = o[w];
+}
+
+__device__ __forceinline__ bf8v as_bf8(u4v x) { return __builtin_bit_cast(bf8v, x); }
+
+// d (complex FP64, [nen][ldd]) -> single precision copy used by the contraction.
+__global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float2* __restrict__ df) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    df[i] = make_float2((float)dw[i].x, (float)dw[i].y);
+}
+
+// grid: (ntile*ntile, nen, nsplit), 512 threads (8 waves, two per SIMD).
+// out: FP64 partial [z][w][ng][ng] (element (G,G') at G' * ng + G), zeroed by
+// the caller.  Columns of A beyond ng are zero, so edge tiles run the full
+// MFMA schedule (no branches between MFMAs).
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t lda,
+             const float2* __restrict__ df, size_t ldd, size_t chunks_per_split, size_t nchunks,
+             int ng, int ntile, int nen, double2* __restrict__ out) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
+  constexpr int kBuf = (kTM + kTN) * kRowBytes;
+  const int tile = blockIdx.x;
+  const int g0 = (tile % ntile) * kTM, h0 = (tile / ntile) * kTN;
+  const int w = blockIdx.y, z = blockIdx.z;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: 64 G x 32 G' each
+  const int li = lane & 15, q = lane >> 4, part = q >> 1, hh = q & 1;
+  const float2* dww = df + (size_t)w * ldd;
+  const size_t c0 = (size_t)z * chunks_per_split;
+  size_t c1 = c0 + chunks_per_split;
+  if (c1 > nchunks) c1 = nchunks;
+
+  // Staging: thread t owns left octet (k8 = t>>7, row = t&127) and right
+  // octet (k8 = t>>7, col = t&127) of every chunk.
+  const int srow = t & 127, sk8 = t >> 7;
+  u4v pl[4];
+  float2 pr[8], pd[8];
+  auto load_chunk = [&](size_t c) {
+    const u4v* src = Ao + ((c * kK8 + sk8) * lda + g0 + srow) * 4;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) pl[x] = src[x];
+    const size_t kb = c * kKC + (size_t)sk8 * 8;
+    const float2* sa = A + kb * lda + h0 + srow;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      pr[e] = sa[(size_t)e * lda];
+      pd[e] = dww[kb + e];
+    }
+  };
+  auto stage = [&](unsigned char* buf) {
+    u4v* dl = (u4v*)(buf + srow * kRowBytes + sk8 * 64);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) dl[x] = pl[x];
+    float2 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      v[e] = make_float2(pd[e].x * pr[e].x - pd[e].y * pr[e].y, pd[e].x * pr[e].y + pd[e].y * pr[e].x);
+    u4v ov[4];
+    make_octet(v, ov);
+    u4v* dr = (u4v*)(buf + (kTM + srow) * kRowBytes + sk8 * 64);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) dr[x] = ov[x];
+  };
+
+  f4v acc_re[4][2], acc_im[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
+
+  double2* o = out + ((size_t)z * nen + w) * (size_t)ng * ng;
+  auto flush = [&]() {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int gi = g0 + wm * 64 + a * 16 + 4 * q + v;
+          const int hj = h0 + wn * 32 + b * 16 + li;
+          if (gi < ng && hj < ng) {
+            double2* p = o + (size_t)hj * ng + gi;
+            double2 cur = *p;
+            cur.x += (double)acc_re[a][b][v];
+            cur.y += (double)acc_im[a][b][v];
+            *p = cur;
+          }
+        }
+        acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
+      }
+  };
+
+  if (c0 < c1) {
+    load_chunk(c0);
+    stage(smem);
+  }
+  __syncthreads();
+  const u4v neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+  int since = 0;
+  for (size_t c = c0; c < c1; ++c) {
+    const int cur = (int)((c - c0) & 1);
+    const unsigned char* bcur = smem + cur * kBuf;
+    const bool more = c + 1 < c1;
+    if (more) load_chunk(c + 1);  // in flight during the MFMAs
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int k8 = 2 * s + hh;
+      bf8v Lh[4], Ll[4], Rrh[2], Rrl[2], Rih[2], Ril[2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const u4v* p = (const u4v*)(bcur + (wm * 64 + a * 16 + li) * kRowBytes + k8 * 64);
+        Lh[a] = as_bf8(p[2 * part]);
+        Ll[a] = as_bf8(p[2 * part + 1]);
+      }
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const u4v* p = (const u4v*)(bcur + (kTM + wn * 32 + b * 16 + li) * kRowBytes + k8 * 64);
+        const u4v rh = p[0], rl = p[1], ih = p[2], il = p[3];
+        // k-slots of the real part: (Re b | Im b); of the imaginary part: (Im b | -Re b)
+        Rrh[b] = as_bf8(part ? ih : rh);
+        Rrl[b] = as_bf8(part ? il : rl);
+        Rih[b] = as_bf8(part ? (rh ^ neg) : ih);
+        Ril[b] = as_bf8(part ? (rl ^ neg) : il);
+      }
+      // three passes over the 8 blocks, so consecutive MFMAs never share an accumulator
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrh[b], acc_re[a][b], 0, 0, 0);
+          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rih[b], acc_im[a][b], 0, 0, 0);
+        }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrl[b], acc_re[a][b], 0, 0, 0);
+          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Ril[b], acc_im[a][b], 0, 0, 0);
+        }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rrh[b], acc_re[a][b], 0, 0, 0);
+          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rih[b], acc_im[a][b], 0, 0, 0);
+        }
+    }
+    if (more) stage(smem + (cur ^ 1) * kBuf);
+    if (++since == kFlushChunks) {
+      since = 0;
+      flush();
+    }
+    __syncthreads();
+  }
+  if (since) flush();
+}
+
+size_t bf16x3_lda_multiple() { return kTM; }
+
+void launch_presplit(const float2* A, size_t lda, size_t kpad, void* Ao, hipStream_t s) {
+  const size_t k8n = kpad / 8;
+  const size_t n = k8n * lda;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_presplit, dim3(blocks), dim3(256), 0, s, A, lda, k8n, (u4v*)Ao);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_weights_to_f32(const double2* dw, size_t n, float2* df, hipStream_t s) {
+  unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_weights_to_f32, dim3(blocks), dim3(256), 0, s, dw, n, df);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2* df, size_t ldd,
+                       size_t kpad, int ng, int nen, int nsplit, double2* partial, hipStream_t s) {
+  RP_CHECK(lda % kTM == 0, "bf16x3 contraction needs lda a multiple of 128");
+  RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
+  const int ntile = (ng + kTM - 1) / kTM;
+  const size_t nchunks = kpad / kKC;
+  const size_t per = (nchunks + nsplit - 1) / nsplit;
+  const size_t lds = 2 * (size_t)(kTM + kTN) * kRowBytes;
+  RP_HIP(hipFuncSetAttribute((const void*)k_chi_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds));
+  dim3 grid(ntile * ntile, nen, nsplit);
+  hipLaunchKernelGGL(k_chi_bf16x3, grid, dim3(512), lds, s, (const u4v*)Ao, A, lda, df, ldd, per,
+                     nchunks, ng, ntile, nen, partial);
+  RP_HIP(hipGetLastError());
+}
+
+}  // namespace rp

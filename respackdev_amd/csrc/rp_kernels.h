@@ -37,6 +37,13 @@ void launch_chi_gemm(int prec, const float2* A, size_t lda, const double2* dw, s
 void launch_chi_reduce(const double2* partial, size_t n, int nsplit, double inv_vol, double2* chi,
                        hipStream_t s);
 
+// rp_chi_bf16.hip
+size_t bf16x3_lda_multiple();
+void launch_presplit(const float2* A, size_t lda, size_t kpad, void* Ao, hipStream_t s);
+void launch_weights_to_f32(const double2* dw, size_t n, float2* df, hipStream_t s);
+void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2* df, size_t ldd,
+                       size_t kpad, int ng, int nen, int nsplit, double2* partial, hipStream_t s);
+
 // rp_dyson.hip
 void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
                       int g0, const double* wd, const double2* em, double delt, double2* eps,
