@@ -10,70 +10,75 @@
 // (fft3_fw), read at -G for the chi sphere and rounded to single precision
 // (chiqw_sub.F:192-202): A[K][lda] complex64, K = pair*NTK + k.
 //
-// Both transforms are LDS-resident: the whole FP64 grid (<= 5000 points, two
-// ping-pong buffers) lives in one workgroup's LDS and each axis is done as
-// independent per-thread line DFTs (the axis lengths are the reference's
-// 2,3,5-smooth sizes, <= 32), so HBM sees one read of the inputs and one write of the
+// Both transforms are LDS-resident: the whole FP64 grid (<= 10000 points)
+// lives in one workgroup's LDS and each axis is done in place as independent
+// per-thread line DFTs (the axis lengths are the reference's 2,3,5-smooth
+// sizes, <= 32), so HBM sees one read of the inputs and one write of the
 // outputs per transform.
 #include "rp_common.h"
 #include "rp_kernels.h"
 
 namespace rp {
 
-// Largest grid the LDS-resident transform holds: two FP64 buffers (ping-pong)
-// plus twiddles within the 160 KB of LDS a gfx950 workgroup may use.
-constexpr int kMaxGrid = 5000;
+// Largest grid the LDS-resident transform holds (FP64 grid plus twiddles within
+// the 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
+constexpr int kMaxGrid = 10000;
 
-// Length-N DFT of one line from src into dst (both LDS):
-// dst[k] = sum_j src[j] w^{jk}, w = exp(-2 pi i / N) (sign < 0) or its
-// conjugate (sign > 0).  The N inputs stay in registers; outputs are streamed.
+// Length-N DFT of the lines of one axis, in place in LDS: out[k] = sum_j v[j]
+// w^{jk}, w = exp(-2 pi i / N) (sign < 0) or its conjugate (sign > 0).  Lines
+// are taken blockDim at a time: every thread loads its line into registers,
+// the block synchronises, then each thread writes its line's outputs.
 template <int N>
-__device__ __attribute__((always_inline)) inline void dft_line(const double2* src, double2* dst, int stride, int sign,
-                                const double2* tw) {
-  double2 v[N];
-#pragma unroll
-  for (int j = 0; j < N; ++j) v[j] = src[j * stride];
-#pragma unroll 1
-  for (int k = 0; k < N; ++k) {
-    double re = 0.0, im = 0.0;
-    int idx = 0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      const double2 w = tw[idx];
-      const double wi = sign > 0 ? -w.y : w.y;
-      re += v[j].x * w.x - v[j].y * wi;
-      im += v[j].x * wi + v[j].y * w.x;
-      idx += k;
-      if (idx >= N) idx -= N;
-    }
-    dst[k * stride] = make_double2(re, im);
-  }
-}
-
-template <int N>
-__device__ __attribute__((always_inline)) inline void dft_axis_n(const double2* src, double2* dst, int axis, int nx, int ny, int nz,
-                           int sign, const double2* tw) {
+__device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int axis, int nx, int ny,
+                                                                 int nz, int sign,
+                                                                 const double2* tw) {
   int nlines, stride;
   if (axis == 0) { nlines = ny * nz; stride = 1; }
   else if (axis == 1) { nlines = nx * nz; stride = nx; }
   else { nlines = nx * ny; stride = nx * ny; }
-  for (int l = threadIdx.x; l < nlines; l += blockDim.x) {
-    int base;
+  for (int l0 = 0; l0 < nlines; l0 += blockDim.x) {
+    const int l = l0 + (int)threadIdx.x;
+    const bool mine = l < nlines;
+    int base = 0;
     if (axis == 0) base = nx * l;
     else if (axis == 1) base = (l % nx) + nx * ny * (l / nx);
     else base = l;
-    dft_line<N>(src + base, dst + base, stride, sign, tw);
+    double2 v[N];
+    if (mine) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) v[j] = g[base + j * stride];
+    }
+    __syncthreads();
+    if (mine) {
+#pragma unroll 1
+      for (int k = 0; k < N; ++k) {
+        double re = 0.0, im = 0.0;
+        int idx = 0;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          const double2 w = tw[idx];
+          const double wi = sign > 0 ? -w.y : w.y;
+          re += v[j].x * w.x - v[j].y * wi;
+          im += v[j].x * wi + v[j].y * w.x;
+          idx += k;
+          if (idx >= N) idx -= N;
+        }
+        g[base + k * stride] = make_double2(re, im);
+      }
+    }
+    __syncthreads();
   }
 }
 
 // MAXN bounds the instantiated line lengths so that kernels for small grids
 // keep a small register footprint.
 template <int MAXN>
-__device__ __attribute__((always_inline)) inline void dft_axis(const double2* src, double2* dst, int axis, int n, int nx, int ny, int nz,
-                         int sign, const double2* tw) {
+__device__ __attribute__((always_inline)) inline void dft_axis(double2* g, int axis, int n, int nx,
+                                                               int ny, int nz, int sign,
+                                                               const double2* tw) {
   switch (n) {
 #define RP_CASE(N) \
-  case N: if constexpr (N <= MAXN) dft_axis_n<N>(src, dst, axis, nx, ny, nz, sign, tw); break;
+  case N: if constexpr (N <= MAXN) dft_axis_n<N>(g, axis, nx, ny, nz, sign, tw); break;
     RP_CASE(1) RP_CASE(2) RP_CASE(3) RP_CASE(4) RP_CASE(5) RP_CASE(6) RP_CASE(8)
     RP_CASE(9) RP_CASE(10) RP_CASE(12) RP_CASE(15) RP_CASE(16) RP_CASE(18) RP_CASE(20)
     RP_CASE(24) RP_CASE(25) RP_CASE(27) RP_CASE(30) RP_CASE(32)
@@ -102,16 +107,14 @@ __device__ inline void fill_twiddles(double2* tw, int n) {
 }
 
 // z, then y, then x (the order of fft3_fw/fft3_bw, m_fft3d_20150826.f90:1598-
-// 1720).  Input in a, result in b.
+// 1720), in place.  Ends synchronised.
 template <int MAXN>
-__device__ __attribute__((always_inline)) inline void dft3(double2* a, double2* b, int nx, int ny, int nz, int sign,
-                            const double2* twx, const double2* twy, const double2* twz) {
-  dft_axis<MAXN>(a, b, 2, nz, nx, ny, nz, sign, twz);
-  __syncthreads();
-  dft_axis<MAXN>(b, a, 1, ny, nx, ny, nz, sign, twy);
-  __syncthreads();
-  dft_axis<MAXN>(a, b, 0, nx, nx, ny, nz, sign, twx);
-  __syncthreads();
+__device__ __attribute__((always_inline)) inline void dft3(double2* g, int nx, int ny, int nz,
+                                                           int sign, const double2* twx,
+                                                           const double2* twy, const double2* twz) {
+  dft_axis<MAXN>(g, 2, nz, nx, ny, nz, sign, twz);
+  dft_axis<MAXN>(g, 1, ny, nx, ny, nz, sign, twy);
+  dft_axis<MAXN>(g, 0, nx, nx, ny, nz, sign, twx);
 }
 
 __device__ inline int wrap(int g, int n) { return ((g % n) + n) % n; }
@@ -126,8 +129,7 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  double2* h = lds + nr;
-  double2* twx = h + nr;
+  double2* twx = g + nr;
   double2* twy = twx + nx;
   double2* twz = twy + ny;
   const int k = blockIdx.x, b = blockIdx.y;
@@ -144,10 +146,10 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
     g[r] = c[i];
   }
   __syncthreads();
-  dft3<MAXN>(g, h, nx, ny, nz, +1, twx, twy, twz);
+  dft3<MAXN>(g, nx, ny, nz, +1, twx, twy, twz);
   float2* out = ur + ((size_t)b * ntk + k) * nr;
   for (int r = threadIdx.x; r < nr; r += blockDim.x)
-    out[r] = make_float2((float)h[r].x, (float)h[r].y);  // CMPLX(): single precision
+    out[r] = make_float2((float)g[r].x, (float)g[r].y);  // CMPLX(): single precision
 }
 
 // grid: K = npair*ntk blocks (flattened).  u_occ = ur of band pocc[p] at k,
@@ -162,8 +164,7 @@ __global__ void __launch_bounds__(256) k_pair_densities(
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  double2* h = lds + nr;
-  double2* twx = h + nr;
+  double2* twx = g + nr;
   double2* twy = twx + nx;
   double2* twz = twy + ny;
   const int K = blockIdx.x + k_offset;
@@ -191,13 +192,13 @@ __global__ void __launch_bounds__(256) k_pair_densities(
     g[r] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
   }
   __syncthreads();
-  dft3<MAXN>(g, h, nx, ny, nz, -1, twx, twy, twz);
+  dft3<MAXN>(g, nx, ny, nz, -1, twx, twy, twz);
   const double scl = 1.0 / (double)nr;
   float2* out = A + (size_t)K * lda;
   for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) {
     if (i < ng) {
       int r = wrap(-lg[3 * i], nx) + nx * (wrap(-lg[3 * i + 1], ny) + ny * wrap(-lg[3 * i + 2], nz));
-      double2 v = h[r];
+      double2 v = g[r];
       out[i] = make_float2((float)(v.x * scl), (float)(v.y * scl));
     } else {
       out[i] = make_float2(0.f, 0.f);
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(64) k_q0_heads(
 
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                        int nband, int nx, int ny, int nz, float2* ur, hipStream_t s) {
-  size_t lds = sizeof(double2) * (2 * (size_t)nx * ny * nz + nx + ny + nz);
+  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
   const bool small = nx <= 16 && ny <= 16 && nz <= 16;
   const void* fn = small ? (const void*)k_band_grids<16> : (const void*)k_band_grids<32>;
   if (lds > 65536)
@@ -276,7 +277,7 @@ void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int
                            const int* ikq, const int* shift, int nx, int ny, int nz,
                            const int* lg, int ng, size_t lda, int nK, float2* A,
                            hipStream_t s) {
-  size_t lds = sizeof(double2) * (2 * (size_t)nx * ny * nz + nx + ny + nz);
+  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
   const bool small = nx <= 16 && ny <= 16 && nz <= 16;
   const void* fn = small ? (const void*)k_pair_densities<16> : (const void*)k_pair_densities<32>;
   if (lds > 65536)
