@@ -465,7 +465,7 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     for (int i = 0; i < 3; ++i)
       for (int d = 0; d < 3; ++d) bflat[3 * i + d] = h->b[i][d];
     mkidx(n1, n2, n3, bflat, h->imt.data());
-    RP_CHECK(tetra_lds_bytes(ntk, h->nen) <= 160 * 1024,
+    RP_CHECK(tetra_lds_bytes(ntk, 1) <= 160 * 1024,
              "k mesh x frequencies too large for the LDS accumulator of the tetrahedron kernel");
     // ---- G lists of the full mesh (chiqw.F:509-545)
     h->ng0.assign(ntk, 0);

@@ -22,7 +22,8 @@ void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk
                      float2* A3, hipStream_t s);
 
 // rp_tetra.hip
-size_t tetra_lds_bytes(int nk3, int nen);
+size_t tetra_lds_bytes(int nk3, int nf);
+int tetra_freqs_per_block(int nk3, int nen);
 void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const int* index_kpt,
                           const int* pocc, const int* pvir, const int* ikq, const double* eband,
                           const double2* pband, double ef, double delt, double dmna, double dmnr,
