@@ -229,11 +229,11 @@ struct rp_ctx {
   std::vector<double> qn;
   size_t lda = 0, kpad = 0;
   double prep_seconds = 0;
-  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv, d_gjwork;
   DevBuf<float2> d_A, d_A3;
   DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand
   DevBuf<float2> d_wf;         // single-precision copy of the weights
-  DevBuf<double2> d_w, d_part, d_chi, d_eps, d_rowc, d_colc;
+  DevBuf<double2> d_w, d_part, d_chi, d_eps;
   DevBuf<double> d_qn, d_wd;
 
   ~rp_ctx() {
@@ -840,8 +840,7 @@ int rp_dyson(rp_handle h, double* epsinv_out) {
     const int n = h->ng, nb = h->nbatch();
     const size_t per = (size_t)n * n;
     h->d_eps.reserve((size_t)nb * per);
-    h->d_rowc.reserve((size_t)nb * n);
-    h->d_colc.reserve((size_t)nb * n);
+    h->d_gjwork.reserve(gauss_jordan_work_ints(nb));
     h->d_ipiv.reserve((size_t)nb * n);
     h->d_wd.reserve(3);
     const double wd0[3] = {0, 0, 0};  // no intraband pairs (cRPA, or gapped fRPA): wpl = 0
@@ -849,7 +848,7 @@ int rp_dyson(rp_handle h, double* epsinv_out) {
     RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
     launch_eps_build(h->d_chi.p, h->d_qn.p, n, nb, h->nen, h->q0 ? 1 : 0, h->g0, h->d_wd.p, h->d_em.p,
                      h->delt, h->d_eps.p, s);
-    launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_rowc.p, h->d_colc.p, h->d_status.p, s);
+    launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_gjwork.p, h->d_status.p, s);
     check_status(h, "Dyson step");
     if (epsinv_out)
       RP_HIP(hipMemcpyAsync(epsinv_out, h->d_eps.p, sizeof(double2) * nb * per, hipMemcpyDeviceToHost, s));

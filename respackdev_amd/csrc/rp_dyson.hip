@@ -4,10 +4,12 @@
 //
 // The inversion is Gauss-Jordan with partial pivoting by |Re|+|Im| (the pivot
 // rule of LAPACK's izamax in zgetrf, which the reference calls through
-// invZGE.F90:1-23).  Per pivot column one small kernel per matrix picks the
-// pivot, swaps rows and scales the pivot row; one batched elementwise kernel
-// applies the rank-1 elimination to every matrix; a final kernel undoes the
-// row interchanges as column swaps.
+// invZGE.F90:1-23), blocked by panels of up to 16 columns: one workgroup per
+// matrix factors a panel in LDS, one batched kernel applies the panel's
+// rank-nb update to all other columns, and a final kernel undoes the row
+// interchanges as column swaps.
+#include <algorithm>
+
 #include "rp_common.h"
 #include "rp_kernels.h"
 
@@ -57,78 +59,172 @@ __global__ void k_eps_build(const double2* __restrict__ chi, const double* __res
 
 __device__ inline double cabs1(double2 z) { return fabs(z.x) + fabs(z.y); }
 
-// One workgroup per matrix: pivot for column c, row swap, pivot-row scaling.
-__global__ void __launch_bounds__(256) k_gj_pivot(double2* __restrict__ a, int n, int c,
-                                                  int* __restrict__ ipiv, double2* __restrict__ rowc,
-                                                  double2* __restrict__ colc,
-                                                  int* __restrict__ status) {
-  const int b = blockIdx.x;
+// Largest panel width; panel rows moved by the pivoting of one panel: at most
+// the nb panel rows plus nb others.
+constexpr int kPanelMax = 16;
+constexpr int kMoveMax = 2 * kPanelMax;
+// Per matrix the panel kernel leaves {count, dst[kMoveMax], src[kMoveMax]}.
+constexpr int kMoveInts = 1 + 2 * kMoveMax;
+
+// Blocked Gauss-Jordan.  The nb pivot steps of panel [c0, c0+nb) are
+// elementary transforms E_k that differ from the identity only in column k;
+// their product differs from the identity only in the panel columns, and
+// running the in-place Gauss-Jordan steps on the panel columns alone leaves
+// exactly those columns W in the panel.  Every other column j then becomes
+//   a'(:,j) = P a(:,j) with the panel rows zeroed + W (P a)(panel rows, j),
+// P the panel's row interchanges: one rank-nb update.
+
+// One workgroup per matrix: the panel (n x nb, FP64 complex) is factored in
+// LDS; row interchanges are applied to the panel and recorded as a move list
+// for the update kernel.  LDS: nb*n double2 + n ints.
+__global__ void __launch_bounds__(1024) k_gj_panel(double2* __restrict__ a, int n, int c0, int nb,
+                                                   int* __restrict__ ipiv, int* __restrict__ moves,
+                                                   int* __restrict__ status) {
+  extern __shared__ double2 lds[];
+  double2* P = lds;                      // P[l*n + i] = a(i, c0+l)
+  int* orig = (int*)(P + (size_t)nb * n);  // orig[i]: original row now in row i
+  __shared__ double wv[16];
+  __shared__ int wi[16];
+  __shared__ double2 prow[kPanelMax];
+  __shared__ int nmove;
+  __shared__ int singular;
+  const int b = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
   double2* m = a + (size_t)b * n * n;
-  __shared__ double sv[256];
-  __shared__ int si[256];
-  double best = -1.0;
-  int bi = n;
-  for (int i = c + threadIdx.x; i < n; i += blockDim.x) {
-    double v = cabs1(m[i + (size_t)n * c]);
-    if (v > best) { best = v; bi = i; }
-  }
-  sv[threadIdx.x] = best;
-  si[threadIdx.x] = bi;
+  for (int l = 0; l < nb; ++l)
+    for (int i = tid; i < n; i += nt) P[(size_t)l * n + i] = m[i + (size_t)n * (c0 + l)];
+  for (int i = tid; i < n; i += nt) orig[i] = i;
+  if (tid == 0) { nmove = 0; singular = 0; }
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      double v2 = sv[threadIdx.x + w];
-      int i2 = si[threadIdx.x + w];
-      if (v2 > sv[threadIdx.x] || (v2 == sv[threadIdx.x] && i2 < si[threadIdx.x])) {
-        sv[threadIdx.x] = v2;
-        si[threadIdx.x] = i2;
+  for (int l = 0; l < nb; ++l) {
+    const int c = c0 + l;
+    const double2* col = P + (size_t)l * n;
+    // pivot: largest |Re|+|Im| in rows c..n-1, first index on ties (izamax)
+    double best = -1.0;
+    int bi = n;
+    for (int i = c + tid; i < n; i += nt) {
+      const double v = cabs1(col[i]);
+      if (v > best) { best = v; bi = i; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double v2 = __shfl_xor(best, off);
+      const int i2 = __shfl_xor(bi, off);
+      if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+    }
+    if ((tid & 63) == 0) { wv[tid >> 6] = best; wi[tid >> 6] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < nt / 64; ++w)
+        if (wv[w] > best || (wv[w] == best && wi[w] < bi)) { best = wv[w]; bi = wi[w]; }
+      wi[0] = bi;
+      ipiv[(size_t)b * n + c] = bi;
+      const int t = orig[c];
+      orig[c] = orig[bi];
+      orig[bi] = t;
+    }
+    __syncthreads();
+    const int p = wi[0];
+    if (p != c && tid < nb) {
+      double2* cl = P + (size_t)tid * n;
+      const double2 t = cl[c];
+      cl[c] = cl[p];
+      cl[p] = t;
+    }
+    __syncthreads();
+    const double2 piv = col[c];
+    if (piv.x == 0.0 && piv.y == 0.0) {
+      if (tid == 0) { atomicOr(status, 4); singular = 1; }
+      break;  // uniform: every thread reads the same pivot
+    }
+    // pivot row c scaled (its column l becomes 1/piv)
+    if (tid < nb) {
+      const zd pinv = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
+      const double2 v = P[(size_t)tid * n + c];
+      const zd u = tid == l ? pinv : mkz(v.x, v.y) * pinv;
+      prow[tid] = make_double2(u.x, u.y);
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += nt) {
+      if (i == c) {
+        for (int q = 0; q < nb; ++q) P[(size_t)q * n + i] = prow[q];
+      } else {
+        const zd t = mkz(col[i].x, col[i].y);
+        for (int q = 0; q < nb; ++q) {
+          double2* e = P + (size_t)q * n + i;
+          const zd v = q == l ? mkz(0.0, 0.0) : mkz(e->x, e->y);
+          const zd u = v - t * mkz(prow[q].x, prow[q].y);
+          *e = make_double2(u.x, u.y);
+        }
       }
     }
     __syncthreads();
   }
-  const int p = si[0];
-  if (threadIdx.x == 0) ipiv[(size_t)b * n + c] = p;
-  if (p != c)
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      double2 t = m[c + (size_t)n * j];
-      m[c + (size_t)n * j] = m[p + (size_t)n * j];
-      m[p + (size_t)n * j] = t;
+  for (int l = 0; l < nb; ++l)
+    for (int i = tid; i < n; i += nt) m[i + (size_t)n * (c0 + l)] = P[(size_t)l * n + i];
+  // move list: the nb panel rows first (in order), then other rows whose
+  // content changed
+  int* mv = moves + (size_t)b * kMoveInts;
+  if (tid < nb) {
+    mv[1 + tid] = c0 + tid;
+    mv[1 + kMoveMax + tid] = orig[c0 + tid];
+  }
+  for (int i = tid; i < n; i += nt) {
+    if ((i < c0 || i >= c0 + nb) && orig[i] != i) {
+      const int e = nb + atomicAdd(&nmove, 1);
+      mv[1 + e] = i;
+      mv[1 + kMoveMax + e] = orig[i];
     }
+  }
   __syncthreads();
-  const double2 piv = m[c + (size_t)n * c];
-  if (piv.x == 0.0 && piv.y == 0.0) {
-    if (threadIdx.x == 0) atomicOr(status, 4);
-    return;
-  }
-  const zd pinv = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
-  double2* rc = rowc + (size_t)b * n;
-  double2* cc = colc + (size_t)b * n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    zd v = j == c ? pinv : mkz(m[c + (size_t)n * j].x, m[c + (size_t)n * j].y) * pinv;
-    rc[j] = make_double2(v.x, v.y);
-    m[c + (size_t)n * j] = rc[j];
-  }
-  for (int i = threadIdx.x; i < n; i += blockDim.x)
-    cc[i] = i == c ? make_double2(0.0, 0.0) : m[i + (size_t)n * c];
+  if (tid == 0) mv[0] = singular ? -1 : nb + nmove;
 }
 
-// a[i][j] -= colc[i] rowc[j] for i != c; a[i][c] = -colc[i] rowc[c].
-// grid: (ceil(n/256), n, nbatch)
-__global__ void __launch_bounds__(256) k_gj_update(double2* __restrict__ a, int n, int c,
-                                                   const double2* __restrict__ rowc,
-                                                   const double2* __restrict__ colc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  const int b = blockIdx.z;
-  if (i >= n || i == c) return;
-  const double2 f = colc[(size_t)b * n + i];
-  const double2 r = rowc[(size_t)b * n + j];
-  double2* e = a + ((size_t)b * n + j) * n + i;
-  if (j == c) {
-    *e = make_double2(-(f.x * r.x - f.y * r.y), -(f.x * r.y + f.y * r.x));
-  } else {
-    double2 v = *e;
-    *e = make_double2(v.x - (f.x * r.x - f.y * r.y), v.y - (f.x * r.y + f.y * r.x));
+// Rank-nb update of every column outside the panel.  grid: (ceil((n-nb)/TJ),
+// nbatch), 256 threads; workgroup = TJ whole columns of one matrix, so the
+// reads of moved rows (staged in LDS before the barrier) never race with
+// another workgroup's writes.
+constexpr int kTJ = 16;
+__global__ void __launch_bounds__(256) k_gj_update(double2* __restrict__ a, int n, int c0, int nb,
+                                                   const int* __restrict__ moves) {
+  __shared__ double2 vals[kMoveMax][kTJ];
+  __shared__ int dst[kMoveMax];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  double2* m = a + (size_t)b * n * n;
+  const int* mv = moves + (size_t)b * kMoveInts;
+  const int nm = mv[0];
+  if (nm < 0) return;  // singular pivot: the host reports it
+  const int nrest = n - nb;
+  const int jt0 = blockIdx.x * kTJ;
+  const int ntj = min(kTJ, nrest - jt0);
+  if (tid < nm) dst[tid] = mv[1 + tid];
+  for (int e = tid; e < nm * kTJ; e += blockDim.x) {
+    const int mi = e / kTJ, t = e % kTJ;
+    if (t < ntj) {
+      const int jj = jt0 + t, j = jj < c0 ? jj : jj + nb;
+      vals[mi][t] = m[mv[1 + kMoveMax + mi] + (size_t)n * j];
+    }
+  }
+  __syncthreads();
+  const double2* W = m + (size_t)n * c0;
+  for (int i = tid; i < n; i += blockDim.x) {
+    zd w[kPanelMax];
+#pragma unroll
+    for (int q = 0; q < kPanelMax; ++q)
+      if (q < nb) w[q] = mkz(W[i + (size_t)n * q].x, W[i + (size_t)n * q].y);
+    const bool inpanel = i >= c0 && i < c0 + nb;
+    int slot = -1;
+    if (!inpanel)
+      for (int e = nb; e < nm; ++e)
+        if (dst[e] == i) slot = e;
+    for (int t = 0; t < ntj; ++t) {
+      const int jj = jt0 + t, j = jj < c0 ? jj : jj + nb;
+      double2* out = m + i + (size_t)n * j;
+      zd acc = inpanel ? mkz(0.0, 0.0)
+                       : (slot >= 0 ? mkz(vals[slot][t].x, vals[slot][t].y) : mkz(out->x, out->y));
+#pragma unroll
+      for (int q = 0; q < kPanelMax; ++q)
+        if (q < nb) acc = acc + w[q] * mkz(vals[q][t].x, vals[q][t].y);
+      *out = make_double2(acc.x, acc.y);
+    }
   }
 }
 
@@ -139,7 +235,7 @@ __global__ void __launch_bounds__(256) k_gj_colswap(double2* __restrict__ a, int
   double2* m = a + (size_t)b * n * n;
   for (int c = n - 1; c >= 0; --c) {
     const int p = ipiv[(size_t)b * n + c];
-    if (p != c)
+    if (p != c && p >= 0 && p < n)  // a singular matrix leaves later entries unset
       for (int i = threadIdx.x; i < n; i += blockDim.x) {
         double2 t = m[i + (size_t)n * c];
         m[i + (size_t)n * c] = m[i + (size_t)n * p];
@@ -157,12 +253,28 @@ void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, i
   RP_HIP(hipGetLastError());
 }
 
-void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, double2* rowc, double2* colc,
-                         int* status, hipStream_t s) {
-  for (int c = 0; c < n; ++c) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(nbatch), dim3(256), 0, s, a, n, c, ipiv, rowc, colc, status);
-    hipLaunchKernelGGL(k_gj_update, dim3((n + 255) / 256, n, nbatch), dim3(256), 0, s, a, n, c,
-                       (const double2*)rowc, (const double2*)colc);
+size_t gauss_jordan_work_ints(int nbatch) { return (size_t)nbatch * kMoveInts; }
+
+void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
+                         hipStream_t s) {
+  // panel width: as many columns as fit in LDS next to the row map, at most 16
+  const size_t budget = 160 * 1024 - 1024 - sizeof(int) * (size_t)n;
+  int nb = (int)std::min<size_t>(kPanelMax, budget / (sizeof(double2) * (size_t)n));
+  RP_CHECK(nb >= 1, "Dyson: matrix too large for the LDS panel");
+  int maxlds = 0;
+  for (int c0 = 0; c0 < n; c0 += nb) {
+    const int w = std::min(nb, n - c0);
+    const size_t lds = sizeof(double2) * (size_t)w * n + sizeof(int) * (size_t)n;
+    if ((int)lds > maxlds && lds > 65536) {
+      RP_HIP(hipFuncSetAttribute((const void*)k_gj_panel,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      maxlds = (int)lds;
+    }
+    hipLaunchKernelGGL(k_gj_panel, dim3(nbatch), dim3(1024), lds, s, a, n, c0, w, ipiv, work,
+                       status);
+    if (n - w > 0)
+      hipLaunchKernelGGL(k_gj_update, dim3((n - w + kTJ - 1) / kTJ, nbatch), dim3(256), 0, s, a, n,
+                         c0, w, (const int*)work);
   }
   hipLaunchKernelGGL(k_gj_colswap, dim3(nbatch), dim3(256), 0, s, a, n, (const int*)ipiv);
   RP_HIP(hipGetLastError());

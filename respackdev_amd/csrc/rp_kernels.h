@@ -49,7 +49,8 @@ void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2
 void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
                       int g0, const double* wd, const double2* em, double delt, double2* eps,
                       hipStream_t s);
-void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, double2* rowc, double2* colc,
-                         int* status, hipStream_t s);
+size_t gauss_jordan_work_ints(int nbatch);
+void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
+                         hipStream_t s);
 
 }  // namespace rp
