@@ -27,6 +27,37 @@ sys.path.insert(0, ROOT)
 
 METRIC = "chi(q,w) s per q-point: mixed/batched vs plain FP64 GPU at <=1e-5 rel err"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}  # dense MFMA, MI355X
+KERNEL = {"fp64": "k_chi_gemm<0>", "fp32": "k_chi_gemm<1>", "bf16x3": "k_chi_bf16x3"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+
+
+def traffic_per_launch(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (lines 'traffic <workload> <kernel> <bytes>'),
+    or None when it has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                p = line.split()
+                if len(p) == 4 and p[0] == "traffic" and p[1] == workload and p[2] == kernel:
+                    return float(p[3])
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe(mib: int, reps: int) -> int:
+    """Elementwise copy of `mib` MiB (read mib, write mib) for calibrating the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    log(f"copy probe: {reps} x {mib} MiB read + {mib} MiB written")
+    return 0
 
 
 def log(msg: str) -> None:
@@ -86,11 +117,15 @@ def main() -> int:
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="run a single precision (profiling)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run an elementwise copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(q-points would shard with no communication; see DESIGN.md)", file=sys.stderr)
+              "(q-points would shard across GPUs with no communication)", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        return copy_probe(args.copy_probe, 3)
 
     from respackdev_amd import gen
     from respackdev_amd.chiqw import Chiqw, from_system
@@ -158,18 +193,25 @@ def main() -> int:
         out["chi_rel_err_vs_fp64"] = rel(res["chi"], results["fp64"]["chi"])
         out["epsinv_rel_err_vs_fp64"] = rel(res["eps"], results["fp64"]["eps"])
     out["stage_s"] = {p: results[p]["stages"] for p in results}
-    t_gemm = res["stages"]["contraction"]
-    achieved = flops / t_gemm
-    rf = {"kernel": f"k_chi_gemm<{main_prec}>", "bound": "mfma", "achieved": achieved / 1e12,
+    # dominant kernel: the main contraction launch, timed with HIP events on the
+    # library's stream around each launch (one launch per q away from Gamma)
+    st = res["stages"]
+    t_launch = st["gemm_kernel"] / max(1, st["gemm_launches"])
+    achieved = flops / t_launch
+    rf = {"kernel": KERNEL[main_prec], "bound": "mfma", "achieved": achieved / 1e12,
           "peak": PEAK[main_prec] / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK[main_prec],
-          "traffic": None, "algorithmic_flops_per_launch": flops}
+          "traffic": traffic_per_launch(args.workload, KERNEL[main_prec]),
+          "algorithmic_flops_per_launch": flops, "avg_launch_ms": t_launch * 1e3}
     if main_prec == "bf16x3":
+        # three bf16 products per real product: executed MFMA work against the same peak
         rf["executed_frac"] = 3.0 * achieved / PEAK[main_prec]
     out["roofline"] = rf
     if "fp64" in results:
-        a64 = flops / results["fp64"]["stages"]["contraction"]
-        out["roofline_fp64"] = {"achieved": a64 / 1e12, "peak": PEAK["fp64"] / 1e12,
-                                "unit": "TFLOP/s", "frac": a64 / PEAK["fp64"]}
+        s64 = results["fp64"]["stages"]
+        a64 = flops / (s64["gemm_kernel"] / max(1, s64["gemm_launches"]))
+        out["roofline_fp64"] = {"kernel": KERNEL["fp64"], "achieved": a64 / 1e12,
+                                "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                                "frac": a64 / PEAK["fp64"]}
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline: oracle on {args.cpu_pairs} pairs with {threads} threads")

@@ -121,8 +121,10 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out);
 /* Dyson step calc_eps_rpa / calc_eps_rpa_q_0 (chiqw_sub.F:386-568) on the GPU
  * in FP64: eps^-1, Fortran order (ig,jg,ie) (times 3 at q = 0). */
 int rp_dyson(rp_handle h, double* epsinv_out);
-/* All four stages for one q.  timings (optional, 6 doubles, seconds): pair
- * densities, weights, contraction, Dyson, total device time, host prepare. */
+/* All four stages for one q.  timings (optional, 8 doubles): seconds of pair
+ * densities, weights, contraction, Dyson, total device time, host prepare,
+ * the main contraction kernel alone (summed over its launches), and the
+ * number of those launches (1, or 3 at q = 0). */
 int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_out,
                double* chi_out, double* timings);
 /* Device-resident number of pairs (after the Max_excitation_energy filter). */

@@ -236,9 +236,9 @@ class Chiqw:
         nen, ng = self.info["nen"], self.ng
         eps = np.zeros((self.ndir, nen, ng, ng), np.complex128)
         chi = np.zeros((self.ndir, nen, ng, ng), np.complex128) if want_chi else None
-        t = np.zeros(6)
+        t = np.zeros(8)
         qq = np.ascontiguousarray(q, np.float64)
         check(_lib.lib().rp_chiqw_q(self._h, ptr(qq), _lib.PREC[precision], ptr(eps), ptr(chi), ptr(t)))
         return dict(lg=lg, epsinv=eps, chi=chi, timings=dict(
             pairdens=t[0], weights=t[1], contraction=t[2], dyson=t[3], device_total=t[4],
-            host_prepare=t[5]))
+            host_prepare=t[5], gemm_kernel=t[6], gemm_launches=int(t[7])))

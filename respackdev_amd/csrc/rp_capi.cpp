@@ -201,6 +201,9 @@ struct rp_ctx {
   int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {};
+  hipEvent_t gev[6] = {};  // around each main contraction launch (<= 3 directions)
+  double gemm_seconds = 0;
+  int gemm_launches = 0;
   // system
   bool open = false;
   double a[3][3], b[3][3], volume = 0;
@@ -239,6 +242,8 @@ struct rp_ctx {
   ~rp_ctx() {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : gev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int nbatch() const { return q0 ? 3 * nen : nen; }
@@ -265,6 +270,7 @@ int rp_init(int32_t device, rp_handle* out) {
     h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     RP_HIP(hipStreamCreate(&h->stream));
     for (auto& e : h->ev) RP_HIP(hipEventCreate(&e));
+    for (auto& e : h->gev) RP_HIP(hipEventCreate(&e));
     h->d_status.reserve(1);
     *out = h;
   });
@@ -809,8 +815,10 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
         const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
         launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
         RP_HIP(hipMemsetAsync(h->d_part.p, 0, sizeof(double2) * nsplit * per, s));
+        RP_HIP(hipEventRecord(h->gev[2 * ix], s));
         launch_chi_bf16x3(h->d_Ao.p, A, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit,
                           h->d_part.p, s);
+        RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
         launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
       }
     } else {
@@ -821,13 +829,22 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
       h->d_part.reserve((size_t)nsplit * per);
       for (int ix = 0; ix < nc; ++ix) {
         const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+        RP_HIP(hipEventRecord(h->gev[2 * ix], s));
         launch_chi_gemm(precision, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, nsplit, h->d_part.p, s);
+        RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
         launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
       }
     }
     if (chi_out)
       RP_HIP(hipMemcpyAsync(chi_out, h->d_chi.p, sizeof(double2) * nc * per, hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
+    h->gemm_seconds = 0;
+    h->gemm_launches = nc;
+    for (int ix = 0; ix < nc; ++ix) {
+      float ms;
+      RP_HIP(hipEventElapsedTime(&ms, h->gev[2 * ix], h->gev[2 * ix + 1]));
+      h->gemm_seconds += ms * 1e-3;
+    }
     h->have_chi = true;
   });
 }
@@ -881,6 +898,8 @@ int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_o
       RP_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[4]));
       timings[4] = ms * 1e-3;
       timings[5] = h->prep_seconds;
+      timings[6] = h->gemm_seconds;
+      timings[7] = h->gemm_launches;
     }
   });
 }
