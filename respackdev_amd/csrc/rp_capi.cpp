@@ -796,17 +796,10 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
     const size_t per = (size_t)ng * ng * nen;
     h->d_chi.reserve((size_t)nc * per);
     if (precision == RP_PREC_BF16X3) {
-      // 128 x 128 tiles, one workgroup per CU at a time: split K (1..8 ways)
-      // to minimise the rounds of workgroups per unit of work (tail effect)
-      const int ntile = (ng + 127) / 128;
-      const size_t nchunks = h->kpad / 32;
-      const long wgs = (long)ntile * ntile * nen;
-      int nsplit = 1;
-      double best = 1e300;
-      for (int n = 1; n <= 8 && (size_t)n <= nchunks; ++n) {
-        const double rounds = (double)((wgs * n + h->ncu - 1) / h->ncu) / n;
-        if (rounds < best - 1e-9) { best = rounds; nsplit = n; }
-      }
+      // 128 x 128 tiles, one persistent workgroup per CU; K split so that the
+      // items fill every workgroup evenly
+      const int nwg = bf16x3_workgroups(h->ncu);
+      const int nsplit = bf16x3_nsplit(ng, nen, h->kpad, nwg);
       h->d_part.reserve((size_t)nsplit * per);
       h->d_Ao.reserve(h->kpad * h->lda * 8);
       h->d_wf.reserve((size_t)nen * h->kpad);
@@ -814,9 +807,8 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
       for (int ix = 0; ix < nc; ++ix) {
         const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
         launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
-        RP_HIP(hipMemsetAsync(h->d_part.p, 0, sizeof(double2) * nsplit * per, s));
         RP_HIP(hipEventRecord(h->gev[2 * ix], s));
-        launch_chi_bf16x3(h->d_Ao.p, A, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit,
+        launch_chi_bf16x3(h->d_Ao.p, A, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit, nwg,
                           h->d_part.p, s);
         RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
         launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);

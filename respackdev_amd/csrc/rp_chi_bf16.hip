@@ -19,9 +19,17 @@
 // reads), double buffered: the next chunk's global data is loaded into
 // registers during the MFMAs and converted/stored after them, with one barrier
 // per chunk.
-// Every kFlush chunks the FP32 accumulators are added into the FP64 output
-// tile in HBM, which the workgroup owns exclusively (FP64 accumulation across
-// K blocks of 1024, FP32 within).
+//
+// Schedule: one persistent workgroup per CU.  The (frequency, tile) pairs are
+// dealt round-robin to the XCDs (workgroup b runs on XCD b % 8), and each XCD
+// walks its pairs split by split: the 32 workgroups of an XCD work on the same
+// K range at the same time, so the operand chunks they share (5 row blocks and
+// 5 column blocks serve all 25 tiles at NG ~ 600) come from that XCD's L2
+// instead of HBM.
+// Every kFlushChunks chunks the FP32 accumulators are added into the FP64
+// partial tile of the item in HBM, which the workgroup owns exclusively (FP64
+// accumulation across K blocks of 4096, FP32 within; the first flush of an
+// item stores).
 #include <hip/hip_bfloat16.h>
 
 #include <algorithm>
@@ -40,7 +48,8 @@ constexpr int kTM = 128, kTN = 128;   // tile (G x G')
 constexpr int kKC = 32;               // complex K per chunk
 constexpr int kK8 = kKC / 8;          // octets per row per chunk
 constexpr int kRowBytes = kK8 * 64 + 16;  // padded LDS row (bytes)
-constexpr int kFlushChunks = 32;      // FP32 accumulation span: 1024 K
+constexpr int kFlushChunks = 128;     // FP32 accumulation span: 4096 K
+constexpr int kXcd = 8;               // XCDs of gfx950 (workgroup b -> XCD b % 8)
 }  // namespace
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -96,23 +105,32 @@ __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float
     df[i] = make_float2((float)dw[i].x, (float)dw[i].y);
 }
 
-// grid: (ntile*ntile, nen, nsplit), 512 threads (8 waves, two per SIMD).
-// out: FP64 partial [z][w][ng][ng] (element (G,G') at G' * ng + G), zeroed by
-// the caller.  Columns of A beyond ng are zero, so edge tiles run the full
-// MFMA schedule (no branches between MFMAs).
+// grid: persistent, a multiple of kXcd workgroups of 512 threads (8 waves, two
+// per SIMD).  Item (split z, frequency w, tile) accumulates K chunks
+// [z*chunks_per_split, ...) into out, the FP64 partial [z][w][ng][ng] (element
+// (G,G') at G' * ng + G); every element of it is stored by exactly one item.
+// Columns of A beyond ng are zero, so edge tiles run the full MFMA schedule
+// (no branches between MFMAs).
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t lda,
              const float2* __restrict__ df, size_t ldd, size_t chunks_per_split, size_t nchunks,
-             int ng, int ntile, int nen, double2* __restrict__ out) {
+             int ng, int ntile, int nen, int nsplit, double2* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
-  const int tile = blockIdx.x;
-  const int g0 = (tile % ntile) * kTM, h0 = (tile / ntile) * kTN;
-  const int w = blockIdx.y, z = blockIdx.z;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: 64 G x 32 G' each
   const int li = lane & 15, q = lane >> 4, part = q >> 1, hh = q & 1;
+  // this XCD's share of the (frequency, tile) pairs, walked split-major
+  const int ntiles = ntile * ntile, npairs = ntiles * nen;
+  const int xcd = (int)(blockIdx.x % kXcd), slot = (int)(blockIdx.x / kXcd);
+  const int nslot = (int)(gridDim.x / kXcd);
+  const int mypairs = (npairs - xcd + kXcd - 1) / kXcd;
+  const int nitems = mypairs * nsplit;
+  for (int item = slot; item < nitems; item += nslot) {
+  const int z = item / mypairs, u = (item % mypairs) * kXcd + xcd;
+  const int w = u / ntiles, tile = u % ntiles;
+  const int g0 = (tile % ntile) * kTM, h0 = (tile / ntile) * kTN;
   const float2* dww = df + (size_t)w * ldd;
   const size_t c0 = (size_t)z * chunks_per_split;
   size_t c1 = c0 + chunks_per_split;
@@ -157,6 +175,7 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
     for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
 
   double2* o = out + ((size_t)z * nen + w) * (size_t)ng * ng;
+  bool first = true;
   auto flush = [&]() {
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -168,7 +187,7 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
           const int hj = h0 + wn * 32 + b * 16 + li;
           if (gi < ng && hj < ng) {
             double2* p = o + (size_t)hj * ng + gi;
-            double2 cur = *p;
+            double2 cur = first ? make_double2(0.0, 0.0) : *p;
             cur.x += (double)acc_re[a][b][v];
             cur.y += (double)acc_im[a][b][v];
             *p = cur;
@@ -176,6 +195,7 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
         }
         acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
       }
+    first = false;
   };
 
   if (c0 < c1) {
@@ -240,7 +260,8 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
     }
     __syncthreads();
   }
-  if (since) flush();
+  if (since || first) flush();
+  }  // item
 }
 
 size_t bf16x3_lda_multiple() { return kTM; }
@@ -259,19 +280,39 @@ void launch_weights_to_f32(const double2* dw, size_t n, float2* df, hipStream_t 
   RP_HIP(hipGetLastError());
 }
 
+// Splits of K for the persistent schedule: the count (1..64) that minimises
+// the rounds of items per unit of work, given nwg workgroups (kXcd groups).
+int bf16x3_nsplit(int ng, int nen, size_t kpad, int nwg) {
+  const int ntile = (ng + kTM - 1) / kTM;
+  const long npairs = (long)ntile * ntile * nen;
+  const long mypairs = (npairs + kXcd - 1) / kXcd;
+  const long nslot = std::max(1, nwg / kXcd);
+  const size_t nchunks = kpad / kKC;
+  int best_s = 1;
+  double best = 1e300;
+  for (int sp = 1; sp <= 64 && (size_t)sp <= nchunks; ++sp) {
+    const double rounds = (double)((mypairs * sp + nslot - 1) / nslot) / sp;
+    if (rounds < best - 1e-9) { best = rounds; best_s = sp; }
+  }
+  return best_s;
+}
+
+int bf16x3_workgroups(int ncu) { return std::max(kXcd, ncu / kXcd * kXcd); }
+
 void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2* df, size_t ldd,
-                       size_t kpad, int ng, int nen, int nsplit, double2* partial, hipStream_t s) {
+                       size_t kpad, int ng, int nen, int nsplit, int nwg, double2* partial,
+                       hipStream_t s) {
   RP_CHECK(lda % kTM == 0, "bf16x3 contraction needs lda a multiple of 128");
   RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
+  RP_CHECK(nwg % kXcd == 0 && nwg > 0, "bf16x3 contraction needs a multiple of 8 workgroups");
   const int ntile = (ng + kTM - 1) / kTM;
   const size_t nchunks = kpad / kKC;
   const size_t per = (nchunks + nsplit - 1) / nsplit;
   const size_t lds = 2 * (size_t)(kTM + kTN) * kRowBytes;
   RP_HIP(hipFuncSetAttribute((const void*)k_chi_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  dim3 grid(ntile * ntile, nen, nsplit);
-  hipLaunchKernelGGL(k_chi_bf16x3, grid, dim3(512), lds, s, (const u4v*)Ao, A, lda, df, ldd, per,
-                     nchunks, ng, ntile, nen, partial);
+  hipLaunchKernelGGL(k_chi_bf16x3, dim3(nwg), dim3(512), lds, s, (const u4v*)Ao, A, lda, df, ldd, per,
+                     nchunks, ng, ntile, nen, nsplit, partial);
   RP_HIP(hipGetLastError());
 }
 
