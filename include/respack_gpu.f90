@@ -100,12 +100,13 @@ module respack_gpu
       integer(c_int32_t), value :: mb, nwf
       complex(c_double_complex) :: umat(mb, nwf, *)
     end function
+    ! em: c_loc of a complex(8) array of nen elements, or c_null_ptr
     integer(c_int) function rp_chiqw_info(h, iout, dout, em) bind(C, name="rp_chiqw_info")
-      import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+      import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t) :: iout(16)
       real(c_double) :: dout(8)
-      complex(c_double_complex) :: em(*)
+      type(c_ptr), value :: em
     end function
     integer(c_int) function rp_chiqw_kmesh(h, sk0, numirr, trs) bind(C, name="rp_chiqw_kmesh")
       import :: c_int, c_int32_t, c_double, c_ptr
@@ -113,12 +114,13 @@ module respack_gpu
       real(c_double) :: sk0(3, *)
       integer(c_int32_t) :: numirr(*), trs(*)
     end function
+    ! lg: c_loc of an integer(c_int32_t) array (3, lg_cap), or c_null_ptr
     integer(c_int) function rp_chiqw_prepare_q(h, q, ng, lg, lg_cap) bind(C, name="rp_chiqw_prepare_q")
       import :: c_int, c_int32_t, c_double, c_ptr
       type(c_ptr), value :: h
       real(c_double) :: q(3)
       integer(c_int32_t) :: ng
-      integer(c_int32_t) :: lg(3, *)
+      type(c_ptr), value :: lg
       integer(c_int32_t), value :: lg_cap
     end function
     integer(c_int) function rp_pairdens_batch(h, m_out) bind(C, name="rp_pairdens_batch")

@@ -30,8 +30,8 @@ SOURCES = ["rp_capi.cpp", "rp_pairdens.hip", "rp_tetra.hip", "rp_chi_gemm.hip", 
            "rp_dyson.hip"]
 
 
-def _run(cmd: list[str]) -> None:
-    r = subprocess.run(cmd, capture_output=True, text=True)
+def _run(cmd: list[str], cwd: str | None = None) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=cwd)
     if r.returncode != 0:
         raise RuntimeError(f"command failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
@@ -70,7 +70,7 @@ def build_hosts(verbose: bool = True) -> list[str]:
         exe = os.path.join(BIN, p[:-4])
         cmd = [FC, "-O2", "-J", mod_dir, iface, common, os.path.join(src_dir, p), "-o", exe,
                f"-L{HERE}", "-lrespack_gpu", f"-Wl,-rpath,{HERE}"]
-        _run(cmd)
+        _run(cmd, cwd=mod_dir)
         out.append(exe)
         if verbose:
             print(f"built {exe}")

@@ -231,10 +231,12 @@ class Chiqw:
         check(_lib.lib().rp_dyson(self._h, ptr(out)))
         return out
 
-    def run_q(self, q, precision: str = "fp64", want_chi: bool = False):
+    def run_q(self, q, precision: str = "fp64", want_chi: bool = False, want_eps: bool = True):
+        """One q through all four stages.  want_eps / want_chi copy eps^-1 / chi
+        to the host; without them the results stay in HBM."""
         lg = self.prepare_q(q)
         nen, ng = self.info["nen"], self.ng
-        eps = np.zeros((self.ndir, nen, ng, ng), np.complex128)
+        eps = np.zeros((self.ndir, nen, ng, ng), np.complex128) if want_eps else None
         chi = np.zeros((self.ndir, nen, ng, ng), np.complex128) if want_chi else None
         t = np.zeros(8)
         qq = np.ascontiguousarray(q, np.float64)

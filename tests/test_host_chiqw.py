@@ -1,0 +1,122 @@
+"""The drop-in Fortran host respackdev_amd/bin/calc_chiqw end to end: namelist
+&param_chiqw on stdin, ./dir-wfn in, ./dir-eps out (chiqw.F:559-687,
+1184-2085), compared file by file with the oracle on the tiny seeded system.
+
+Tolerances: eps^-1 in dat.epsqw.600 (and 601/602 at q = 0) within 1e-6
+relative Frobenius of the FP64 oracle (the FP64 path's bar); text files within
+their printed precision (F15.10 for dat.wgrid, F20.10 for the diagonals).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from respackdev_amd import gen
+from respackdev_amd.chiqw import _records, from_system
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "respackdev_amd", "bin", "calc_chiqw")
+
+
+def _case(tmp_path, calc_k, prec=0, name="tiny"):
+    wl = gen.WORKLOADS[name]
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    gen.write_dir_wfn(s, str(tmp_path))
+    nml = ("&param_chiqw\n"
+           f" Ecut_for_eps = {wl['ecut_eps']!r}\n"
+           f" Num_freq_grid = {wl['nen']}\n"
+           " flg_calc_type = 2\n"
+           f" n_calc_q = {len(calc_k)}\n"
+           f" Contraction_precision = {prec}\n"
+           "/\n" + " ".join(str(k) for k in calc_k) + "\n")
+    return wl, from_system(s), nml
+
+
+def _run(tmp_path, nml):
+    assert os.path.exists(BIN), f"{BIN} not built (python -m respackdev_amd.build)"
+    return subprocess.run([BIN], input=nml, cwd=str(tmp_path), capture_output=True, text=True,
+                          timeout=300)
+
+
+def _kindex(w, q):
+    """1-based index of q in dat.sample-k."""
+    for i, k in enumerate(w.ski):
+        if np.allclose(k, q, atol=1e-9):
+            return i + 1
+    raise AssertionError(f"{q} not in the k list")
+
+
+def _epsqw(path, ng, nen):
+    recs = list(_records(path))
+    assert len(recs) == 1
+    return np.frombuffer(recs[0], np.complex128).reshape(nen, ng, ng)
+
+
+def test_host_fails_loudly_without_gpu(tmp_path):
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    wl, w, nml = _case(tmp_path, [1])
+    r = _run(tmp_path, nml)
+    assert r.returncode != 0
+    assert "ERROR: rp_init" in r.stdout
+    # the file side (dir-eps/qNNN) is done before the GPU is touched
+    assert os.path.isdir(tmp_path / "dir-eps" / "q001")
+    assert not (tmp_path / "dir-eps" / "q001" / "dat.log.400").exists()
+
+
+@pytest.mark.gpu
+def test_host_dir_eps_matches_oracle(tmp_path):
+    from oracle import pipeline
+    wl0, w, _ = _case(tmp_path, [1])
+    iq = _kindex(w, wl0["q"])
+    ig = _kindex(w, (0.0, 0.0, 0.0))
+    wl, w, nml = _case(tmp_path, [iq, ig])
+    print(nml, flush=True)
+    r = _run(tmp_path, nml)
+    print(r.stdout[-3000:], r.stderr[-3000:], flush=True)
+    assert r.returncode == 0
+    eps_dir = tmp_path / "dir-eps"
+    nen = wl["nen"]
+    o = pipeline.run_q(w, wl["q"], ecut_for_eps=wl["ecut_eps"], num_freq_grid=nen)
+    # dat.wgrid / dat.chi_cutoff
+    lines = (eps_dir / "dat.wgrid").read_text().split("\n")
+    assert int(lines[0]) == nen
+    em = np.array([complex(float(l[:15]), float(l[15:30])) for l in lines[1:1 + nen]])
+    np.testing.assert_allclose(em, o["em"], rtol=0, atol=6e-11)
+    assert abs(float((eps_dir / "dat.chi_cutoff").read_text()) - wl["ecut_eps"]) < 1e-12
+    # q != 0
+    qd = eps_dir / f"q{iq:03d}"
+    np.testing.assert_allclose([float(x) for x in (qd / "dat.sq").read_text().split()], wl["q"])
+    ng = o["lg"].shape[0]
+    eps = _epsqw(qd / "dat.epsqw.600", ng, nen)
+    err = np.linalg.norm(eps - o["epsinv"][0]) / np.linalg.norm(o["epsinv"][0])
+    print(f"q={wl['q']} eps^-1 rel err {err:.3e}", flush=True)
+    assert err <= 1e-6
+    for ie in (0, nen - 1):
+        t = (qd / f"dat.epsqw.{600001 + ie}").read_text().split("\n")
+        assert t[0].startswith("#epsqw diagonal")
+        vals = np.array([[float(x) for x in l.split()] for l in t[2:2 + ng]])
+        np.testing.assert_allclose(vals[:, 1] + 1j * vals[:, 2], np.diagonal(eps[ie]), atol=2e-10)
+        b = o["setup"]["b"]
+        qg = np.linalg.norm((np.asarray(wl["q"]) + o["lg"]) @ b, axis=1)
+        np.testing.assert_allclose(vals[:, 0], qg, atol=2e-10)
+    assert (qd / "dat.log.400").exists()
+    # q = 0: three directions
+    o0 = pipeline.run_q(w, (0.0, 0.0, 0.0), ecut_for_eps=wl["ecut_eps"], num_freq_grid=nen)
+    gd = eps_dir / f"q{ig:03d}"
+    ng0 = o0["lg"].shape[0]
+    for ix in range(3):
+        e0 = _epsqw(gd / f"dat.epsqw.{600 + ix}", ng0, nen)
+        err = np.linalg.norm(e0 - o0["epsinv"][ix]) / np.linalg.norm(o0["epsinv"][ix])
+        print(f"q=0 dir {ix} eps^-1 rel err {err:.3e}", flush=True)
+        assert err <= 1e-6
+        assert (gd / f"dat.epsqw.{600001 + 1000 * ix}").exists()
+    # restart semantics: both q are flagged done, a rerun does nothing
+    r2 = _run(tmp_path, nml)
+    assert r2.returncode == 0
+    assert "dat.log.400 exists" in r2.stdout
