@@ -130,6 +130,48 @@ int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_o
 /* Device-resident number of pairs (after the Max_excitation_energy filter). */
 int rp_chiqw_npair(rp_handle h, int32_t* npair);
 
+/* ------------------------------------------------------------------------
+ * calc_int drop-in, calc_w3d (replaces calc_intW.F:8-985): Wannier-basis bare
+ * and screened Coulomb integrals V_ij(R) and W_ij(R, w).
+ * ------------------------------------------------------------------------ */
+/* Opens the W3D system from dir-wfn (calc_intW.F:8-330): the same arrays as
+ * rp_chiqw_open (no eigenvalues or wavefunctions), the chi cutoff of
+ * dir-eps/dat.chi_cutoff (Ry) and the frequencies of dir-eps/dat.wgrid
+ * (nen complex, Ha).  Unfolds the k mesh and builds the G lists of every
+ * full-mesh k. */
+int rp_w3d_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
+                const int32_t* rg, const int32_t* pg, int32_t nk_irr, const double* ski,
+                const int32_t* ngi, int32_t ntg, const int32_t* kgi, double ecut_for_eps,
+                int32_t nen, const double* em);
+/* Sizes.  iout (16 ints): NTK, nkb1, nkb2, nkb3, Na1, Na2, Na3, nwx, nwy, nwz,
+ * nen, NTG, NWF, Nq_irr, NTQ, NoG0 (0-based, after rp_w3d_set_q).  dout (8):
+ * Ecut_for_psi, Ecut_for_eps, volume, chead (after rp_w3d_vw), 0... */
+int rp_w3d_info(rp_handle h, int32_t* iout, double* dout);
+/* Full-mesh k points (3*NTK, SK0 order) and NG0(k): the record layout of
+ * dir-wan/dat.wan. */
+int rp_w3d_kmesh(rp_handle h, double* sk0, int32_t* ng0);
+/* Wannier functions of dir-wan/dat.wan (calc_intW.F:332-341): complex
+ * C0(NTG, NWF, NTK), Fortran order, G in make_KG0 order of SK0(k). */
+int rp_w3d_set_wannier(rp_handle h, int32_t nwf, const double* wan);
+/* The irreducible q list (dir-eps/qNNN/dat.sq, calc_intW.F:411-433; Nq_irr =
+ * Nk_irr): unfolds the q mesh (calc_intW.F:420-487) and builds the eps/psi G
+ * lists (make_LG0).  ng_eps (optional): the eps sphere size per irreducible q,
+ * i.e. the leading dimensions of its dat.epsqw.600. */
+int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps);
+/* eps^-1 of irreducible q iq (0-based), complex (ig,jg,ie) as read from
+ * dat.epsqw.600 (at Gamma: the average of 600/601/602, calc_intW.F:595-612).
+ * Stored in single precision, as the reference's epsirr. */
+int rp_w3d_set_epsinv(rp_handle h, int32_t iq, const double* epsinv);
+/* rho, func and funcw for every full-mesh q (calc_intW.F:647-912) on the GPU.
+ * func_out (optional): complex func(i,j,q), Fortran order (NWF,NWF,NTQ);
+ * funcw_out (optional): complex (NWF,NWF,nen,NTQ). */
+int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out);
+/* V and W (calc_intW.F:782-795, 942-985).  vmat: complex V_MAT(NWF,NWF,
+ * -Na1:Na1,-Na2:Na2,-Na3:Na3); wmat: complex W_MAT(...,nen); heads (optional,
+ * 2+2*nen doubles): chead, 0, cheadw(nen) complex.  Runs rp_w3d_funcs first if
+ * needed. */
+int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,9 +19,11 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(_HERE, "_build", "librespack_oracle.so")
 REF_SO = os.path.join(_HERE, "_ref", "librespack_ref.so")
+REF_INT_SO = os.path.join(_HERE, "_ref", "librespack_ref_int.so")
 
 _lib = None
 _ref = None
+_ref_int = None
 
 P = C.c_void_p
 I32 = C.c_int
@@ -71,6 +73,12 @@ def lib():
         L.ro_chiqw_q.argtypes = [I32, I32, P, P, I32, P, I32, P, P, P, P, P, I32, P, P, P, P,
                                  I32, I32, I32, I32, P, I32, I32, I32, P, P, D, D, D, D, I32,
                                  P, D, P, P, P, I32]
+        L.ro_make_lg0.restype = I32
+        L.ro_make_lg0.argtypes = [I32, P, P, P, D, D, D, D, D, P, P]
+        L.ro_make_eps.argtypes = [I32, I32, I32, I32, P, P, P, P, I32, I32, I32, I32, P, P, P]
+        L.ro_w3d_func.argtypes = [I32, I32, I32, P, P]
+        L.ro_w3d_funcw.argtypes = [I32, I32, I32, I32, P, I32, I32, P, P]
+        L.ro_w3d_vw.argtypes = [I32, I32, P, P, P, I32, I32, I32, I32, D, D, P, P, P]
         L.ro_chiqw_q0.restype = I32
         L.ro_chiqw_q0.argtypes = [I32, I32, P, P, I32, P, I32, P, P, P, P, P, I32, P, P,
                                   I32, I32, I32, I32, P, I32, P, P, P, I32, I32, I32, P, P,
@@ -108,6 +116,22 @@ def ref():
         R.rpref_vmaa.argtypes = [I32, I32, P, P, P, P, I32, P, P, P, P, P]
         _ref = R
     return _ref
+
+
+def have_ref_int() -> bool:
+    return os.path.exists(REF_INT_SO)
+
+
+def ref_int():
+    """calc_int_sub.F subroutines (make_LG0, make_eps, calc_InterStateMatrix)."""
+    global _ref_int
+    if _ref_int is None:
+        R = C.CDLL(REF_INT_SO, mode=os.RTLD_LAZY)
+        R.rpref_make_lg0.argtypes = [I32, P, P, P, D, D, D, D, D, P, P, P]
+        R.rpref_make_eps.argtypes = [I32, I32, I32, I32, I32, P, P, P, P, I32, I32, I32, I32, P, P, P]
+        R.rpref_int_pairdens.argtypes = [I32, I32, P, P, P, P, I32, I32, I32, I32, I32, P, I32, P, P]
+        _ref_int = R
+    return _ref_int
 
 
 # ----------------------------------------------------------------- helpers

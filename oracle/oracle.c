@@ -978,3 +978,119 @@ int ro_chiqw_q0(int ntk, int ntg, const int32_t *ng0, const int32_t *kg0,
   for (size_t i = 0; i < 3 * nchi; ++i) chi3[i] /= volume;
   return 0;
 }
+
+/* ======================================================================== */
+/* calc_int, calc_w3d                                                         */
+/* ======================================================================== */
+
+int ro_make_lg0(int ntg, const double *b1, const double *b2, const double *b3,
+                double gcut_eps, double gcut_psi, double q1, double q2, double q3,
+                int32_t *lg, int32_t *ng_eps) { /* calc_int_sub.F:447-504 */
+  int n = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i1 = -100; i1 <= 100; ++i1)
+      for (int i2 = -100; i2 <= 100; ++i2)
+        for (int i3 = -100; i3 <= 100; ++i3) {
+          double v[3];
+          for (int d = 0; d < 3; ++d)
+            v[d] = (q1 + i1) * b1[d] + (q2 + i2) * b2[d] + (q3 + i3) * b3[d];
+          double s = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+          int take = pass == 0 ? (s <= gcut_eps) : (s > gcut_eps && s <= gcut_psi);
+          if (!take) continue;
+          if (n < ntg) { lg[3 * n] = i1; lg[3 * n + 1] = i2; lg[3 * n + 2] = i3; }
+          ++n;
+        }
+    if (pass == 0) *ng_eps = n;
+  }
+  return n;
+}
+
+void ro_make_eps(int ntgq, int ne, int itrs, int ng, const int32_t *lg,
+                 const int32_t *rw, const double *rginv, const int32_t *pg, int nnp,
+                 int l1, int l2, int l3, const int32_t *pack, const float _Complex *epsirr,
+                 float _Complex *epsmk) { /* calc_int_sub.F:222-311 */
+  const double tpi = 2.0 * acos(-1.0);
+  const size_t plane = (size_t)ntgq * ntgq;
+  for (size_t i = 0; i < plane * ne; ++i) epsmk[i] = 0;
+  int *map = malloc(sizeof(int) * (ng > 0 ? ng : 1));
+  ro_z *pf = malloc(sizeof(ro_z) * (ng > 0 ? ng : 1));
+  for (int ig = 0; ig < ng; ++ig) {
+    int i1 = itrs * lg[3 * ig], j1 = itrs * lg[3 * ig + 1], k1 = itrs * lg[3 * ig + 2];
+    int i2 = i1 + rw[0], j2 = j1 + rw[1], k2 = k1 + rw[2];
+    int i3 = (int)rginv[0] * i2 + (int)rginv[3] * j2 + (int)rginv[6] * k2;
+    int j3 = (int)rginv[1] * i2 + (int)rginv[4] * j2 + (int)rginv[7] * k2;
+    int k3 = (int)rginv[2] * i2 + (int)rginv[5] * j2 + (int)rginv[8] * k2;
+    map[ig] = pack[(i3 + l1) + (2 * l1 + 1) * ((j3 + l2) + (size_t)(2 * l2 + 1) * (k3 + l3))] - 1;
+    double ph = tpi * ((double)i1 * pg[0] + (double)j1 * pg[1] + (double)k1 * pg[2]);
+    pf[ig] = cexp(-I * ph / (double)nnp); /* pf1; pf2 is its conjugate */
+  }
+  for (int ig = 0; ig < ng; ++ig)
+    for (int jg = 0; jg < ng; ++jg) {
+      const int a = itrs == 1 ? map[ig] : map[jg], b = itrs == 1 ? map[jg] : map[ig];
+      for (int ie = 0; ie < ne; ++ie) {
+        const ro_z e = (ro_z)epsirr[(size_t)ie * plane + a + (size_t)ntgq * b];
+        /* epsirr*pf1*pf2, left to right, stored in single precision */
+        epsmk[(size_t)ie * plane + ig + (size_t)ntgq * jg] = (float _Complex)((e * pf[ig]) * conj(pf[jg]));
+      }
+    }
+  free(map);
+  free(pf);
+}
+
+void ro_w3d_func(int nwf, int ng_psi, int ldr, const ro_z *rho, ro_z *func) {
+  /* calc_intW.F:833-847 */
+  for (int iw = 0; iw < nwf; ++iw)
+    for (int jw = 0; jw < nwf; ++jw) {
+      ro_z s = 0;
+      for (int g = 0; g < ng_psi; ++g) s += rho[(size_t)iw * ldr + g] * conj(rho[(size_t)jw * ldr + g]);
+      func[jw * nwf + iw] = s;
+    }
+}
+
+void ro_w3d_funcw(int nwf, int ng_eps, int ng_psi, int ldr, const ro_z *rho, int ntgq,
+                  int ne, const float _Complex *epsmk, ro_z *funcw) { /* calc_intW.F:848-912 */
+  const size_t plane = (size_t)ntgq * ntgq;
+  for (int ie = 0; ie < ne; ++ie)
+    for (int iw = 0; iw < nwf; ++iw)
+      for (int jw = 0; jw < nwf; ++jw) {
+        ro_z s = 0;
+        for (int ig = 0; ig < ng_eps; ++ig)
+          for (int jg = 0; jg < ng_eps; ++jg)
+            s += rho[(size_t)iw * ldr + ig] * (ro_z)epsmk[(size_t)ie * plane + ig + (size_t)ntgq * jg] *
+                 conj(rho[(size_t)jw * ldr + jg]);
+        for (int g = ng_eps; g < ng_psi; ++g)
+          s += rho[(size_t)iw * ldr + g] * conj(rho[(size_t)jw * ldr + g]);
+        funcw[((size_t)ie * nwf + jw) * nwf + iw] = s;
+      }
+}
+
+void ro_w3d_vw(int nwf, int ntq, const double *sq, const ro_z *func, const ro_z *funcw,
+               int ne, int na1, int na2, int na3, double volume, double chead,
+               const ro_z *cheadw, ro_z *vmat, ro_z *wmat) { /* calc_intW.F:942-985 */
+  const double tpi = 2.0 * acos(-1.0);
+  const int n1 = 2 * na1 + 1, n2 = 2 * na2 + 1, n3 = 2 * na3 + 1;
+  const size_t nr = (size_t)n1 * n2 * n3;
+  for (int ia1 = -na1; ia1 <= na1; ++ia1)
+    for (int ia2 = -na2; ia2 <= na2; ++ia2)
+      for (int ia3 = -na3; ia3 <= na3; ++ia3) {
+        const size_t r = (size_t)(ia1 + na1) + n1 * ((size_t)(ia2 + na2) + (size_t)n2 * (ia3 + na3));
+        for (int iw = 0; iw < nwf; ++iw)
+          for (int jw = 0; jw < nwf; ++jw) {
+            ro_z s = 0;
+            for (int q = 0; q < ntq; ++q) {
+              double ph = tpi * (sq[3 * q] * ia1 + sq[3 * q + 1] * ia2 + sq[3 * q + 2] * ia3);
+              s += func[((size_t)q * nwf + jw) * nwf + iw] * cexp(-I * ph);
+            }
+            vmat[(r * nwf + jw) * nwf + iw] = 2.0 * tpi * s / volume / (double)ntq + chead;
+            for (int ie = 0; ie < ne; ++ie) {
+              ro_z t = 0;
+              for (int q = 0; q < ntq; ++q) {
+                double ph = tpi * (sq[3 * q] * ia1 + sq[3 * q + 1] * ia2 + sq[3 * q + 2] * ia3);
+                t += funcw[(((size_t)q * ne + ie) * nwf + jw) * nwf + iw] * cexp(-I * ph);
+              }
+              wmat[(((size_t)ie * nr + r) * nwf + jw) * nwf + iw] =
+                  2.0 * tpi * t / volume / (double)ntq + cheadw[ie];
+            }
+          }
+      }
+}

@@ -153,6 +153,36 @@ int ro_chiqw_q0(int ntk, int ntg, const int32_t *ng0, const int32_t *kg0,
                 double dmnr, int nen, const ro_z *em, double volume, ro_z *chi3,
                 int nthreads);
 
+/* ------------------------------------------------------------------------
+ * calc_int, calc_w3d (src/calc_int/calc_intW.F, calc_int_sub.F)
+ * ------------------------------------------------------------------------ */
+/* make_LG0 (calc_int_sub.F:447-504): the eps sphere |q+G|^2 <= gcut_eps, then
+ * gcut_eps < |q+G|^2 <= gcut_psi, each over [-100,100]^3 in g1 > g2 > g3 loop
+ * order; lg[3*i+d].  Returns NG_for_psi (writes capped at ntg). */
+int ro_make_lg0(int ntg, const double *b1, const double *b2, const double *b3,
+                double gcut_eps, double gcut_psi, double q1, double q2, double q3,
+                int32_t *lg, int32_t *ng_eps);
+/* make_eps (calc_int_sub.F:222-311).  epsirr/epsmk single-precision complex,
+ * Fortran (NTGQ,NTGQ,ne); lg[3*i+d] (ng entries used); rginv[i + 3*j];
+ * pack[(i3+l1) + (2l1+1)*((j3+l2) + (2l2+1)*(k3+l3))] = 1-based G index. */
+void ro_make_eps(int ntgq, int ne, int itrs, int ng, const int32_t *lg,
+                 const int32_t *rw, const double *rginv, const int32_t *pg, int nnp,
+                 int l1, int l2, int l3, const int32_t *pack, const float _Complex *epsirr,
+                 float _Complex *epsmk);
+/* func (calc_intW.F:833-847): func[(q*nwf + jw)*nwf + iw] for one q;
+ * rho[w*ldr + g] (ng_psi used). */
+void ro_w3d_func(int nwf, int ng_psi, int ldr, const ro_z *rho, ro_z *func);
+/* funcw (calc_intW.F:848-912) for one q, the reference's loop order;
+ * epsmk (NTGQ,NTGQ,ne) single precision; funcw[(ie*nwf + jw)*nwf + iw]. */
+void ro_w3d_funcw(int nwf, int ng_eps, int ng_psi, int ldr, const ro_z *rho, int ntgq,
+                  int ne, const float _Complex *epsmk, ro_z *funcw);
+/* V_MAT / W_MAT (calc_intW.F:942-985): func[(q*nwf + j)*nwf + i],
+ * funcw[((q*ne + ie)*nwf + j)*nwf + i], sq[3*q+d]; outputs Fortran
+ * V(nwf,nwf,-na1:na1,-na2:na2,-na3:na3) and W(...,ne). */
+void ro_w3d_vw(int nwf, int ntq, const double *sq, const ro_z *func, const ro_z *funcw,
+               int ne, int na1, int na2, int na3, double volume, double chead,
+               const ro_z *cheadw, ro_z *vmat, ro_z *wmat);
+
 #ifdef __cplusplus
 }
 #endif

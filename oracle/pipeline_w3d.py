@@ -1,0 +1,154 @@
+"""The oracle's calc_w3d run (TEST INFRASTRUCTURE, the checker).
+
+Restates the setup of calc_intW.F (k and q unfolding, make_LG0, make_eps
+packing) with the oracle's C routines and Python loops, builds rho_w(q+G) with
+the oracle's pair density (ro_pairdens, the twin of calc_int_sub.F:102-220), and
+evaluates func / funcw / V / W with ro_w3d_func, ro_w3d_funcw and ro_w3d_vw in
+the reference's loop order (calc_intW.F:647-985).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import lib, ptr
+from .pipeline import setup as chiqw_setup
+
+AU = 27.21151
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
+    """w: dir-wfn contents (WfnData); wan: [k][w][G] over the full mesh in the
+    reference's G-list order; sqi: [nq_irr][3]; epsinv: list per irreducible q of
+    [ie][jg][ig] complex (already averaged at Gamma); em: nen complex."""
+    L = lib()
+    # k mesh, G lists, FFT grid: identical to chiqw's setup (calc_intW.F:124-330)
+    s = chiqw_setup(w, ecut_for_eps=ecut_eps, num_freq_grid=1)
+    b, vol = s["b"], s["volume"]
+    ntk, ntg = s["ntk"], s["ntg"]
+    nkb = [int(x) for x in s["nkb"]]
+    na = [n // 2 for n in nkb]
+    ecut_psi = s["ecut_psi"]
+    nwf = wan.shape[1]
+    nen = len(em)
+    rg = _i32(w.rg)
+    nsym = len(rg)
+    nq_irr = len(sqi)
+    # q unfolding (calc_intW.F:420-482)
+    sq = [np.asarray(q, np.float64) for q in sqi]
+    numirrq = list(range(nq_irr)); numrotq = [0] * nq_irr; trsq = [1] * nq_irr
+    rwq = [np.zeros(3, np.int32) for _ in range(nq_irr)]
+    for iq in range(nq_irr):
+        for op in range(nsym):
+            r = rg[op].reshape(3, 3).T  # r[i, j] = rg(i,j)
+            for trs in (1, -1):
+                k = np.ascontiguousarray(r @ sqi[iq], np.float64)
+                rw = np.zeros(3, np.int32)
+                (L.ro_kcheck if trs == 1 else L.ro_kcheck_trs)(ptr(k), ptr(rw))
+                kk = k * trs
+                if any(np.all(np.abs(x - kk) < 1e-4) for x in sq):
+                    continue
+                sq.append(kk.copy()); numirrq.append(iq); numrotq.append(op); trsq.append(trs)
+                rwq.append(rw)
+    ntq = len(sq)
+    assert ntq == ntk, "NTQ should be NTK"
+    # G lists per q (calc_intW.F:500-545)
+    tmp = np.zeros((ntg, 3), np.int32)
+    lg0, nge, ngp = [], [], []
+    for iq in range(ntq):
+        if iq < nq_irr:
+            q = np.asarray(sqi[iq], np.float64)
+        else:
+            r = rg[numrotq[iq]].reshape(3, 3).T
+            q = r @ np.asarray(sqi[numirrq[iq]], np.float64) + rwq[iq]
+        ne = np.zeros(1, np.int32)
+        n = L.ro_make_lg0(ntg, ptr(b[0]), ptr(b[1]), ptr(b[2]), ecut_eps, ecut_psi, q[0], q[1], q[2],
+                          ptr(tmp), ptr(ne))
+        assert n <= ntg
+        lg = tmp[:n].copy() * trsq[iq]
+        lg0.append(np.ascontiguousarray(lg)); nge.append(int(ne[0])); ngp.append(n)
+    ntgq = max(nge)
+    # packing of the irreducible eps spheres, epsirr in single precision
+    lq = [int(max(np.abs(l[:, d]).max() for l in lg0)) + 1 for d in range(3)]
+    epsirr = []
+    packs = []
+    for iq in range(nq_irr):
+        e = np.zeros((nen, ntgq, ntgq), np.complex64)  # Fortran (ig,jg,ie) with ld ntgq
+        n = nge[iq]
+        e[:, :n, :n] = epsinv[iq].astype(np.complex64)
+        epsirr.append(np.ascontiguousarray(e))
+        pk = np.zeros((2 * lq[2] + 1, 2 * lq[1] + 1, 2 * lq[0] + 1), np.int32)
+        g = lg0[iq][:n]
+        pk[g[:, 2] + lq[2], g[:, 1] + lq[1], g[:, 0] + lq[0]] = np.arange(1, n + 1)
+        packs.append(pk)
+    rginv = [np.ascontiguousarray(np.linalg.inv(rg[op].reshape(3, 3).T).T.reshape(-1))
+             for op in range(nsym)]
+    # rho (calc_intW.F:647-767)
+    sk0 = s["sk0"]
+    ng0 = s["ng0"]
+    kg0 = s["kg0"]
+    nw = s["nw"]
+    rho = np.zeros((ntq, nwf, ntg), np.complex128)
+    m = np.zeros(ntg, np.complex128)
+    for iq in range(ntq):
+        q = sq[iq]
+        q0 = q[0] == 0.0 and q[1] == 0.0 and q[2] == 0.0
+        n = ngp[iq]
+        lg = lg0[iq]
+        qg = np.linalg.norm((q[None, :] + lg) @ b, axis=1)
+        nog0 = -1
+        if q0:
+            nog0 = int(np.where((lg == 0).all(1))[0][0])
+        for k in range(ntk):
+            sg = np.zeros(3, np.int32)
+            if q0:
+                kq = k
+            else:
+                kq = L.ro_search_kq(ntk, ptr(sk0), q[0], q[1], q[2], k, ptr(sg))
+            kgk = np.ascontiguousarray(kg0[k, :ng0[k]])
+            kgq = np.ascontiguousarray(kg0[kq, :ng0[kq]])
+            for iw in range(nwf):
+                ck = np.ascontiguousarray(wan[k, iw, :ng0[k]])
+                cq = np.ascontiguousarray(wan[kq, iw, :ng0[kq]])
+                L.ro_pairdens(int(ng0[k]), ptr(kgk), ptr(ck), int(ng0[kq]), ptr(kgq), ptr(cq),
+                              nw[0], nw[1], nw[2], n, ptr(lg), ptr(sg), ptr(m))
+                if q0:
+                    m[nog0] = 0.0
+                rho[iq, iw, :n] += m[:n]
+        for iw in range(nwf):
+            for g in range(n):
+                if g != nog0:
+                    rho[iq, iw, g] = rho[iq, iw, g] / qg[g]
+    rho /= float(ntk)
+    # func / funcw (calc_intW.F:833-912)
+    func = np.zeros((ntq, nwf, nwf), np.complex128)
+    funcw = np.zeros((ntq, nen, nwf, nwf), np.complex128)
+    epsmk = np.zeros((nen, ntgq, ntgq), np.complex64)
+    for iq in range(ntq):
+        r = np.ascontiguousarray(rho[iq])
+        L.ro_w3d_func(nwf, ngp[iq], ntg, ptr(r), ptr(func[iq]))
+        ir, op = numirrq[iq], numrotq[iq]
+        pgo = _i32(w.pg[op])
+        L.ro_make_eps(ntgq, nen, trsq[iq], nge[iq], ptr(lg0[iq]), ptr(_i32(rwq[iq])), ptr(rginv[op]),
+                      ptr(pgo), int(w.nnp), lq[0], lq[1], lq[2], ptr(packs[ir]), ptr(epsirr[ir]),
+                      ptr(epsmk))
+        fw = np.zeros((nen, nwf, nwf), np.complex128)
+        L.ro_w3d_funcw(nwf, nge[iq], ngp[iq], ntg, ptr(r), ntgq, nen, ptr(epsmk), ptr(fw))
+        funcw[iq] = fw
+    # heads (calc_intW.F:782-795): Gamma of the k list, G = 0 of its eps list
+    iqgm = [i for i in range(nq_irr) if np.all(np.abs(w.ski[i]) < 1e-5)][-1]
+    g0 = int(np.where((lg0[iqgm] == 0).all(1))[0][0])
+    qsz = (6.0 * np.pi ** 2 / ntq / vol) ** (1.0 / 3.0)
+    chead = (2.0 / np.pi) * qsz
+    cheadw = np.array([(2.0 / np.pi) * complex(epsirr[iqgm][ie, g0, g0]) * qsz for ie in range(nen)])
+    n1, n2, n3 = (2 * x + 1 for x in na)
+    vmat = np.zeros((n3, n2, n1, nwf, nwf), np.complex128)
+    wmat = np.zeros((nen, n3, n2, n1, nwf, nwf), np.complex128)
+    sqa = np.ascontiguousarray(np.array(sq), np.float64)
+    L.ro_w3d_vw(nwf, ntq, ptr(sqa), ptr(func), ptr(funcw), nen, na[0], na[1], na[2], vol, chead,
+                ptr(np.ascontiguousarray(cheadw)), ptr(vmat), ptr(wmat))
+    return dict(func=func, funcw=funcw, vmat=vmat, wmat=wmat, chead=chead, cheadw=cheadw,
+                rho=rho, sq=sqa, lg0=lg0, nge=nge, ngp=ngp, na=na, trsq=trsq, numirrq=numirrq)

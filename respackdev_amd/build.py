@@ -26,8 +26,8 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
             "-Wno-unused-function", "-Wno-unused-variable",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["rp_capi.cpp", "rp_pairdens.hip", "rp_tetra.hip", "rp_chi_gemm.hip", "rp_chi_bf16.hip",
-           "rp_dyson.hip"]
+SOURCES = ["rp_setup.cpp", "rp_capi.cpp", "rp_w3d.cpp", "rp_pairdens.hip", "rp_tetra.hip",
+           "rp_chi_gemm.hip", "rp_chi_bf16.hip", "rp_dyson.hip", "rp_w3d.hip"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -1,0 +1,116 @@
+// Internal declarations shared by the host-side sources of librespack_gpu:
+// the error-reporting wrapper of the C ABI, the restated setup routines of the
+// reference (k-mesh, G spheres, grids) and the handle behind rp_handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "rp_common.h"
+#include "rp_kernels.h"
+
+namespace rp {
+
+// Message of the last failed C-ABI call on this thread (rp_last_error).
+extern thread_local std::string g_err;
+
+// Runs f; a thrown exception becomes status 1 with its message in g_err.
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  } catch (...) {
+    g_err = "unknown error";
+    return 1;
+  }
+}
+
+constexpr double kAu = 27.21151;  // chiqw/config.h:19
+
+int algn235(int n);                                   // est_nwx2.f90:1-42
+void gen_grid(int nt, double e, double* em);          // gen_grid.f90:1-39
+// make_KG0 (chiqw_sub.F:773-812): {G : |q+G|^2 <= gcut} in the reference's order
+std::vector<int> make_kg0(const double b[3][3], const double a[3][3], double gcut, const double* q);
+void kcheck(double* k, int* rw);                      // chiqw_sub.F:668-716
+void kcheck_trs(double* k, int* rw);                  // chiqw_sub.F:718-771
+void search_list(const double* in, double* sq);       // chiqw_sub.F:76-95
+int search_kq(int ntk, const double* sk0, const double* q, int ik, int* sg);  // chiqw_sub.F:1-75
+void mkidx(int n1, int n2, int n3, const double* b, int* idx);  // m_tetrahedron_20170325.F:505-584
+inline size_t roundup(size_t x, size_t m) { return (x + m - 1) / m * m; }
+// b_i = 2 pi (a_j x a_k) / volume (chiqw.F:145-156); returns the volume
+double reciprocal(const double a[3][3], double b[3][3]);
+// FFT grid of the reference (chiqw.F:389-401 / calc_intW.F:176-190)
+void fft_grid(const double a[3][3], const double b[3][3], double ecut_psi, int nw[3]);
+// Ecut_for_psi = max |k+G|^2 + 1e-8 over the irreducible k lists (chiqw.F:313-352)
+double ecut_for_psi(const double b[3][3], int nk_irr, const double* ski, const int* ngi, int ntg,
+                    const int* kgi);
+
+struct W3d;  // calc_int state (rp_w3d.cpp)
+
+}  // namespace rp
+
+// ------------------------------------------------------------------ context
+using rp::DevBuf;
+
+struct rp_ctx {
+  std::unique_ptr<rp::W3d> w3d;  // calc_int (Wannier-basis V/W), when opened
+  int device = 0;
+  int ncu = 256;  // compute units of the device
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {};
+  hipEvent_t gev[6] = {};  // around each main contraction launch (<= 3 directions)
+  double gemm_seconds = 0;
+  int gemm_launches = 0;
+  // system
+  bool open = false;
+  double a[3][3], b[3][3], volume = 0;
+  int nk_irr = 0, ntg = 0, ntb = 0, nsym = 0, nnp = 1;
+  double ef = 0, delt = 0, dmna = 0, dmnr = 0, delta_ex = 0, ecut_psi = 0, ecut_eps = 0, emax = 0;
+  int nw[3] = {0, 0, 0};
+  int ntk = 0, nkb[3] = {0, 0, 0};
+  std::vector<double> sk0;
+  std::vector<int> numirr, numrot, trs, rwv, index_kpt, imt, ng0, kg0;
+  int n_calc = 0, n_docc = 0, n_pocc = 0, n_uocc = 0, n_occ = 0, n_vir = 0, nen = 0, flg_crpa = 0;
+  std::vector<int> win, tindx;
+  std::vector<double> eave, em;
+  std::vector<double> eband;   // [band][k]
+  std::vector<double> pband;   // complex [band][k]
+  // device: system
+  DevBuf<double2> d_coef;
+  DevBuf<int> d_kg0, d_ng0, d_imt, d_index;
+  DevBuf<float2> d_ur;
+  DevBuf<double> d_eband, d_bvec;
+  DevBuf<double2> d_pband, d_em;
+  // per q
+  bool qready = false, q0 = false, have_A = false, have_w = false, have_chi = false;
+  double q[3] = {0, 0, 0};
+  int ng = 0, g0 = -1, npair = 0;
+  std::vector<int> lg, ikq, shift, pocc, pvir;
+  std::vector<double> qn;
+  size_t lda = 0, kpad = 0;
+  double prep_seconds = 0;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv, d_gjwork;
+  DevBuf<float2> d_A, d_A3;
+  DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand
+  DevBuf<float2> d_wf;         // single-precision copy of the weights
+  DevBuf<double2> d_w, d_part, d_chi, d_eps;
+  DevBuf<double> d_qn, d_wd;
+
+  rp_ctx();   // defined where rp::W3d is complete (rp_w3d.cpp)
+  ~rp_ctx();
+  void release_events() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : gev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  int nbatch() const { return q0 ? 3 * nen : nen; }
+};
+

@@ -56,4 +56,20 @@ size_t gauss_jordan_work_ints(int nbatch);
 void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
                          hipStream_t s);
 
+// rp_w3d.hip
+int w3d_max_wannier();
+int w3d_max_eps_g();
+void launch_w_rho(const float2* A, size_t lda, int ntk, int nwf, int ngpsi, const double* qg,
+                  int g0, double2* rho, hipStream_t s);
+void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
+                   int nwf, double2* func, double2* shell, hipStream_t s);
+void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
+                       const long long* rho_off, const int4* qinfo, const int* minv,
+                       const double2* pf1, const double2* pf2, const long long* map_off,
+                       const double2* shell, int ntq, int nwf, int nen, double2* funcw,
+                       hipStream_t s);
+void launch_w_fourier(const double2* func, const double2* funcw, const double* sq, int ntq, int nwf,
+                      int nen, const int na[3], double volume, double chead, const double2* cheadw,
+                      double2* vmat, double2* wmat, hipStream_t s);
+
 }  // namespace rp

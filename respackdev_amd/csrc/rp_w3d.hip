@@ -1,0 +1,277 @@
+// calc_int (calc_w3d) on gfx950: the Wannier-basis bare and screened Coulomb
+// integrals V_ij(R), W_ij(R,w) of calc_intW.F:636-985.
+//
+// rho_w(q+G) = (1/NTK) sum_k M_ww,k(G) / |q+G| comes from the pair-density
+// kernels (the diagonal Wannier pairs) followed by k_w_rho.  Then per full-mesh
+// q (all q of one launch):
+//   func_ij(q)     = sum_{G in psi} rho_i(G) conj(rho_j(G))                (k_w_func)
+//   funcw_ij(q,w)  = sum_{G,G' in eps} rho_i(G) eps^-1_q(G,G',w) conj(rho_j(G'))
+//                    + sum_{G in psi \ eps} rho_i(G) conj(rho_j(G))        (k_w_contract)
+// where eps^-1_q is the irreducible matrix (complex64, as the reference stores
+// it) seen through the rotation of make_eps (calc_int_sub.F:222-311): an index
+// map and a phase per G, folded into the rho vectors so that eps^-1 is read
+// once, coalesced, straight from its irreducible copy.  Finally the lattice
+// Fourier sums with the Hybertsen-Louie heads (k_w_fourier).
+#include "rp_common.h"
+#include "rp_kernels.h"
+
+namespace rp {
+
+// rho[q][w][G] (complex FP64) from the pair densities A[(w*ntk + k)][lda]
+// (complex64): sum over k in order, / |q+G|, / NTK; G = 0 at q = 0 is zero
+// (calc_intW.F:681-767).  grid: (ceil(ngpsi/256), nwf).
+__global__ void __launch_bounds__(256) k_w_rho(const float2* __restrict__ A, size_t lda, int ntk,
+                                               int ngpsi, const double* __restrict__ qg, int g0,
+                                               double2* __restrict__ rho) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (g >= ngpsi) return;
+  double sr = 0.0, si = 0.0;
+  const float2* a = A + (size_t)w * ntk * lda + g;
+  for (int k = 0; k < ntk; ++k) {
+    const float2 v = a[(size_t)k * lda];
+    sr += (double)v.x;
+    si += (double)v.y;
+  }
+  double2 r;
+  if (g == g0) {
+    r = make_double2(0.0, 0.0);
+  } else {
+    r = make_double2(sr / qg[g] / (double)ntk, si / qg[g] / (double)ntk);
+  }
+  rho[(size_t)w * ngpsi + g] = r;
+}
+
+// Per q: func (all G of the psi list) and the shell part of funcw (G beyond the
+// eps sphere).  grid: ntq blocks of 256; out [q][j][i].
+__global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
+                                                const long long* __restrict__ rho_off,
+                                                const int4* __restrict__ qinfo, int nwf,
+                                                double2* __restrict__ func,
+                                                double2* __restrict__ shell) {
+  const int q = blockIdx.x;
+  const int4 qi = qinfo[q];  // (ng_eps, ng_psi, iq_irr, trs)
+  const double2* r = rho + rho_off[q];
+  const int ngpsi = qi.y, nge = qi.x;
+  __shared__ double red[256];
+  for (int ij = 0; ij < nwf * nwf; ++ij) {
+    const int i = ij % nwf, j = ij / nwf;
+    const double2* ri = r + (size_t)i * ngpsi;
+    const double2* rj = r + (size_t)j * ngpsi;
+    double f[4] = {0, 0, 0, 0};  // func re/im, shell re/im
+    for (int g = threadIdx.x; g < ngpsi; g += blockDim.x) {
+      const double2 a = ri[g], b = rj[g];
+      const double pr = a.x * b.x + a.y * b.y, pi = a.y * b.x - a.x * b.y;  // a conj(b)
+      f[0] += pr;
+      f[1] += pi;
+      if (g >= nge) { f[2] += pr; f[3] += pi; }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      red[threadIdx.x] = f[c];
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        double2* o = (c < 2 ? func : shell) + ((size_t)q * nwf + j) * nwf + i;
+        if (c % 2 == 0) o->x = red[0];
+        else o->y = red[0];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// funcw for one (q, w): Y(a,c) = sum_b E(a,b) X_c(b) over the irreducible
+// eps^-1 E (complex64, column-major), then funcw_ij = sum_a Z(a) Y(a,.) + shell.
+//   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
+//   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
+// grid: (nen, ntq), 256 threads; kMaxWf Wannier functions at most.
+constexpr int kMaxWf = 8;
+constexpr int kTB = 128;  // columns of E staged per pass
+__global__ void __launch_bounds__(256) k_w_contract(
+    const float2* __restrict__ eps, const long long* __restrict__ eps_off,
+    const double2* __restrict__ rho, const long long* __restrict__ rho_off,
+    const int4* __restrict__ qinfo, const int* __restrict__ minv,
+    const double2* __restrict__ pf1, const double2* __restrict__ pf2,
+    const long long* __restrict__ map_off, const double2* __restrict__ shell, int nwf, int nen,
+    double2* __restrict__ funcw) {
+  const int w = blockIdx.x, q = blockIdx.y;
+  const int4 qi = qinfo[q];
+  const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
+  const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
+  const double2* r = rho + rho_off[q];
+  const int* mi = minv + map_off[q];
+  const double2* p1 = pf1 + map_off[q];
+  const double2* p2 = pf2 + map_off[q];
+  __shared__ double2 X[kMaxWf][kTB];
+  __shared__ double red[256];
+  // rows a = threadIdx.x + 256 t
+  constexpr int kRows = 4;  // nge <= 1024
+  double yr[kRows][kMaxWf], yi[kRows][kMaxWf];
+#pragma unroll
+  for (int t = 0; t < kRows; ++t)
+#pragma unroll
+    for (int c = 0; c < kMaxWf; ++c) yr[t][c] = yi[t][c] = 0.0;
+  for (int b0 = 0; b0 < nge; b0 += kTB) {
+    const int nb = min(kTB, nge - b0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nwf * kTB; e += blockDim.x) {
+      const int c = e / kTB, bb = e % kTB;
+      double2 v = make_double2(0.0, 0.0);
+      if (bb < nb) {
+        const int b = b0 + bb, g = mi[b];
+        const double2 rv = r[(size_t)c * ngpsi + g];
+        const double2 p = trs == 1 ? p2[g] : p1[g];
+        // trs = +1: conj(rho) pf2; trs = -1: rho pf1
+        const double xr = rv.x, xi = trs == 1 ? -rv.y : rv.y;
+        v = make_double2(xr * p.x - xi * p.y, xr * p.y + xi * p.x);
+      }
+      X[c][bb] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int a = threadIdx.x + 256 * t;
+      if (a >= nge) continue;
+      for (int bb = 0; bb < nb; ++bb) {
+        const float2 ef = E[(size_t)(b0 + bb) * nge + a];
+        const double er = (double)ef.x, ei = (double)ef.y;
+#pragma unroll
+        for (int c = 0; c < kMaxWf; ++c) {
+          if (c < nwf) {
+            const double2 x = X[c][bb];
+            yr[t][c] += er * x.x - ei * x.y;
+            yi[t][c] += er * x.y + ei * x.x;
+          }
+        }
+      }
+    }
+  }
+  // funcw_ij = sum_a Z(a) Y(a, .)
+  for (int ij = 0; ij < nwf * nwf; ++ij) {
+    const int i = ij % nwf, j = ij / nwf;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int t = 0; t < kRows; ++t) {
+      const int a = threadIdx.x + 256 * t;
+      if (a >= nge) continue;
+      const int g = mi[a];
+      double zr, zi, ycr = 0.0, yci = 0.0;
+      if (trs == 1) {
+        const double2 rv = r[(size_t)i * ngpsi + g], p = p1[g];
+        zr = rv.x * p.x - rv.y * p.y;
+        zi = rv.x * p.y + rv.y * p.x;
+#pragma unroll
+        for (int c = 0; c < kMaxWf; ++c)
+          if (c == j) { ycr = yr[t][c]; yci = yi[t][c]; }
+      } else {
+        const double2 rv = r[(size_t)j * ngpsi + g], p = p2[g];
+        zr = rv.x * p.x + rv.y * p.y;   // conj(rho) pf2
+        zi = rv.x * p.y - rv.y * p.x;
+#pragma unroll
+        for (int c = 0; c < kMaxWf; ++c)
+          if (c == i) { ycr = yr[t][c]; yci = yi[t][c]; }
+      }
+      sr += zr * ycr - zi * yci;
+      si += zr * yci + zi * ycr;
+    }
+    for (int part = 0; part < 2; ++part) {
+      red[threadIdx.x] = part == 0 ? sr : si;
+      __syncthreads();
+      for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0) {
+        const double2 sh = shell[((size_t)q * nwf + j) * nwf + i];
+        double2* o = funcw + (((size_t)q * nen + w) * nwf + j) * nwf + i;
+        if (part == 0) o->x = red[0] + sh.x;
+        else o->y = red[0] + sh.y;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// V_ij(R) (slot w = nen) and W_ij(R,w): 2*2pi sum_q f_ij(q) e^{-i 2pi q.R} / V / NTQ
+// + head (calc_intW.F:942-985).  grid: (nR, nen + 1), threads over (i,j).
+// out: V [R][j][i], W [w][R][j][i].
+__global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ func,
+                                                  const double2* __restrict__ funcw,
+                                                  const double* __restrict__ sq, int ntq, int nwf,
+                                                  int nen, int na1, int na2, int na3,
+                                                  double volume, double chead,
+                                                  const double2* __restrict__ cheadw,
+                                                  double2* __restrict__ vmat,
+                                                  double2* __restrict__ wmat) {
+  const int r = blockIdx.x, w = blockIdx.y;
+  const int n1 = 2 * na1 + 1, n2 = 2 * na2 + 1;
+  const int ia1 = r % n1 - na1, ia2 = (r / n1) % n2 - na2, ia3 = r / (n1 * n2) - na3;
+  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+  const bool isv = w == nen;
+  for (int ij = threadIdx.x; ij < nwf * nwf; ij += blockDim.x) {
+    const int i = ij % nwf, j = ij / nwf;
+    double sr = 0.0, si = 0.0;
+    for (int q = 0; q < ntq; ++q) {
+      const double phase = tpi * (sq[3 * q] * (double)ia1 + sq[3 * q + 1] * (double)ia2 +
+                                  sq[3 * q + 2] * (double)ia3);
+      double s, c;
+      sincos(phase, &s, &c);
+      const double2 f = isv ? func[((size_t)q * nwf + j) * nwf + i]
+                            : funcw[(((size_t)q * nen + w) * nwf + j) * nwf + i];
+      // f * exp(-i phase)
+      sr += f.x * c + f.y * s;
+      si += f.y * c - f.x * s;
+    }
+    const double sc = 2.0 * tpi;
+    if (isv) {
+      vmat[((size_t)r * nwf + j) * nwf + i] =
+          make_double2(sc * sr / volume / (double)ntq + chead, sc * si / volume / (double)ntq);
+    } else {
+      const double2 h = cheadw[w];
+      wmat[(((size_t)w * (n1 * n2 * (2 * na3 + 1)) + r) * nwf + j) * nwf + i] =
+          make_double2(sc * sr / volume / (double)ntq + h.x, sc * si / volume / (double)ntq + h.y);
+    }
+  }
+}
+
+int w3d_max_wannier() { return kMaxWf; }
+int w3d_max_eps_g() { return 4 * 256; }
+
+void launch_w_rho(const float2* A, size_t lda, int ntk, int nwf, int ngpsi, const double* qg,
+                  int g0, double2* rho, hipStream_t s) {
+  hipLaunchKernelGGL(k_w_rho, dim3((ngpsi + 255) / 256, nwf), dim3(256), 0, s, A, lda, ntk, ngpsi,
+                     qg, g0, rho);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
+                   int nwf, double2* func, double2* shell, hipStream_t s) {
+  hipLaunchKernelGGL(k_w_func, dim3(ntq), dim3(256), 0, s, rho, rho_off, qinfo, nwf, func, shell);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
+                       const long long* rho_off, const int4* qinfo, const int* minv,
+                       const double2* pf1, const double2* pf2, const long long* map_off,
+                       const double2* shell, int ntq, int nwf, int nen, double2* funcw,
+                       hipStream_t s) {
+  RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
+  hipLaunchKernelGGL(k_w_contract, dim3(nen, ntq), dim3(256), 0, s, eps, eps_off, rho, rho_off,
+                     qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_w_fourier(const double2* func, const double2* funcw, const double* sq, int ntq, int nwf,
+                      int nen, const int na[3], double volume, double chead, const double2* cheadw,
+                      double2* vmat, double2* wmat, hipStream_t s) {
+  const int nr = (2 * na[0] + 1) * (2 * na[1] + 1) * (2 * na[2] + 1);
+  hipLaunchKernelGGL(k_w_fourier, dim3(nr, nen + 1), dim3(64), 0, s, func, funcw, sq, ntq, nwf,
+                     nen, na[0], na[1], na[2], volume, chead, cheadw, vmat, wmat);
+  RP_HIP(hipGetLastError());
+}
+
+}  // namespace rp

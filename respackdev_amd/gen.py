@@ -139,6 +139,94 @@ def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
     return sysm
 
 
+def full_kmesh(ski: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The full mesh in the reference's unfolding order for the identity
+    symmetry (calc_intW.F:208-262): each listed k, then its time-reversal
+    partner -k when new.  Returns (sk0 (NTK,3), trs (NTK,) +1/-1)."""
+    sk0, trs = [], []
+
+    def known(k):
+        return any(np.all(np.abs(np.asarray(x) - k) < 1e-4) for x in sk0)
+
+    for k in ski:
+        kk = np.array([_fold(v) for v in k])
+        if not known(kk):
+            sk0.append(kk)
+            trs.append(1)
+        mk = np.array([_fold(-v) for v in k])
+        if not known(mk):
+            sk0.append(mk)
+            trs.append(-1)
+    return np.array(sk0), np.array(trs, np.int32)
+
+
+def full_kg(sysm: System) -> list:
+    """G lists of the full mesh as the reference builds them (calc_intW.F:277-330):
+    the sphere at k, or for a time-reversal partner the negated sphere at -k."""
+    sk0, trs = full_kmesh(sysm.ski)
+    gmax = int(np.ceil(np.sqrt(sysm.ecut_psi) / (2 * np.pi / sysm.a))) + 2
+    out = []
+    for k, t in zip(sk0, trs):
+        kg, _ = _sphere(k * t, sysm.bvec, sysm.ecut_psi, gmax)
+        out.append(kg * t)
+    return out
+
+
+# angular factors of the model Wannier orbitals, functions of p = k+G (Cartesian)
+_SHAPES = {
+    "s": lambda p: np.ones(len(p)),
+    "px": lambda p: p[:, 0], "py": lambda p: p[:, 1], "pz": lambda p: p[:, 2],
+    "dxy": lambda p: p[:, 0] * p[:, 1], "dyz": lambda p: p[:, 1] * p[:, 2],
+    "dzx": lambda p: p[:, 2] * p[:, 0],
+    "dx2y2": lambda p: p[:, 0] ** 2 - p[:, 1] ** 2,
+    "dz2": lambda p: 3 * p[:, 2] ** 2 - np.einsum("ij,ij->i", p, p),
+}
+
+
+def make_wannier(sysm: System, nwf: int, seed: int, alpha: float = 1.0, spread: float = 0.5):
+    """Model Wannier functions for dir-wan/dat.wan: Gaussian orbitals
+    exp(-alpha r^2) x angular factor (d-like for nwf = 5, else s, p, d in turn)
+    centred within `spread` bohr of the origin, as plane-wave coefficients
+    c_w(k+G) ~ exp(-|k+G|^2/(4 alpha)) Y_w(k+G) exp(-i (k+G).tau_w) on every
+    full-mesh k, normalised per k.  Returns (wan list per k of (nwf, NG0) complex,
+    centres (nwf, 3) bohr)."""
+    names = (["dxy", "dyz", "dzx", "dx2y2", "dz2"] if nwf == 5 else
+             ["s", "px", "py", "pz", "dxy", "dyz", "dzx", "dx2y2", "dz2"][:nwf])
+    if len(names) != nwf:
+        raise ValueError("at most 9 model Wannier functions")
+    rng = np.random.Generator(np.random.Philox(seed))
+    tau = rng.uniform(-spread, spread, (nwf, 3)) / np.sqrt(3.0)
+    sk0, _ = full_kmesh(sysm.ski)
+    wan = []
+    for k, kg in zip(sk0, full_kg(sysm)):
+        p = (k[None, :] + kg) @ sysm.bvec
+        e2 = np.einsum("ij,ij->i", p, p)
+        c = np.empty((nwf, len(kg)), np.complex128)
+        for w, nm in enumerate(names):
+            v = np.exp(-e2 / (4.0 * alpha)) * _SHAPES[nm](p) * np.exp(-1j * (p @ tau[w]))
+            n = np.linalg.norm(v)
+            c[w] = v / n if n > 0 else v
+        wan.append(c)
+    return wan, tau
+
+
+def write_dir_wan(wan: list, centres: np.ndarray, root: str) -> str:
+    """dir-wan/dat.wan (unformatted: NWF, then per full-mesh k one record
+    ((C0(G,w), G), w)) and dat.wan-center (wannier.F:2127-2177)."""
+    d = os.path.join(root, "dir-wan")
+    os.makedirs(d, exist_ok=True)
+    nwf = len(centres)
+    with open(os.path.join(d, "dat.wan"), "wb") as f:
+        _fortran_record(f, struct.pack("<i", nwf))
+        for c in wan:
+            _fortran_record(f, np.ascontiguousarray(c, dtype=np.complex128).tobytes())
+    with open(os.path.join(d, "dat.wan-center"), "w") as f:
+        f.write("# Wannier centres (bohr)\n# x y z\n")
+        for t in centres:
+            f.write(" " + " ".join(f"{v:.16E}" for v in t) + "\n")
+    return d
+
+
 def _fortran_record(fh, payload: bytes) -> None:
     n = struct.pack("<i", len(payload))
     fh.write(n)
@@ -196,4 +284,10 @@ WORKLOADS = {
     # small parity case used by tests
     "tiny": dict(a=7.0, nk=2, nocc=2, nvir=3, ecut_psi=4.5, seed=7,
                  ecut_eps=2.0, nen=14, q=(0.0, 0.0, 0.5)),
+    # calc_w3d parity case: 3^3 mesh (time-reversal partners present), 3 Wannier functions
+    "w3d_tiny": dict(a=7.0, nk=3, nocc=2, nvir=3, ecut_psi=4.5, seed=11,
+                     ecut_eps=2.0, nen=14, q=(0.0, 0.0, 1.0 / 3.0), nwf=3),
+    # calc_int scale (C4): 6^3 mesh, 5 d-like Wannier functions, NG ~600
+    "c4_calc_int": dict(a=7.0, nk=6, nocc=20, nvir=180, ecut_psi=30.97, seed=1003,
+                        ecut_eps=22.2, nen=32, q=(0.0, 0.0, 1.0 / 6.0), nwf=5),
 }
