@@ -34,6 +34,8 @@ module respack_gpu
   public :: rp_chiqw_open, rp_chiqw_set_crpa, rp_chiqw_info, rp_chiqw_kmesh
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair
+  public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
+  public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw
 
   interface
     integer(c_int) function rp_init(device, h) bind(C, name="rp_init")
@@ -157,6 +159,65 @@ module respack_gpu
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t) :: npair
+    end function
+    ! calc_w3d (calc_intW.F)
+    integer(c_int) function rp_w3d_open(h, lattice, nsym, nnp, rg, pg, nk_irr, ski, ngi, ntg, &
+                                        kgi, ecut_for_eps, nen, em) bind(C, name="rp_w3d_open")
+      import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      real(c_double) :: lattice(3, 3)
+      integer(c_int32_t), value :: nsym, nnp
+      integer(c_int32_t) :: rg(3, 3, *), pg(3, *)
+      integer(c_int32_t), value :: nk_irr
+      real(c_double) :: ski(3, *)
+      integer(c_int32_t) :: ngi(*)
+      integer(c_int32_t), value :: ntg
+      integer(c_int32_t) :: kgi(3, ntg, *)
+      real(c_double), value :: ecut_for_eps
+      integer(c_int32_t), value :: nen
+      complex(c_double_complex) :: em(*)
+    end function
+    integer(c_int) function rp_w3d_info(h, iout, dout) bind(C, name="rp_w3d_info")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: iout(16)
+      real(c_double) :: dout(8)
+    end function
+    integer(c_int) function rp_w3d_kmesh(h, sk0, ng0) bind(C, name="rp_w3d_kmesh")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double) :: sk0(3, *)
+      integer(c_int32_t) :: ng0(*)
+    end function
+    integer(c_int) function rp_w3d_set_wannier(h, nwf, wan) bind(C, name="rp_w3d_set_wannier")
+      import :: c_int, c_int32_t, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nwf
+      complex(c_double_complex) :: wan(*)
+    end function
+    integer(c_int) function rp_w3d_set_q(h, nq_irr, sqi, ng_eps) bind(C, name="rp_w3d_set_q")
+      import :: c_int, c_int32_t, c_double, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nq_irr
+      real(c_double) :: sqi(3, *)
+      integer(c_int32_t) :: ng_eps(*)
+    end function
+    integer(c_int) function rp_w3d_set_epsinv(h, iq, epsinv) bind(C, name="rp_w3d_set_epsinv")
+      import :: c_int, c_int32_t, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: iq
+      complex(c_double_complex) :: epsinv(*)
+    end function
+    integer(c_int) function rp_w3d_funcs(h, func_out, funcw_out) bind(C, name="rp_w3d_funcs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), value :: func_out, funcw_out
+    end function
+    integer(c_int) function rp_w3d_vw(h, vmat, wmat, heads) bind(C, name="rp_w3d_vw")
+      import :: c_int, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      complex(c_double_complex) :: vmat(*), wmat(*)
+      real(c_double) :: heads(*)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr

@@ -59,8 +59,9 @@ contains
 
   ! dir-wfn (qe2respack / xtapp2respack output), read as chiqw.F:88-329 and
   ! chiqw.F:1100-1160 do.
-  subroutine read_dir_wfn(w)
+  subroutine read_dir_wfn(w, with_bands)
     type(wfn_data), intent(out) :: w
+    logical, intent(in), optional :: with_bands  ! .false.: no dat.eigenvalue / dat.wfn
     integer :: u, ik, ib, ig, i, j, iop, ng, ncomp
     open(newunit=u, file='./dir-wfn/dat.bandcalc', status='old', action='read')
     read(u, *) w%ecut_psi
@@ -106,6 +107,9 @@ contains
       end do
     end do
     close(u)
+    if (present(with_bands)) then
+      if (.not. with_bands) return
+    end if
     open(newunit=u, file='./dir-wfn/dat.eigenvalue', status='old', action='read')
     read(u, *) w%ntb
     allocate(w%eig(w%ntb, w%nk_irr))
