@@ -1,0 +1,130 @@
+"""The drop-in Fortran host respackdev_amd/bin/calc_w3d end to end: dir-eps
+is produced by the drop-in calc_chiqw for every irreducible q of a 3^3 mesh,
+then calc_w3d reads dir-wfn, dir-wan and dir-eps and writes dir-intW and
+dir-mvmc; the oracle's calc_w3d runs on the same files.
+
+Tolerances: dat.Vmat / dat.Wmat within their printed precision (f20.10 eV) of
+the oracle; the GPU path itself agrees with the oracle to ~1e-15
+(test_gpu_w3d.py).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from respackdev_amd import gen
+from respackdev_amd.chiqw import from_system
+from respackdev_amd.w3d import AU, read_dir_eps_grid, read_epsqw
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "respackdev_amd", "bin")
+
+
+def _run(prog, tmp_path, nml):
+    exe = os.path.join(BIN, prog)
+    assert os.path.exists(exe), f"{exe} not built (python -m respackdev_amd.build)"
+    return subprocess.run([exe], input=nml, cwd=str(tmp_path), capture_output=True, text=True,
+                          timeout=300)
+
+
+def _write_inputs(tmp_path, nwf=3):
+    wl = gen.WORKLOADS["w3d_tiny"]
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    gen.write_dir_wfn(s, str(tmp_path))
+    wan, tau = gen.make_wannier(s, nwf, 5)
+    gen.write_dir_wan(wan, tau, str(tmp_path))
+    return wl, s, wan
+
+
+def _read_mat(path, nwf, na):
+    """dat.Vmat / dat.Wmat -> [a3][a2][a1][j][i] complex (eV)."""
+    lines = open(path).read().split("\n")[3:]
+    n = 2 * na + 1
+    out = np.zeros((n, n, n, nwf, nwf), np.complex128)
+    pos = 0
+    for a1 in range(-na, na + 1):
+        for a2 in range(-na, na + 1):
+            for a3 in range(-na, na + 1):
+                assert [int(x) for x in lines[pos].split()] == [a1, a2, a3]
+                pos += 1
+                for _ in range(nwf * nwf):
+                    i, j, re, im = lines[pos].split()
+                    out[a3 + na, a2 + na, a1 + na, int(j) - 1, int(i) - 1] = float(re) + 1j * float(im)
+                    pos += 1
+                pos += 1  # blank line
+    return out
+
+
+def test_w3d_host_fails_loudly_without_gpu(tmp_path):
+    try:
+        import torch
+        if torch.cuda.device_count() > 0:
+            pytest.skip("a GPU is visible")
+    except ImportError:
+        pass
+    wl, s, wan = _write_inputs(tmp_path)
+    d = tmp_path / "dir-eps"
+    d.mkdir()
+    (d / "dat.chi_cutoff").write_text(f" {wl['ecut_eps']}\n")
+    (d / "dat.wgrid").write_text(" 2\n   0.0000000000   0.0000000000\n   0.1000000000   0.0000000000\n")
+    for iq, k in enumerate(s.ski):
+        q = d / f"q{iq + 1:03d}"
+        q.mkdir()
+        (q / "dat.sq").write_text(" " + " ".join(str(gen._fold(x)) for x in k) + "\n")
+    r = _run("calc_w3d", tmp_path, "&param_calc_int\n/\n")
+    assert r.returncode != 0
+    assert "ERROR: rp_init" in r.stdout
+    assert not (tmp_path / "dir-intW").exists()
+
+
+@pytest.mark.gpu
+def test_chiqw_then_w3d_match_oracle(tmp_path):
+    from oracle import pipeline_w3d
+    wl, s, wan = _write_inputs(tmp_path)
+    nml = (f"&param_chiqw\n Ecut_for_eps = {wl['ecut_eps']!r}\n Num_freq_grid = {wl['nen']}\n"
+           " flg_calc_type = 0\n/\n")
+    r = _run("calc_chiqw", tmp_path, nml)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    r = _run("calc_w3d", tmp_path, "&param_calc_int\n calc_ifreq = 2\n/\n")
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    # the oracle on the same files
+    w = from_system(s)
+    ecut, em = read_dir_eps_grid(str(tmp_path))
+    nen = len(em)
+    sqi, eps = [], []
+    b = s.bvec
+    import oracle
+    for iq, k in enumerate(w.ski):
+        qd = tmp_path / "dir-eps" / f"q{iq + 1:03d}"
+        q = np.array([float(x.replace("D", "E")) for x in (qd / "dat.sq").read_text().split()])
+        sqi.append(q)
+        ng = len(oracle.make_kg0(b, ecut, q))
+        if np.all(np.abs(k) < 1e-5):
+            e = sum(read_epsqw(str(qd / f"dat.epsqw.{600 + ix}"), ng, nen) for ix in range(3)) / 3.0
+        else:
+            e = read_epsqw(str(qd / "dat.epsqw.600"), ng, nen)
+        eps.append(e)
+    ntg = max(c.shape[1] for c in wan)
+    W = np.zeros((len(wan), wan[0].shape[0], ntg), np.complex128)
+    for kk, c in enumerate(wan):
+        W[kk, :, :c.shape[1]] = c
+    o = pipeline_w3d.run_w3d(w, W, np.array(sqi), eps, em, ecut)
+    na = o["na"][0]
+    v = _read_mat(str(tmp_path / "dir-intW" / "dat.Vmat"), 3, na)
+    wm = _read_mat(str(tmp_path / "dir-intW" / "dat.Wmat"), 3, na)
+    dv = np.abs(v - o["vmat"] * AU).max()
+    dw = np.abs(wm - o["wmat"][1] * AU).max()
+    print(f"dat.Vmat max |dV| {dv:.2e} eV, dat.Wmat max |dW| {dw:.2e} eV", flush=True)
+    assert dv <= 2e-10 and dw <= 2e-10
+    intw = tmp_path / "dir-intW"
+    for i in range(1, 4):
+        assert (intw / f"dat.VvsR.{i:03d}").exists() and (intw / f"dat.WvsR.{i:03d}").exists()
+        for j in range(i, 4):
+            u = (intw / f"dat.UvsE.{i:03d}-{j:03d}").read_text().split("\n")
+            assert len([l for l in u[2:] if l.strip()]) == nen
+    intra = (tmp_path / "dir-mvmc" / "zCoulombIntra.def").read_text().split("\n")
+    assert int(intra[1].split()[1]) == 27 * 3
+    assert (tmp_path / "dir-mvmc" / "zCoulombInter.def").exists()
