@@ -35,7 +35,7 @@ module respack_gpu
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
-  public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw
+  public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run
 
   interface
     integer(c_int) function rp_init(device, h) bind(C, name="rp_init")
@@ -217,6 +217,15 @@ module respack_gpu
       import :: c_int, c_double, c_double_complex, c_ptr
       type(c_ptr), value :: h
       complex(c_double_complex) :: vmat(*), wmat(*)
+      real(c_double) :: heads(*)
+    end function
+    ! calc_j3d (calc_intJ.F) on the opened system
+    integer(c_int) function rp_j3d_run(h, nr, rvec, xmat, jmat, heads) bind(C, name="rp_j3d_run")
+      import :: c_int, c_int32_t, c_double, c_double_complex, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nr
+      integer(c_int32_t) :: rvec(3, *)
+      complex(c_double_complex) :: xmat(*), jmat(*)
       real(c_double) :: heads(*)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")

@@ -172,6 +172,14 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out);
  * needed. */
 int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads);
 
+/* calc_j3d (replaces calc_intJ.F:606-904) on the same opened system: the bare
+ * and screened exchange X_ij(R), J_ij(R, w) for the nr lattice vectors rvec
+ * (3*nr ints; the reference's IX/IY/IZ_INTJ ranges, ia1 outer, ia3 inner).
+ * xmat: complex (NWF,NWF,nr); jmat: complex (NWF,NWF,nr,nen); heads as
+ * rp_w3d_vw (added on i = j at R = 0 only). */
+int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, double* jmat,
+               double* heads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,8 +20,9 @@ def _i32(a):
     return np.ascontiguousarray(a, np.int32)
 
 
-def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
-    """w: dir-wfn contents (WfnData); wan: [k][w][G] over the full mesh in the
+def setup_w3d(w, wan, sqi, epsinv, em, ecut_eps):
+    """The setup shared by calc_w3d and calc_j3d (calc_intW.F:8-630).
+    w: dir-wfn contents (WfnData); wan: [k][w][G] over the full mesh in the
     reference's G-list order; sqi: [nq_irr][3]; epsinv: list per irreducible q of
     [ie][jg][ig] complex (already averaged at Gamma); em: nen complex."""
     L = lib()
@@ -86,6 +87,38 @@ def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
         packs.append(pk)
     rginv = [np.ascontiguousarray(np.linalg.inv(rg[op].reshape(3, 3).T).T.reshape(-1))
              for op in range(nsym)]
+    return dict(L=L, s=s, b=b, vol=vol, ntk=ntk, ntg=ntg, na=na, nwf=nwf, nen=nen, nq_irr=nq_irr,
+                sq=sq, numirrq=numirrq, numrotq=numrotq, trsq=trsq, rwq=rwq, ntq=ntq, lg0=lg0,
+                nge=nge, ngp=ngp, ntgq=ntgq, lq=lq, epsirr=epsirr, packs=packs, rginv=rginv)
+
+
+def _epsmk(c, w, iq, out):
+    """make_eps for full-mesh q iq into out (single precision)."""
+    ir, op = c["numirrq"][iq], c["numrotq"][iq]
+    c["L"].ro_make_eps(c["ntgq"], c["nen"], c["trsq"][iq], c["nge"][iq], ptr(c["lg0"][iq]),
+                       ptr(_i32(c["rwq"][iq])), ptr(c["rginv"][op]), ptr(_i32(w.pg[op])), int(w.nnp),
+                       c["lq"][0], c["lq"][1], c["lq"][2], ptr(c["packs"][ir]), ptr(c["epsirr"][ir]),
+                       ptr(out))
+
+
+def _heads(c, w):
+    """Hybertsen-Louie heads (calc_intW.F:782-795): Gamma of the k list, G = 0."""
+    iqgm = [i for i in range(c["nq_irr"]) if np.all(np.abs(w.ski[i]) < 1e-5)][-1]
+    g0 = int(np.where((c["lg0"][iqgm] == 0).all(1))[0][0])
+    qsz = (6.0 * np.pi ** 2 / c["ntq"] / c["vol"]) ** (1.0 / 3.0)
+    chead = (2.0 / np.pi) * qsz
+    cheadw = np.array([(2.0 / np.pi) * complex(c["epsirr"][iqgm][ie, g0, g0]) * qsz
+                       for ie in range(c["nen"])])
+    return chead, cheadw
+
+
+def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
+    """calc_w3d: rho, func, funcw, V, W (calc_intW.F:647-985)."""
+    c = setup_w3d(w, wan, sqi, epsinv, em, ecut_eps)
+    L, s, b = c["L"], c["s"], c["b"]
+    ntk, ntg, ntq, nwf, nen = c["ntk"], c["ntg"], c["ntq"], c["nwf"], c["nen"]
+    sq, lg0, ngp, nge, na, vol = c["sq"], c["lg0"], c["ngp"], c["nge"], c["na"], c["vol"]
+    ntgq = c["ntgq"]
     # rho (calc_intW.F:647-767)
     sk0 = s["sk0"]
     ng0 = s["ng0"]
@@ -130,20 +163,11 @@ def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     for iq in range(ntq):
         r = np.ascontiguousarray(rho[iq])
         L.ro_w3d_func(nwf, ngp[iq], ntg, ptr(r), ptr(func[iq]))
-        ir, op = numirrq[iq], numrotq[iq]
-        pgo = _i32(w.pg[op])
-        L.ro_make_eps(ntgq, nen, trsq[iq], nge[iq], ptr(lg0[iq]), ptr(_i32(rwq[iq])), ptr(rginv[op]),
-                      ptr(pgo), int(w.nnp), lq[0], lq[1], lq[2], ptr(packs[ir]), ptr(epsirr[ir]),
-                      ptr(epsmk))
+        _epsmk(c, w, iq, epsmk)
         fw = np.zeros((nen, nwf, nwf), np.complex128)
         L.ro_w3d_funcw(nwf, nge[iq], ngp[iq], ntg, ptr(r), ntgq, nen, ptr(epsmk), ptr(fw))
         funcw[iq] = fw
-    # heads (calc_intW.F:782-795): Gamma of the k list, G = 0 of its eps list
-    iqgm = [i for i in range(nq_irr) if np.all(np.abs(w.ski[i]) < 1e-5)][-1]
-    g0 = int(np.where((lg0[iqgm] == 0).all(1))[0][0])
-    qsz = (6.0 * np.pi ** 2 / ntq / vol) ** (1.0 / 3.0)
-    chead = (2.0 / np.pi) * qsz
-    cheadw = np.array([(2.0 / np.pi) * complex(epsirr[iqgm][ie, g0, g0]) * qsz for ie in range(nen)])
+    chead, cheadw = _heads(c, w)
     n1, n2, n3 = (2 * x + 1 for x in na)
     vmat = np.zeros((n3, n2, n1, nwf, nwf), np.complex128)
     wmat = np.zeros((nen, n3, n2, n1, nwf, nwf), np.complex128)
@@ -151,4 +175,71 @@ def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     L.ro_w3d_vw(nwf, ntq, ptr(sqa), ptr(func), ptr(funcw), nen, na[0], na[1], na[2], vol, chead,
                 ptr(np.ascontiguousarray(cheadw)), ptr(vmat), ptr(wmat))
     return dict(func=func, funcw=funcw, vmat=vmat, wmat=wmat, chead=chead, cheadw=cheadw,
-                rho=rho, sq=sqa, lg0=lg0, nge=nge, ngp=ngp, na=na, trsq=trsq, numirrq=numirrq)
+                rho=rho, sq=sqa, lg0=lg0, nge=nge, ngp=ngp, na=na, trsq=c["trsq"],
+                numirrq=c["numirrq"])
+
+
+def run_j3d(w, wan, sqi, epsinv, em, ecut_eps, rlist):
+    """calc_j3d: per R of rlist, rho_ij(q+G) = (1/NTK) sum_k e^{-i2pi k.R}
+    M_(jw at k, iw at k+q)(G)/|q+G|, func/funcw of each pair, X and J
+    (calc_intJ.F:606-904).  Returns X [r][j][i], J [r][ie][j][i]."""
+    c = setup_w3d(w, wan, sqi, epsinv, em, ecut_eps)
+    L, s = c["L"], c["s"]
+    ntk, ntg, ntq, nwf, nen = c["ntk"], c["ntg"], c["ntq"], c["nwf"], c["nen"]
+    sq, lg0, ngp, nge, vol, b = c["sq"], c["lg0"], c["ngp"], c["nge"], c["vol"], c["b"]
+    sk0, ng0, kg0, nw = s["sk0"], s["ng0"], s["kg0"], s["nw"]
+    chead, cheadw = _heads(c, w)
+    epsmk = np.zeros((nen, c["ntgq"], c["ntgq"]), np.complex64)
+    X = np.zeros((len(rlist), nwf, nwf), np.complex128)
+    J = np.zeros((len(rlist), nen, nwf, nwf), np.complex128)
+    m = np.zeros(ntg, np.complex128)
+    for ir, R in enumerate(rlist):
+        func = np.zeros((ntq, nwf, nwf), np.complex128)
+        funcw = np.zeros((ntq, nen, nwf, nwf), np.complex128)
+        for iq in range(ntq):
+            q = sq[iq]
+            q0 = q[0] == 0.0 and q[1] == 0.0 and q[2] == 0.0
+            n = ngp[iq]
+            lg = lg0[iq]
+            qg = np.linalg.norm((q[None, :] + lg) @ b, axis=1)
+            nog0 = int(np.where((lg == 0).all(1))[0][0]) if q0 else -1
+            rho = np.zeros((nwf, nwf, ntg), np.complex128)  # [iw][jw][G]
+            for k in range(ntk):
+                pf = np.exp(-1j * 2 * np.pi * float(sk0[k] @ np.asarray(R, np.float64)))
+                sg = np.zeros(3, np.int32)
+                kq = k if q0 else L.ro_search_kq(ntk, ptr(sk0), q[0], q[1], q[2], k, ptr(sg))
+                kgk = np.ascontiguousarray(kg0[k, :ng0[k]])
+                kgq = np.ascontiguousarray(kg0[kq, :ng0[kq]])
+                for iw in range(nwf):
+                    for jw in range(nwf):
+                        ck = np.ascontiguousarray(wan[k, jw, :ng0[k]])
+                        cq = np.ascontiguousarray(wan[kq, iw, :ng0[kq]])
+                        L.ro_pairdens(int(ng0[k]), ptr(kgk), ptr(ck), int(ng0[kq]), ptr(kgq), ptr(cq),
+                                      nw[0], nw[1], nw[2], n, ptr(lg), ptr(sg), ptr(m))
+                        if q0:
+                            m[nog0] = 0.0
+                            rho[iw, jw, :n] += m[:n] * pf
+                        else:
+                            rho[iw, jw, :n] += m[:n] * pf / qg
+            if q0:
+                for g in range(n):
+                    rho[:, :, g] = rho[:, :, g] / (qg[g] if g != nog0 else 1.0) / float(ntk)
+            else:
+                rho /= float(ntk)
+            _epsmk(c, w, iq, epsmk)
+            for iw in range(nwf):
+                for jw in range(nwf):
+                    rp = np.ascontiguousarray(rho[iw, jw])
+                    f = np.zeros(1, np.complex128)
+                    L.ro_w3d_func(1, n, ntg, ptr(rp), ptr(f))
+                    func[iq, jw, iw] = f[0]
+                    fw = np.zeros(nen, np.complex128)
+                    L.ro_w3d_funcw(1, nge[iq], n, ntg, ptr(rp), c["ntgq"], nen, ptr(epsmk), ptr(fw))
+                    funcw[iq, :, jw, iw] = fw
+        X[ir] = 2.0 * 2 * np.pi * func.sum(0) / vol / ntq
+        J[ir] = 2.0 * 2 * np.pi * funcw.sum(0) / vol / ntq
+        if tuple(R) == (0, 0, 0):
+            for iw in range(nwf):
+                X[ir, iw, iw] += chead
+                J[ir, :, iw, iw] += cheadw
+    return dict(X=X, J=J, chead=chead, cheadw=cheadw, trsq=c["trsq"])

@@ -26,7 +26,7 @@ EXPORTS = [
     "rp_chiqw_prepare_q", "rp_pairdens_batch", "rp_tetra_weights_batch", "rp_chi_accumulate",
     "rp_dyson", "rp_chiqw_q", "rp_chiqw_npair",
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
-    "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw",
+    "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run",
 ]
 
 
@@ -77,6 +77,7 @@ def lib():
         L.rp_w3d_set_epsinv.argtypes = [P, I32, P]
         L.rp_w3d_funcs.argtypes = [P, P, P]
         L.rp_w3d_vw.argtypes = [P, P, P, P]
+        L.rp_j3d_run.argtypes = [P, I32, P, P, P, P]
         for name in EXPORTS:
             getattr(L, name).restype = getattr(L, name).restype or I32
         _lib = L

@@ -343,8 +343,10 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     h->d_ng0.upload(h->ng0, s);
     const size_t nr = (size_t)h->nw[0] * h->nw[1] * h->nw[2];
     h->d_ur.reserve((size_t)nb * ntk * nr);
+    h->d_tw.reserve(twiddle_entries());
+    launch_twiddles(h->d_tw.p, s);
     launch_band_grids(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, ntk, ntg, nb, h->nw[0], h->nw[1], h->nw[2],
-                      h->d_ur.p, s);
+                      h->d_tw.p, h->d_ur.p, s);
     h->d_eband.upload(h->eband, s);
     h->d_pband.upload((const double2*)h->pband.data(), (size_t)nb * ntk, s);
     h->d_imt.upload(h->imt, s);
@@ -510,7 +512,7 @@ int rp_pairdens_batch(rp_handle h, float* m_out) {
     if (h->kpad > K)
       RP_HIP(hipMemsetAsync(h->d_A.p + K * h->lda, 0, (h->kpad - K) * h->lda * sizeof(float2), s));
     launch_pair_densities(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
-                          h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_A.p, s);
+                          h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_tw.p, h->d_A.p, s);
     if (h->q0) {
       h->d_A3.reserve(3 * h->kpad * h->lda);
       for (int ix = 0; ix < 3; ++ix)

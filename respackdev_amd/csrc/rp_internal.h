@@ -87,6 +87,7 @@ struct rp_ctx {
   DevBuf<float2> d_ur;
   DevBuf<double> d_eband, d_bvec;
   DevBuf<double2> d_pband, d_em;
+  DevBuf<double2> d_tw;  // line-DFT twiddle table
   // per q
   bool qready = false, q0 = false, have_A = false, have_w = false, have_chi = false;
   double q[3] = {0, 0, 0};

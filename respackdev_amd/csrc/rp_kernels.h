@@ -11,11 +11,15 @@ namespace rp {
 // rp_pairdens.hip
 bool dft_size_supported(int n);
 int max_grid_points();
+size_t twiddle_entries();
+void launch_twiddles(double2* tab, hipStream_t s);
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
-                       int nband, int nx, int ny, int nz, float2* ur, hipStream_t s);
+                       int nband, int nx, int ny, int nz, const double2* twtab, float2* ur,
+                       hipStream_t s);
 void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
                            const int* ikq, const int* shift, int nx, int ny, int nz,
-                           const int* lg, int ng, size_t lda, int nK, float2* A, hipStream_t s);
+                           const int* lg, int ng, size_t lda, int nK, const double2* twtab,
+                           float2* A, hipStream_t s);
 void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                      const int* pocc, const int* pvir, const double* eband, const double* bvec,
                      double delt, int g0, int ng, size_t lda, size_t rows, int nK, const float2* A,
@@ -68,6 +72,18 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* pf1, const double2* pf2, const long long* map_off,
                        const double2* shell, int ntq, int nwf, int nen, double2* funcw,
                        hipStream_t s);
+void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
+                       const long long* rho_off, const int4* qinfo, const int* minv,
+                       const double2* pf1, const double2* pf2, const long long* map_off,
+                       const double2* shell, int ntq, int npair, int nen, double2* funcw,
+                       hipStream_t s);
+void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
+                  int g0, int q0, const double2* pfk, double2* rho, hipStream_t s);
+void launch_j_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
+                   int npair, double2* func, double2* shell, hipStream_t s);
+void launch_j_sum(const double2* func, const double2* funcw, int ntq, int nwf, int nen, int r0,
+                  double volume, double chead, const double2* cheadw, double2* xmat,
+                  double2* jmat, size_t wstride, hipStream_t s);
 void launch_w_fourier(const double2* func, const double2* funcw, const double* sq, int ntq, int nwf,
                       int nen, const int na[3], double volume, double chead, const double2* cheadw,
                       double2* vmat, double2* wmat, hipStream_t s);

@@ -20,14 +20,14 @@
 
 namespace rp {
 
-// Largest grid the LDS-resident transform holds (FP64 grid plus twiddles within
-// the 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
+// Largest grid the LDS-resident transform holds (the FP64 grid within the
+// 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
 constexpr int kMaxGrid = 10000;
 
 // Length-N DFT of the lines of one axis, in place in LDS: out[k] = sum_j v[j]
-// w^{jk}, w = exp(-2 pi i / N) (sign < 0) or its conjugate (sign > 0).  Lines
-// are taken blockDim at a time: every thread loads its line into registers,
-// the block synchronises, then each thread writes its line's outputs.
+// w^{jk}, w = exp(-2 pi i / N) (sign < 0) or its conjugate (sign > 0).  Each
+// thread transforms whole lines (loaded into registers first); the caller
+// synchronises between axes.
 template <int N>
 __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int axis, int nx, int ny,
                                                                  int nz, int sign,
@@ -43,13 +43,11 @@ __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int
     if (axis == 0) base = nx * l;
     else if (axis == 1) base = (l % nx) + nx * ny * (l / nx);
     else base = l;
-    double2 v[N];
+    // the lines of one axis are disjoint: each thread reads and rewrites its own
     if (mine) {
+      double2 v[N];
 #pragma unroll
       for (int j = 0; j < N; ++j) v[j] = g[base + j * stride];
-    }
-    __syncthreads();
-    if (mine) {
 #pragma unroll 1
       for (int k = 0; k < N; ++k) {
         double re = 0.0, im = 0.0;
@@ -66,7 +64,6 @@ __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int
         g[base + k * stride] = make_double2(re, im);
       }
     }
-    __syncthreads();
   }
 }
 
@@ -98,12 +95,22 @@ bool dft_size_supported(int n) {
 }
 int max_grid_points() { return kMaxGrid; }
 
-__device__ inline void fill_twiddles(double2* tw, int n) {
-  for (int m = threadIdx.x; m < n; m += blockDim.x) {
-    double s, c;
-    sincospi(2.0 * (double)m / (double)n, &s, &c);
-    tw[m] = make_double2(c, -s);
-  }
+// Twiddle table in HBM for every line length 1..32: w_N^m = exp(-2 pi i m/N)
+// at tw[N(N-1)/2 + m].  The line DFTs index it with wave-uniform offsets, so
+// the twiddles arrive through scalar loads into SGPRs and cost neither LDS
+// bandwidth nor vector instructions.
+constexpr int kTwMax = 32;
+constexpr int kTwEntries = kTwMax * (kTwMax + 1) / 2;
+__device__ __forceinline__ const double2* twiddles_of(const double2* tab, int n) {
+  return tab + n * (n - 1) / 2;
+}
+__global__ void k_fill_twiddles(double2* __restrict__ tab) {
+  for (int n = 1; n <= kTwMax; ++n)
+    for (int m = threadIdx.x; m < n; m += blockDim.x) {
+      double s, c;
+      sincospi(2.0 * (double)m / (double)n, &s, &c);
+      tab[n * (n - 1) / 2 + m] = make_double2(c, -s);
+    }
 }
 
 // z, then y, then x (the order of fft3_fw/fft3_bw, m_fft3d_20150826.f90:1598-
@@ -113,8 +120,11 @@ __device__ __attribute__((always_inline)) inline void dft3(double2* g, int nx, i
                                                            int sign, const double2* twx,
                                                            const double2* twy, const double2* twz) {
   dft_axis<MAXN>(g, 2, nz, nx, ny, nz, sign, twz);
+  __syncthreads();
   dft_axis<MAXN>(g, 1, ny, nx, ny, nz, sign, twy);
+  __syncthreads();
   dft_axis<MAXN>(g, 0, nx, nx, ny, nz, sign, twx);
+  __syncthreads();
 }
 
 __device__ inline int wrap(int g, int n) { return ((g % n) + n) % n; }
@@ -125,17 +135,15 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
                                                     const int* __restrict__ kg,
                                                     const int* __restrict__ ng0, int ntk,
                                                     int ntg, int nx, int ny, int nz,
+                                                    const double2* __restrict__ twtab,
                                                     float2* __restrict__ ur) {
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  double2* twx = g + nr;
-  double2* twy = twx + nx;
-  double2* twz = twy + ny;
+  const double2* twx = twiddles_of(twtab, nx);
+  const double2* twy = twiddles_of(twtab, ny);
+  const double2* twz = twiddles_of(twtab, nz);
   const int k = blockIdx.x, b = blockIdx.y;
-  fill_twiddles(twx, nx);
-  fill_twiddles(twy, ny);
-  fill_twiddles(twz, nz);
   for (int r = threadIdx.x; r < nr; r += blockDim.x) g[r] = make_double2(0.0, 0.0);
   __syncthreads();
   const double2* c = coef + ((size_t)b * ntk + k) * ntg;
@@ -160,36 +168,55 @@ __global__ void __launch_bounds__(256) k_pair_densities(
     const float2* __restrict__ ur, int ntk, const int* __restrict__ pocc,
     const int* __restrict__ pvir, const int* __restrict__ ikq, const int* __restrict__ shift,
     int nx, int ny, int nz, const int* __restrict__ lg, int ng, size_t lda, int k_offset,
-    float2* __restrict__ A) {
+    const double2* __restrict__ twtab, float2* __restrict__ A) {
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  double2* twx = g + nr;
-  double2* twy = twx + nx;
-  double2* twz = twy + ny;
+  const double2* twx = twiddles_of(twtab, nx);
+  const double2* twy = twiddles_of(twtab, ny);
+  const double2* twz = twiddles_of(twtab, nz);
   const int K = blockIdx.x + k_offset;
   const int p = K / ntk, k = K % ntk;
   const int kq = ikq[k];
-  fill_twiddles(twx, nx);
-  fill_twiddles(twy, ny);
-  fill_twiddles(twz, nz);
   const float2* uk = ur + ((size_t)pocc[p] * ntk + k) * nr;
   const float2* ukq = ur + ((size_t)pvir[p] * ntk + kq) * nr;
-  const double d1 = shift[3 * k], d2 = shift[3 * k + 1], d3 = shift[3 * k + 2];
-  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
-  for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-    int r1 = r % nx, r2 = (r / nx) % ny, r3 = r / (nx * ny);
-    double ph = tpi * ((double)r1 * d1 / (double)nx + (double)r2 * d2 / (double)ny +
-                       (double)r3 * d3 / (double)nz);
-    double s, c;
-    sincos(ph, &s, &c);
-    float2 a = uk[r], b = ukq[r];
-    // u_kq * exp(-i ph)
-    double bx = (double)b.x * c + (double)b.y * s;
-    double by = (double)b.y * c - (double)b.x * s;
-    // conj(u_kq') * u_k
-    double ax = a.x, ay = a.y;
-    g[r] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
+  const int d1 = shift[3 * k], d2 = shift[3 * k + 1], d3 = shift[3 * k + 2];
+  if (d1 == 0 && d2 == 0 && d3 == 0) {
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+      const float2 a = uk[r], b = ukq[r];
+      // conj(u_kq') * u_k
+      const double bx = b.x, by = b.y, ax = a.x, ay = a.y;
+      g[r] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
+    }
+  } else {
+    // umklapp phase exp(-i 2 pi r.dG/N) = phx(r1) phy(r2) phz(r3), the 1D
+    // factors of calc_InterStateMatrix (chiqw_sub.F:165-185), staged in LDS
+    // behind the grid
+    double2* ph = g + nr;
+    const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+    for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x) {
+      const int ax = i < nx ? 0 : (i < nx + ny ? 1 : 2);
+      const int ri = ax == 0 ? i : (ax == 1 ? i - nx : i - nx - ny);
+      const int n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+      const int dg = ax == 0 ? d1 : (ax == 1 ? d2 : d3);
+      double sn, cs;
+      sincos(tpi * ((double)ri * (double)dg / (double)n), &sn, &cs);
+      ph[i] = make_double2(cs, -sn);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
+      const int r1 = r % nx, r2 = (r / nx) % ny, r3 = r / (nx * ny);
+      const double2 px = ph[r1], py = ph[nx + r2], pz = ph[nx + ny + r3];
+      // pf = phx * phy * phz
+      const double qx = px.x * py.x - px.y * py.y, qy = px.x * py.y + px.y * py.x;
+      const double fx = qx * pz.x - qy * pz.y, fy = qx * pz.y + qy * pz.x;
+      const float2 a = uk[r], b = ukq[r];
+      // u_kq * pf, then conj(u_kq') * u_k
+      const double bx = (double)b.x * fx - (double)b.y * fy;
+      const double by = (double)b.x * fy + (double)b.y * fx;
+      const double ax2 = a.x, ay2 = a.y;
+      g[r] = make_double2(bx * ax2 + by * ay2, bx * ay2 - by * ax2);
+    }
   }
   __syncthreads();
   dft3<MAXN>(g, nx, ny, nz, -1, twx, twy, twz);
@@ -258,24 +285,32 @@ __global__ void __launch_bounds__(64) k_q0_heads(
   }
 }
 
+size_t twiddle_entries() { return kTwEntries; }
+
+void launch_twiddles(double2* tab, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_twiddles, dim3(1), dim3(64), 0, s, tab);
+  RP_HIP(hipGetLastError());
+}
+
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
-                       int nband, int nx, int ny, int nz, float2* ur, hipStream_t s) {
-  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
+                       int nband, int nx, int ny, int nz, const double2* twtab, float2* ur,
+                       hipStream_t s) {
+  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz);
   const bool small = nx <= 16 && ny <= 16 && nz <= 16;
   const void* fn = small ? (const void*)k_band_grids<16> : (const void*)k_band_grids<32>;
   if (lds > 65536)
     RP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   dim3 grid(ntk, nband);
   if (small)
-    hipLaunchKernelGGL(k_band_grids<16>, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, ur);
+    hipLaunchKernelGGL(k_band_grids<16>, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, twtab, ur);
   else
-    hipLaunchKernelGGL(k_band_grids<32>, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, ur);
+    hipLaunchKernelGGL(k_band_grids<32>, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, twtab, ur);
   RP_HIP(hipGetLastError());
 }
 
 void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
                            const int* ikq, const int* shift, int nx, int ny, int nz,
-                           const int* lg, int ng, size_t lda, int nK, float2* A,
+                           const int* lg, int ng, size_t lda, int nK, const double2* twtab, float2* A,
                            hipStream_t s) {
   size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
   const bool small = nx <= 16 && ny <= 16 && nz <= 16;
@@ -288,10 +323,10 @@ void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int
     int n = nK - off < chunk ? nK - off : chunk;
     if (small)
       hipLaunchKernelGGL(k_pair_densities<16>, dim3(n), dim3(256), lds, s, ur, ntk, pocc, pvir,
-                         ikq, shift, nx, ny, nz, lg, ng, lda, off, A);
+                         ikq, shift, nx, ny, nz, lg, ng, lda, off, twtab, A);
     else
       hipLaunchKernelGGL(k_pair_densities<32>, dim3(n), dim3(256), lds, s, ur, ntk, pocc, pvir,
-                         ikq, shift, nx, ny, nz, lg, ng, lda, off, A);
+                         ikq, shift, nx, ny, nz, lg, ng, lda, off, twtab, A);
     RP_HIP(hipGetLastError());
   }
 }

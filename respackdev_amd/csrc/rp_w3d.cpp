@@ -41,7 +41,7 @@ struct W3d {
   double chead = 0;
   std::vector<double2> cheadw;
   // device
-  DevBuf<double2> d_coef;
+  DevBuf<double2> d_coef, d_tw;
   DevBuf<int> d_kg0, d_ng0;
   DevBuf<float2> d_ur, d_eps, d_A;
   DevBuf<long long> d_eps_off, d_rho_off, d_map_off;
@@ -82,6 +82,80 @@ std::vector<int> make_lg0(const W3d& w, const double* q, int* n_eps) {
     if (s > w.ecut_eps && s <= w.ecut_psi) out.insert(out.end(), {all[i], all[i + 1], all[i + 2]});
   }
   return out;
+}
+
+// Hybertsen-Louie heads (calc_intW.F:782-795, calc_intJ.F:883-904)
+void compute_heads(W3d& w) {
+  const double pi = std::acos(-1.0);
+  const double qsz = std::pow(6.0 * pi * pi / (double)w.ntq / w.volume, 1.0 / 3.0);
+  w.chead = (2.0 / pi) * qsz;
+  w.cheadw.assign(w.nen, make_double2(0, 0));
+  const size_t ng = w.nge[w.iqgm];
+  for (int ie = 0; ie < w.nen; ++ie) {
+    const float2 e = w.epsirr[w.iqgm][(size_t)w.nog0 + ng * ((size_t)w.nog0 + ng * ie)];
+    w.cheadw[ie] = make_double2((2.0 / pi) * (double)e.x * qsz, (2.0 / pi) * (double)e.y * qsz);
+  }
+}
+
+void check_ready(const W3d& w, const char* what) {
+  RP_CHECK(w.have_wan && w.have_q, std::string(what) + ": Wannier functions or q list missing");
+  for (int iq = 0; iq < w.nq_irr; ++iq)
+    RP_CHECK(!w.epsirr[iq].empty(), std::string(what) + ": eps^-1 of irreducible q " +
+                                        std::to_string(iq + 1) + " missing");
+}
+
+// eps^-1 of every irreducible q and the per-q rotation maps into HBM
+void upload_eps_maps(W3d& w, hipStream_t s) {
+  w.eps_off.assign(w.nq_irr, 0);
+  size_t tot = 0;
+  for (int iq = 0; iq < w.nq_irr; ++iq) { w.eps_off[iq] = (long long)tot; tot += w.epsirr[iq].size(); }
+  w.d_eps.reserve(tot);
+  for (int iq = 0; iq < w.nq_irr; ++iq)
+    RP_HIP(hipMemcpyAsync(w.d_eps.p + w.eps_off[iq], w.epsirr[iq].data(),
+                          sizeof(float2) * w.epsirr[iq].size(), hipMemcpyHostToDevice, s));
+  w.d_eps_off.upload(w.eps_off, s);
+  w.d_map_off.upload(w.map_off, s);
+  w.d_qinfo.upload(w.qinfo, s);
+  w.d_minv.upload(w.minv, s);
+  w.d_pf1.upload(w.pf1, s);
+  w.d_pf2.upload(w.pf2, s);
+}
+
+// Per q: k+q map, umklapp shifts, |q+G| and the G = 0 slot at q = 0.
+struct QGeom {
+  bool q0;
+  int g0;
+  std::vector<int> ikq, shift;
+  std::vector<double> qg;
+};
+QGeom q_geometry(const W3d& w, int iq) {
+  QGeom r;
+  const double* q = &w.sq[3 * (size_t)iq];
+  r.q0 = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;  // calc_intW.F:650,710
+  r.ikq.assign(w.ntk, 0);
+  r.shift.assign(3 * (size_t)w.ntk, 0);
+  for (int k = 0; k < w.ntk; ++k) {
+    if (r.q0) { r.ikq[k] = k; continue; }
+    int sg[3];
+    const int kq = search_kq(w.ntk, w.sk0.data(), q, k, sg);
+    RP_CHECK(kq >= 0, "k+q is not on the k mesh");
+    r.ikq[k] = kq;
+    for (int d = 0; d < 3; ++d) r.shift[3 * k + d] = sg[d];
+  }
+  const int np = w.ngpsi[iq];
+  r.qg.assign(np, 0.0);
+  r.g0 = -1;
+  for (int g = 0; g < np; ++g) {
+    double v[3];
+    for (int d = 0; d < 3; ++d)
+      v[d] = (q[0] + (double)w.lg0[iq][3 * g]) * w.b[0][d] +
+             (q[1] + (double)w.lg0[iq][3 * g + 1]) * w.b[1][d] +
+             (q[2] + (double)w.lg0[iq][3 * g + 2]) * w.b[2][d];
+    r.qg[g] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (r.q0 && w.lg0[iq][3 * g] == 0 && w.lg0[iq][3 * g + 1] == 0 && w.lg0[iq][3 * g + 2] == 0)
+      r.g0 = g;
+  }
+  return r;
 }
 
 }  // namespace
@@ -238,8 +312,10 @@ int rp_w3d_set_wannier(rp_handle h, int32_t nwf, const double* wan) {
     w.d_ng0.upload(w.ng0, s);
     const size_t nr = (size_t)w.nw[0] * w.nw[1] * w.nw[2];
     w.d_ur.reserve((size_t)nwf * ntk * nr);
+    w.d_tw.reserve(twiddle_entries());
+    launch_twiddles(w.d_tw.p, s);
     launch_band_grids(w.d_coef.p, w.d_kg0.p, w.d_ng0.p, ntk, ntg, nwf, w.nw[0], w.nw[1], w.nw[2],
-                      w.d_ur.p, s);
+                      w.d_tw.p, w.d_ur.p, s);
     RP_HIP(hipStreamSynchronize(s));
     w.have_wan = true;
     w.have_funcs = false;
@@ -385,28 +461,12 @@ int rp_w3d_set_epsinv(rp_handle h, int32_t iq, const double* epsinv) {
 int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
   return guarded([&] {
     W3d& w = w3d_of(h, "rp_w3d_funcs");
-    RP_CHECK(w.have_wan && w.have_q, "rp_w3d_funcs: Wannier functions or q list missing");
-    for (int iq = 0; iq < w.nq_irr; ++iq)
-      RP_CHECK(!w.epsirr[iq].empty(), "rp_w3d_funcs: eps^-1 of irreducible q " +
-                                          std::to_string(iq + 1) + " missing");
+    check_ready(w, "rp_w3d_funcs");
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const int ntk = w.ntk, nwf = w.nwf, nen = w.nen;
-    // eps^-1 of every irreducible q in HBM (complex64)
-    w.eps_off.assign(w.nq_irr, 0);
-    size_t tot = 0;
-    for (int iq = 0; iq < w.nq_irr; ++iq) { w.eps_off[iq] = (long long)tot; tot += w.epsirr[iq].size(); }
-    w.d_eps.reserve(tot);
-    for (int iq = 0; iq < w.nq_irr; ++iq)
-      RP_HIP(hipMemcpyAsync(w.d_eps.p + w.eps_off[iq], w.epsirr[iq].data(),
-                            sizeof(float2) * w.epsirr[iq].size(), hipMemcpyHostToDevice, s));
-    w.d_eps_off.upload(w.eps_off, s);
+    upload_eps_maps(w, s);
     w.d_rho_off.upload(w.rho_off, s);
-    w.d_map_off.upload(w.map_off, s);
-    w.d_qinfo.upload(w.qinfo, s);
-    w.d_minv.upload(w.minv, s);
-    w.d_pf1.upload(w.pf1, s);
-    w.d_pf2.upload(w.pf2, s);
     w.d_rho.reserve((size_t)(w.rho_off.back() + (long long)w.ngpsi.back() * w3d_max_wannier()));
     // rho for every q (calc_intW.F:647-767): diagonal Wannier pair densities
     std::vector<int> pw(nwf);
@@ -419,39 +479,16 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     w.d_qg.reserve((size_t)maxpsi);
     w.d_ikq.reserve(ntk);
     w.d_shift.reserve(3 * (size_t)ntk);
-    std::vector<int> ikq(ntk), shift(3 * (size_t)ntk);
-    std::vector<double> qg;
     for (int iq = 0; iq < w.ntq; ++iq) {
-      const double* q = &w.sq[3 * (size_t)iq];
-      const bool q0 = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;  // calc_intW.F:650,710
-      for (int k = 0; k < ntk; ++k) {
-        if (q0) { ikq[k] = k; shift[3 * k] = shift[3 * k + 1] = shift[3 * k + 2] = 0; continue; }
-        int sg[3];
-        const int kq = search_kq(ntk, w.sk0.data(), q, k, sg);
-        RP_CHECK(kq >= 0, "k+q is not on the k mesh");
-        ikq[k] = kq;
-        for (int d = 0; d < 3; ++d) shift[3 * k + d] = sg[d];
-      }
+      const QGeom geo = q_geometry(w, iq);
       const int np = w.ngpsi[iq];
-      qg.assign(np, 0.0);
-      for (int g = 0; g < np; ++g) {
-        double v[3];
-        for (int d = 0; d < 3; ++d)
-          v[d] = (q[0] + (double)w.lg0[iq][3 * g]) * w.b[0][d] +
-                 (q[1] + (double)w.lg0[iq][3 * g + 1]) * w.b[1][d] +
-                 (q[2] + (double)w.lg0[iq][3 * g + 2]) * w.b[2][d];
-        qg[g] = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-      }
-      int g0 = -1;
-      if (q0)
-        for (int g = 0; g < np; ++g)
-          if (w.lg0[iq][3 * g] == 0 && w.lg0[iq][3 * g + 1] == 0 && w.lg0[iq][3 * g + 2] == 0) g0 = g;
+      const int g0 = geo.g0;
       RP_HIP(hipMemcpyAsync(w.d_lg.p, w.lg0[iq].data(), sizeof(int) * 3 * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_qg.p, qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_ikq.p, ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_shift.p, shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
       launch_pair_densities(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_ikq.p, w.d_shift.p, w.nw[0], w.nw[1],
-                            w.nw[2], w.d_lg.p, np, lda, nwf * ntk, w.d_A.p, s);
+                            w.nw[2], w.d_lg.p, np, lda, nwf * ntk, w.d_tw.p, w.d_A.p, s);
       launch_w_rho(w.d_A.p, lda, ntk, nwf, np, w.d_qg.p, g0, w.d_rho.p + w.rho_off[iq], s);
       // the host vectors are reused by the next q: wait for the copies
       RP_HIP(hipStreamSynchronize(s));
@@ -481,16 +518,7 @@ int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads) {
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const int nwf = w.nwf, nen = w.nen;
-    // Hybertsen-Louie heads (calc_intW.F:782-795)
-    const double pi = std::acos(-1.0);
-    const double qsz = std::pow(6.0 * pi * pi / (double)w.ntq / w.volume, 1.0 / 3.0);
-    w.chead = (2.0 / pi) * qsz;
-    w.cheadw.assign(nen, make_double2(0, 0));
-    const size_t ng = w.nge[w.iqgm];
-    for (int ie = 0; ie < nen; ++ie) {
-      const float2 e = w.epsirr[w.iqgm][(size_t)w.nog0 + ng * ((size_t)w.nog0 + ng * ie)];
-      w.cheadw[ie] = make_double2((2.0 / pi) * (double)e.x * qsz, (2.0 / pi) * (double)e.y * qsz);
-    }
+    compute_heads(w);
     const int nr = (2 * w.na[0] + 1) * (2 * w.na[1] + 1) * (2 * w.na[2] + 1);
     w.d_sq.upload(w.sq, s);
     w.d_cheadw.upload(w.cheadw, s);
@@ -503,6 +531,95 @@ int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads) {
     if (wmat)
       RP_HIP(hipMemcpyAsync(wmat, w.d_wmat.p, sizeof(double2) * nr * nwf * nwf * nen,
                             hipMemcpyDeviceToHost, s));
+    RP_HIP(hipStreamSynchronize(s));
+    if (heads) {
+      heads[0] = w.chead;
+      heads[1] = 0.0;
+      for (int ie = 0; ie < nen; ++ie) {
+        heads[2 + 2 * ie] = w.cheadw[ie].x;
+        heads[3 + 2 * ie] = w.cheadw[ie].y;
+      }
+    }
+  });
+}
+
+int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, double* jmat,
+               double* heads) {
+  return guarded([&] {
+    W3d& w = w3d_of(h, "rp_j3d_run");
+    check_ready(w, "rp_j3d_run");
+    RP_CHECK(nr >= 1 && rvec, "rp_j3d_run: empty R list");
+    RP_HIP(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const int ntk = w.ntk, nwf = w.nwf, nen = w.nen, npair = nwf * nwf;
+    upload_eps_maps(w, s);
+    compute_heads(w);
+    // pair p = iw + nwf*jw: C0_K = C0(:,jw,k), C0_KQ = C0(:,iw,k+q) (calc_intJ.F:651-657)
+    std::vector<int> po(npair), pv(npair);
+    for (int jw = 0; jw < nwf; ++jw)
+      for (int iw = 0; iw < nwf; ++iw) { po[iw + nwf * jw] = jw; pv[iw + nwf * jw] = iw; }
+    DevBuf<int> d_po, d_pv;
+    d_po.upload(po, s);
+    d_pv.upload(pv, s);
+    // e^{-i 2pi k.R} per R and k (calc_intJ.F:646-650)
+    const double tpi = 2.0 * std::acos(-1.0);
+    std::vector<double2> pfk((size_t)nr * ntk);
+    for (int r = 0; r < nr; ++r)
+      for (int k = 0; k < ntk; ++k) {
+        const double ph = tpi * (w.sk0[3 * k] * rvec[3 * r] + w.sk0[3 * k + 1] * rvec[3 * r + 1] +
+                                 w.sk0[3 * k + 2] * rvec[3 * r + 2]);
+        pfk[(size_t)r * ntk + k] = make_double2(std::cos(ph), -std::sin(ph));
+      }
+    DevBuf<double2> d_pfk;
+    d_pfk.upload(pfk, s);
+    // rho per (R, q): [r][q][p][G]
+    std::vector<long long> joff(w.ntq);
+    long long tot = 0;
+    for (int iq = 0; iq < w.ntq; ++iq) { joff[iq] = tot; tot += (long long)npair * w.ngpsi[iq]; }
+    DevBuf<long long> d_joff;
+    d_joff.upload(joff, s);
+    DevBuf<double2> d_jrho, d_jfunc, d_jshell, d_jfuncw, d_x, d_j, d_cw;
+    d_jrho.reserve((size_t)tot * nr);
+    const int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
+    const size_t lda = roundup((size_t)maxpsi, 64);
+    w.d_A.reserve((size_t)npair * ntk * lda);
+    w.d_lg.reserve(3 * (size_t)maxpsi);
+    w.d_qg.reserve((size_t)maxpsi);
+    w.d_ikq.reserve(ntk);
+    w.d_shift.reserve(3 * (size_t)ntk);
+    for (int iq = 0; iq < w.ntq; ++iq) {
+      const QGeom geo = q_geometry(w, iq);
+      const int np = w.ngpsi[iq];
+      RP_HIP(hipMemcpyAsync(w.d_lg.p, w.lg0[iq].data(), sizeof(int) * 3 * np, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
+      RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
+      launch_pair_densities(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_ikq.p, w.d_shift.p, w.nw[0], w.nw[1],
+                            w.nw[2], w.d_lg.p, np, lda, npair * ntk, w.d_tw.p, w.d_A.p, s);
+      for (int r = 0; r < nr; ++r)
+        launch_j_rho(w.d_A.p, lda, ntk, npair, np, w.d_qg.p, geo.g0, geo.q0 ? 1 : 0,
+                     d_pfk.p + (size_t)r * ntk, d_jrho.p + (size_t)r * tot + joff[iq], s);
+      RP_HIP(hipStreamSynchronize(s));  // the host vectors of this q are reused
+    }
+    d_jfunc.reserve((size_t)w.ntq * npair);
+    d_jshell.reserve((size_t)w.ntq * npair);
+    d_jfuncw.reserve((size_t)w.ntq * nen * npair);
+    d_x.reserve((size_t)nr * npair);
+    d_j.reserve((size_t)nr * nen * npair);
+    d_cw.upload(w.cheadw, s);
+    for (int r = 0; r < nr; ++r) {
+      const double2* rho = d_jrho.p + (size_t)r * tot;
+      launch_j_func(rho, d_joff.p, w.d_qinfo.p, w.ntq, npair, d_jfunc.p, d_jshell.p, s);
+      launch_j_contract(w.d_eps.p, w.d_eps_off.p, rho, d_joff.p, w.d_qinfo.p, w.d_minv.p, w.d_pf1.p,
+                        w.d_pf2.p, w.d_map_off.p, d_jshell.p, w.ntq, npair, nen, d_jfuncw.p, s);
+      const int r0 = rvec[3 * r] == 0 && rvec[3 * r + 1] == 0 && rvec[3 * r + 2] == 0;
+      launch_j_sum(d_jfunc.p, d_jfuncw.p, w.ntq, nwf, nen, r0, w.volume, w.chead, d_cw.p,
+                   d_x.p + (size_t)r * npair, d_j.p + (size_t)r * npair, (size_t)nr * npair, s);
+    }
+    if (xmat)
+      RP_HIP(hipMemcpyAsync(xmat, d_x.p, sizeof(double2) * nr * npair, hipMemcpyDeviceToHost, s));
+    if (jmat)
+      RP_HIP(hipMemcpyAsync(jmat, d_j.p, sizeof(double2) * nr * nen * npair, hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
     if (heads) {
       heads[0] = w.chead;

@@ -85,24 +85,31 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 }
 
 // funcw for one (q, w): Y(a,c) = sum_b E(a,b) X_c(b) over the irreducible
-// eps^-1 E (complex64, column-major), then funcw_ij = sum_a Z(a) Y(a,.) + shell.
+// eps^-1 E (complex64, column-major), then funcw = sum_a Z(a) Y(a,.) + shell.
 //   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
-// grid: (nen, ntq), 256 threads; kMaxWf Wannier functions at most.
+// kDiag = false (calc_w3d): the vectors are the nv <= 8 Wannier functions and
+// every (i,j) is formed, out [q][w][j][i].  kDiag = true (calc_j3d): the
+// vectors are the (iw,jw) pair densities, 8 per workgroup (blockIdx.z), and
+// only (p,p) is formed, out [q][w][p].
+// grid: (nen, ntq, groups), 256 threads.
 constexpr int kMaxWf = 8;
 constexpr int kTB = 128;  // columns of E staged per pass
+template <bool kDiag>
 __global__ void __launch_bounds__(256) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
     const int4* __restrict__ qinfo, const int* __restrict__ minv,
     const double2* __restrict__ pf1, const double2* __restrict__ pf2,
-    const long long* __restrict__ map_off, const double2* __restrict__ shell, int nwf, int nen,
+    const long long* __restrict__ map_off, const double2* __restrict__ shell, int nv, int nen,
     double2* __restrict__ funcw) {
   const int w = blockIdx.x, q = blockIdx.y;
   const int4 qi = qinfo[q];
   const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
-  const double2* r = rho + rho_off[q];
+  const int c0 = kDiag ? (int)blockIdx.z * kMaxWf : 0;
+  const int nwf = kDiag ? min(kMaxWf, nv - c0) : nv;  // vectors of this workgroup
+  const double2* r = rho + rho_off[q] + (size_t)c0 * ngpsi;
   const int* mi = minv + map_off[q];
   const double2* p1 = pf1 + map_off[q];
   const double2* p2 = pf2 + map_off[q];
@@ -151,8 +158,9 @@ __global__ void __launch_bounds__(256) k_w_contract(
     }
   }
   // funcw_ij = sum_a Z(a) Y(a, .)
-  for (int ij = 0; ij < nwf * nwf; ++ij) {
-    const int i = ij % nwf, j = ij / nwf;
+  const int npairs = kDiag ? nwf : nwf * nwf;
+  for (int ij = 0; ij < npairs; ++ij) {
+    const int i = kDiag ? ij : ij % nwf, j = kDiag ? ij : ij / nwf;
     double sr = 0.0, si = 0.0;
 #pragma unroll
     for (int t = 0; t < kRows; ++t) {
@@ -186,12 +194,118 @@ __global__ void __launch_bounds__(256) k_w_contract(
         __syncthreads();
       }
       if (threadIdx.x == 0) {
-        const double2 sh = shell[((size_t)q * nwf + j) * nwf + i];
-        double2* o = funcw + (((size_t)q * nen + w) * nwf + j) * nwf + i;
+        double2* o;
+        double2 sh;
+        if (kDiag) {
+          sh = shell[(size_t)q * nv + c0 + i];
+          o = funcw + ((size_t)q * nen + w) * nv + c0 + i;
+        } else {
+          sh = shell[((size_t)q * nwf + j) * nwf + i];
+          o = funcw + (((size_t)q * nen + w) * nwf + j) * nwf + i;
+        }
         if (part == 0) o->x = red[0] + sh.x;
         else o->y = red[0] + sh.y;
       }
       __syncthreads();
+    }
+  }
+}
+
+// calc_j3d rho (calc_intJ.F:634-790): rho_p(G) for the pair p = iw + nwf*jw,
+// sum_k M_(jw at k, iw at k+q)(G) e^{-i 2pi k.R} from A[(p*ntk + k)][lda]: at
+// q != 0 each term is divided by |q+G|, at q = 0 the sum is (G = 0 dropped);
+// then / NTK.  grid: (ceil(ngpsi/256), npair).
+__global__ void __launch_bounds__(256) k_j_rho(const float2* __restrict__ A, size_t lda, int ntk,
+                                               int ngpsi, const double* __restrict__ qg, int g0,
+                                               int q0, const double2* __restrict__ pfk,
+                                               double2* __restrict__ rho) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (g >= ngpsi) return;
+  double sr = 0.0, si = 0.0;
+  const float2* a = A + (size_t)p * ntk * lda + g;
+  const double len = qg[g];
+  for (int k = 0; k < ntk; ++k) {
+    const float2 v = a[(size_t)k * lda];
+    const double2 f = pfk[k];
+    double tr = (double)v.x * f.x - (double)v.y * f.y;
+    double ti = (double)v.x * f.y + (double)v.y * f.x;
+    if (!q0) { tr = tr / len; ti = ti / len; }
+    sr += tr;
+    si += ti;
+  }
+  double2 r;
+  if (q0 && g == g0) r = make_double2(0.0, 0.0);
+  else if (q0) r = make_double2(sr / len / (double)ntk, si / len / (double)ntk);
+  else r = make_double2(sr / (double)ntk, si / (double)ntk);
+  rho[(size_t)p * ngpsi + g] = r;
+}
+
+// calc_j3d func / shell (calc_intJ.F:680-690): per (q, p) sum_G |rho_p(G)|^2 as
+// rho conj(rho).  grid: (ntq, npair), 256 threads; out [q][p].
+__global__ void __launch_bounds__(256) k_j_func(const double2* __restrict__ rho,
+                                                const long long* __restrict__ rho_off,
+                                                const int4* __restrict__ qinfo, int np,
+                                                double2* __restrict__ func,
+                                                double2* __restrict__ shell) {
+  const int q = blockIdx.x, p = blockIdx.y;
+  const int4 qi = qinfo[q];
+  const int nge = qi.x, ngpsi = qi.y;
+  const double2* r = rho + rho_off[q] + (size_t)p * ngpsi;
+  __shared__ double red[256];
+  double f[4] = {0, 0, 0, 0};
+  for (int g = threadIdx.x; g < ngpsi; g += blockDim.x) {
+    const double2 a = r[g];
+    const double pr = a.x * a.x + a.y * a.y, pi = a.y * a.x - a.x * a.y;
+    f[0] += pr;
+    f[1] += pi;
+    if (g >= nge) { f[2] += pr; f[3] += pi; }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    red[threadIdx.x] = f[c];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double2* o = (c < 2 ? func : shell) + (size_t)q * np + p;
+      if (c % 2 == 0) o->x = red[0];
+      else o->y = red[0];
+    }
+    __syncthreads();
+  }
+}
+
+// X_ij(R) (slot w = nen) and J_ij(R, w) = 2*2pi sum_q f(q) / V / NTQ, heads on
+// i = j at R = 0 (calc_intJ.F:830-904).  One R: grid (nen + 1), threads over p.
+// out: X [p] and J [w * wstride + p] at this R's slot.
+__global__ void __launch_bounds__(64) k_j_sum(const double2* __restrict__ func,
+                                              const double2* __restrict__ funcw, int ntq, int nwf,
+                                              int nen, int r0, double volume, double chead,
+                                              const double2* __restrict__ cheadw,
+                                              double2* __restrict__ xmat,
+                                              double2* __restrict__ jmat, size_t wstride) {
+  const int w = blockIdx.x;
+  const bool isx = w == nen;
+  const int np = nwf * nwf;
+  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+  for (int p = threadIdx.x; p < np; p += blockDim.x) {
+    double sr = 0.0, si = 0.0;
+    for (int q = 0; q < ntq; ++q) {
+      const double2 f = isx ? func[(size_t)q * np + p] : funcw[((size_t)q * nen + w) * np + p];
+      sr += f.x;
+      si += f.y;
+    }
+    double xr = 2.0 * tpi * sr / volume / (double)ntq, xi = 2.0 * tpi * si / volume / (double)ntq;
+    const bool diag = r0 && (p % nwf) == (p / nwf);
+    if (isx) {
+      if (diag) xr += chead;
+      xmat[p] = make_double2(xr, xi);
+    } else {
+      if (diag) { xr += cheadw[w].x; xi += cheadw[w].y; }
+      jmat[(size_t)w * wstride + p] = make_double2(xr, xi);
     }
   }
 }
@@ -260,8 +374,41 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* shell, int ntq, int nwf, int nen, double2* funcw,
                        hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
-  hipLaunchKernelGGL(k_w_contract, dim3(nen, ntq), dim3(256), 0, s, eps, eps_off, rho, rho_off,
-                     qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
+  hipLaunchKernelGGL(k_w_contract<false>, dim3(nen, ntq), dim3(256), 0, s, eps, eps_off, rho,
+                     rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
+                       const long long* rho_off, const int4* qinfo, const int* minv,
+                       const double2* pf1, const double2* pf2, const long long* map_off,
+                       const double2* shell, int ntq, int npair, int nen, double2* funcw,
+                       hipStream_t s) {
+  const int groups = (npair + kMaxWf - 1) / kMaxWf;
+  hipLaunchKernelGGL(k_w_contract<true>, dim3(nen, ntq, groups), dim3(256), 0, s, eps, eps_off, rho,
+                     rho_off, qinfo, minv, pf1, pf2, map_off, shell, npair, nen, funcw);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
+                  int g0, int q0, const double2* pfk, double2* rho, hipStream_t s) {
+  hipLaunchKernelGGL(k_j_rho, dim3((ngpsi + 255) / 256, npair), dim3(256), 0, s, A, lda, ntk,
+                     ngpsi, qg, g0, q0, pfk, rho);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_j_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
+                   int npair, double2* func, double2* shell, hipStream_t s) {
+  hipLaunchKernelGGL(k_j_func, dim3(ntq, npair), dim3(256), 0, s, rho, rho_off, qinfo, npair, func,
+                     shell);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_j_sum(const double2* func, const double2* funcw, int ntq, int nwf, int nen, int r0,
+                  double volume, double chead, const double2* cheadw, double2* xmat,
+                  double2* jmat, size_t wstride, hipStream_t s) {
+  hipLaunchKernelGGL(k_j_sum, dim3(nen + 1), dim3(64), 0, s, func, funcw, ntq, nwf, nen, r0, volume,
+                     chead, cheadw, xmat, jmat, wstride);
   RP_HIP(hipGetLastError());
 }
 

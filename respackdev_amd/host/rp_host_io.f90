@@ -4,11 +4,12 @@
 ! 2000-2085; chiqw_sub.F:440-454).
 module rp_host_io
   use iso_c_binding
+  use respack_gpu
   implicit none
   private
   public :: wfn_data, read_dir_wfn, read_dir_wan, fold_q, make_dir, file_exists
   public :: write_chi_cutoff, write_wgrid, write_sq, write_epsqw, write_eps_diag, write_log
-  public :: reciprocal
+  public :: reciprocal, int_system, load_int_system, write_mvmc_sites
 
   type :: wfn_data
     real(8) :: ecut_psi = 0, ef = 0, etot = 0
@@ -21,6 +22,15 @@ module rp_host_io
     integer(c_int32_t), allocatable :: kgi(:, :, :)  ! (3, ntg, nk_irr)
     real(8), allocatable :: eig(:, :)           ! (ntb, nk_irr), Ha
     complex(8), allocatable :: c0(:, :, :)      ! (ntg, ntb, nk_irr)
+  end type
+
+  ! calc_int inputs as loaded into the library (calc_intW.F:8-630)
+  type :: int_system
+    type(c_ptr) :: h = c_null_ptr
+    integer :: nen = 0, ntk = 0, nwf = 0, nkb(3) = 0, na(3) = 0
+    complex(8), allocatable :: em(:)
+    real(8), allocatable :: coord(:, :)      ! (3, nwf) Wannier centres (bohr)
+    real(8) :: a(3, 3) = 0
   end type
 
   interface
@@ -274,6 +284,146 @@ contains
     write(u, *) '#finish chi0 evaluation for', qnum
     write(u, *) '#'
     close(u)
+  end subroutine
+
+
+  ! dir-wfn (no bands), dir-eps (chi cutoff, frequencies, q list, eps^-1 per
+  ! irreducible q, Gamma as the average of 600/601/602) and dir-wan, handed to
+  ! the library (calc_intW.F:8-630, calc_intJ.F:8-600).
+  subroutine load_int_system(sys)
+    type(int_system), intent(inout) :: sys
+    type(wfn_data) :: w
+    integer(c_int32_t) :: iout(16)
+    integer(c_int32_t), allocatable :: ng0(:), nge(:)
+    real(8) :: dout(8), ecut_eps
+    real(8), allocatable :: sk0(:, :), sqi(:, :)
+    complex(8), allocatable :: wan(:, :, :), eps(:, :, :), e3(:, :, :)
+    character(len=64) :: dirname
+    character(len=256) :: line
+    integer :: u, ik, iq, ie, ix, iw, ig, ng, nwf
+    call read_dir_wfn(w, with_bands=.false.)
+    sys%a = w%a
+    open(newunit=u, file='./dir-eps/dat.chi_cutoff', status='old', action='read')
+    read(u, *) ecut_eps
+    close(u)
+    open(newunit=u, file='./dir-eps/dat.wgrid', status='old', action='read')
+    read(u, *) sys%nen
+    allocate(sys%em(sys%nen))
+    do ie = 1, sys%nen
+      read(u, '(2f15.10)') sys%em(ie)
+    end do
+    close(u)
+    allocate(sqi(3, w%nk_irr))
+    do iq = 1, w%nk_irr
+      write(dirname, "('dir-eps/q',i3.3)") iq
+      if (.not. file_exists(trim(dirname) // '/dat.log.400')) &
+        write(6, *) 'error: no dat.log.400 in ', trim(dirname)
+      if (.not. file_exists(trim(dirname) // '/dat.sq')) call fail('no dat.sq in ' // trim(dirname))
+      open(newunit=u, file=trim(dirname) // '/dat.sq', status='old', action='read')
+      read(u, *) sqi(:, iq)
+      close(u)
+    end do
+    call rp_check(rp_init(0_c_int32_t, sys%h), 'rp_init')
+    call rp_check(rp_w3d_open(sys%h, w%a, int(w%nsym, c_int32_t), int(w%nnp, c_int32_t), w%rg, &
+                              w%pg, int(w%nk_irr, c_int32_t), w%ski, w%ngi, int(w%ntg, c_int32_t), &
+                              w%kgi, ecut_eps, int(sys%nen, c_int32_t), sys%em), 'rp_w3d_open')
+    call rp_check(rp_w3d_info(sys%h, iout, dout), 'rp_w3d_info')
+    sys%ntk = iout(1)
+    sys%nkb = iout(2:4)
+    sys%na = iout(5:7)
+    write(6, '(a,i6,a,3i4,a,3i4,a,3i4)') ' NTK=', sys%ntk, ' nkb=', sys%nkb, ' Na=', sys%na, &
+      ' FFT grid=', iout(8:10)
+    allocate(sk0(3, sys%ntk), ng0(sys%ntk))
+    call rp_check(rp_w3d_kmesh(sys%h, sk0, ng0), 'rp_w3d_kmesh')
+    ! dat.wan: NWF, then per full-mesh k ((C0(G,w),G=1,NG0(k)),w=1,NWF)
+    open(newunit=u, file='./dir-wan/dat.wan', status='old', action='read', form='unformatted')
+    read(u) nwf
+    sys%nwf = nwf
+    allocate(wan(w%ntg, nwf, sys%ntk))
+    wan = (0.0d0, 0.0d0)
+    do ik = 1, sys%ntk
+      read(u) ((wan(ig, iw, ik), ig = 1, ng0(ik)), iw = 1, nwf)
+    end do
+    close(u)
+    allocate(sys%coord(3, nwf))
+    open(newunit=u, file='./dir-wan/dat.wan-center', status='old', action='read')
+    read(u, '(a)') line
+    read(u, '(a)') line
+    do iw = 1, nwf
+      read(u, *) sys%coord(:, iw)
+    end do
+    close(u)
+    call rp_check(rp_w3d_set_wannier(sys%h, int(nwf, c_int32_t), wan), 'rp_w3d_set_wannier')
+    deallocate(wan)
+    allocate(nge(w%nk_irr))
+    call rp_check(rp_w3d_set_q(sys%h, int(w%nk_irr, c_int32_t), sqi, nge), 'rp_w3d_set_q')
+    do iq = 1, w%nk_irr
+      write(dirname, "('dir-eps/q',i3.3)") iq
+      ng = nge(iq)
+      if (allocated(eps)) deallocate(eps)
+      allocate(eps(ng, ng, sys%nen))
+      if (all(abs(w%ski(:, iq)) < 1.0d-5)) then
+        allocate(e3(ng, ng, sys%nen))
+        eps = (0.0d0, 0.0d0)
+        do ix = 1, 3
+          call read_epsqw(trim(dirname), 600 + ix - 1, e3)
+          eps = eps + e3
+        end do
+        eps = eps / 3.0d0
+        deallocate(e3)
+      else
+        call read_epsqw(trim(dirname), 600, eps)
+      end if
+      call rp_check(rp_w3d_set_epsinv(sys%h, int(iq - 1, c_int32_t), eps), 'rp_w3d_set_epsinv')
+    end do
+  end subroutine
+
+  subroutine read_epsqw(dir, num, e)
+    character(len=*), intent(in) :: dir
+    integer, intent(in) :: num
+    complex(8), intent(out) :: e(:, :, :)
+    character(len=32) :: name
+    integer :: uu
+    write(name, '("dat.epsqw.",i3.3)') num
+    if (.not. file_exists(dir // '/' // trim(name))) call fail('no ' // trim(name) // ' in ' // dir)
+    open(newunit=uu, file=dir // '/' // trim(name), form='unformatted', status='old', action='read')
+    read(uu) e
+    close(uu)
+  end subroutine
+
+  ! H(R) on the lattice of NTK sites (a3 fastest), folded onto site pairs by the
+  ! minimum-image vector (wrt_mvmc_w3d.f90, wrt_mvmc_j3d.f90): h2(:,:,is,js).
+  subroutine write_mvmc_sites(nkb, na, nwf, hr, h2)
+    integer, intent(in) :: nkb(3), na(3), nwf
+    complex(8), intent(in) :: hr(:, :, -na(1):, -na(2):, -na(3):)
+    complex(8), intent(out) :: h2(:, :, :, :)
+    integer :: nsite, isite, jsite, d, lo, t(3), ia(3), ja(3)
+    nsite = nkb(1) * nkb(2) * nkb(3)
+    lo = 0
+    if (mod(nsite, 2) == 0) lo = 1
+    do isite = 1, nsite
+      call site_vec(isite, ia)
+      do jsite = 1, nsite
+        call site_vec(jsite, ja)
+        do d = 1, 3
+          t(d) = ja(d) - ia(d)
+          if (t(d) < -na(d) + lo) t(d) = t(d) + nkb(d)
+          if (t(d) > na(d)) t(d) = t(d) - nkb(d)
+        end do
+        h2(:, :, isite, jsite) = hr(:, :, t(1), t(2), t(3))
+      end do
+    end do
+  contains
+    subroutine site_vec(is, a)
+      integer, intent(in) :: is
+      integer, intent(out) :: a(3)
+      integer :: r
+      r = is - 1
+      a(3) = mod(r, nkb(3)) - na(3) + lo
+      r = r / nkb(3)
+      a(2) = mod(r, nkb(2)) - na(2) + lo
+      a(1) = r / nkb(2) - na(1) + lo
+    end subroutine
   end subroutine
 
 end module rp_host_io

@@ -130,3 +130,15 @@ class W3d:
         heads = np.zeros(2 + 2 * nen)
         check(_lib.lib().rp_w3d_vw(self._h, ptr(v), ptr(wm), ptr(heads)))
         return v, wm, heads[0], heads[2::2] + 1j * heads[3::2]
+
+    def j3d(self, rlist):
+        """calc_j3d exchange X_ij(R), J_ij(R, w) for the lattice vectors rlist.
+        Returns X [r][j][i], J [ie][r][j][i], chead, cheadw."""
+        inf = self.info()
+        nwf, nen = inf["nwf"], inf["nen"]
+        r = np.ascontiguousarray(rlist, np.int32).reshape(-1, 3)
+        x = np.zeros((len(r), nwf, nwf), np.complex128)
+        j = np.zeros((nen, len(r), nwf, nwf), np.complex128)
+        heads = np.zeros(2 + 2 * nen)
+        check(_lib.lib().rp_j3d_run(self._h, len(r), ptr(r), ptr(x), ptr(j), ptr(heads)))
+        return x, j, heads[0], heads[2::2] + 1j * heads[3::2]

@@ -53,3 +53,24 @@ def test_w3d_matches_oracle(kind):
     assert relf(wm, o["wmat"]) <= 1e-6
     assert np.abs(v - o["vmat"]).max() * AU <= 1e-3
     assert np.abs(wm - o["wmat"]).max() * AU <= 1e-3
+
+
+def test_j3d_matches_oracle():
+    from oracle import pipeline_w3d
+    rlist = [(0, 0, 0), (1, 0, 0), (0, -1, 1)]
+    wl, s, w, W, em, sqi, eps = _w3d_case("random")
+    o = pipeline_w3d.run_j3d(w, W, sqi, eps, em, wl["ecut_eps"], rlist)
+    g = W3d(0)
+    try:
+        g.open(w, wl["ecut_eps"], em)
+        g.set_wannier(W)
+        g.set_q(sqi)
+        for iq, e in enumerate(eps):
+            g.set_epsinv(iq, e)
+        x, j, chead, cheadw = g.j3d(rlist)
+    finally:
+        g.close()
+    jo = np.moveaxis(o["J"], 1, 0)  # [ie][r][j][i]
+    print(f"X {relf(x, o['X']):.3e} J {relf(j, jo):.3e}", flush=True)
+    assert relf(x, o["X"]) <= 1e-6 and relf(j, jo) <= 1e-6
+    assert np.abs(x - o["X"]).max() * AU <= 1e-3 and np.abs(j - jo).max() * AU <= 1e-3

@@ -128,3 +128,54 @@ def test_chiqw_then_w3d_match_oracle(tmp_path):
     intra = (tmp_path / "dir-mvmc" / "zCoulombIntra.def").read_text().split("\n")
     assert int(intra[1].split()[1]) == 27 * 3
     assert (tmp_path / "dir-mvmc" / "zCoulombInter.def").exists()
+
+
+@pytest.mark.gpu
+def test_chiqw_then_j3d_match_oracle(tmp_path):
+    from oracle import pipeline_w3d
+    wl, s, wan = _write_inputs(tmp_path, nwf=2)
+    nml = (f"&param_chiqw\n Ecut_for_eps = {wl['ecut_eps']!r}\n Num_freq_grid = {wl['nen']}\n"
+           " flg_calc_type = 0\n/\n")
+    assert _run("calc_chiqw", tmp_path, nml).returncode == 0
+    r = _run("calc_j3d", tmp_path, "&param_calc_int\n calc_ifreq = 3\n ix_intJ_min = 0\n ix_intJ_max = 1\n"
+             " iz_intJ_min = -1\n iz_intJ_max = 0\n/\n")
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    w = from_system(s)
+    ecut, em = read_dir_eps_grid(str(tmp_path))
+    nen = len(em)
+    import oracle
+    sqi, eps = [], []
+    for iq, k in enumerate(w.ski):
+        qd = tmp_path / "dir-eps" / f"q{iq + 1:03d}"
+        q = np.array([float(x.replace("D", "E")) for x in (qd / "dat.sq").read_text().split()])
+        sqi.append(q)
+        ng = len(oracle.make_kg0(s.bvec, ecut, q))
+        if np.all(np.abs(k) < 1e-5):
+            e = sum(read_epsqw(str(qd / f"dat.epsqw.{600 + ix}"), ng, nen) for ix in range(3)) / 3.0
+        else:
+            e = read_epsqw(str(qd / "dat.epsqw.600"), ng, nen)
+        eps.append(e)
+    ntg = max(c.shape[1] for c in wan)
+    W = np.zeros((len(wan), 2, ntg), np.complex128)
+    for kk, c in enumerate(wan):
+        W[kk, :, :c.shape[1]] = c
+    rlist = [(a1, 0, a3) for a1 in (0, 1) for a3 in (-1, 0)]
+    o = pipeline_w3d.run_j3d(w, W, np.array(sqi), eps, em, ecut, rlist)
+    for name, want in (("dat.Xmat", o["X"]), ("dat.Jmat", o["J"][:, 2])):
+        lines = (tmp_path / "dir-intJ" / name).read_text().split("\n")[3:]
+        pos = 0
+        got = np.zeros_like(want)
+        for ir, R in enumerate(rlist):
+            assert [int(x) for x in lines[pos].split()] == list(R)
+            pos += 1
+            for _ in range(4):
+                i, j, re, im = lines[pos].split()
+                got[ir, int(j) - 1, int(i) - 1] = float(re) + 1j * float(im)
+                pos += 1
+            pos += 1
+        d = np.abs(got - want * AU).max()
+        print(f"{name} max diff {d:.2e} eV", flush=True)
+        assert d <= 2e-10
+    for f in ("zHund.def", "zExchange.def", "zPairHop.def"):
+        assert (tmp_path / "dir-mvmc" / f).exists()

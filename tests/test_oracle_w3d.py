@@ -158,3 +158,24 @@ def test_w3d_v_hermitian_in_r():
     np.testing.assert_allclose(np.conj(np.swapaxes(vr, -1, -2)), v, rtol=1e-12, atol=1e-13)
     # time-reversal partners were exercised
     assert -1 in o["trsq"]
+
+
+def test_j3d_diagonal_at_r0_equals_w3d():
+    """At R = 0 the exchange pair density of (i, i) is the direct one of i, so
+    X_ii(0) = V_ii(0) and J_ii(0, w) = W_ii(0, w) (calc_intJ.F vs calc_intW.F)."""
+    from oracle import pipeline_w3d
+    wl, s, w, W, em, sqi, eps = _w3d_case("random", nwf=2)
+    o = pipeline_w3d.run_w3d(w, W, sqi, eps, em, wl["ecut_eps"])
+    j = pipeline_w3d.run_j3d(w, W, sqi, eps, em, wl["ecut_eps"], [(0, 0, 0)])
+    na = o["na"][0]
+    for i in range(2):
+        np.testing.assert_allclose(j["X"][0, i, i], o["vmat"][na, na, na, i, i], rtol=1e-12)
+        np.testing.assert_allclose(j["J"][0, :, i, i], o["wmat"][:, na, na, na, i, i], rtol=1e-12)
+
+
+def test_j3d_identity_eps_gives_j_equal_x():
+    from oracle import pipeline_w3d
+    wl, s, w, W, em, sqi, eps = _w3d_case("identity", nwf=2)
+    j = pipeline_w3d.run_j3d(w, W, sqi, eps, em, wl["ecut_eps"], [(0, 0, 0), (1, 0, -1)])
+    for ie in range(len(em)):
+        np.testing.assert_allclose(j["J"][:, ie], j["X"], rtol=1e-12, atol=1e-14)
