@@ -24,14 +24,37 @@ namespace rp {
 // 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
 constexpr int kMaxGrid = 10000;
 
+// Twiddle tables in HBM for every line length N = 1..32, one row per output
+// index: T_N[k][j] = w^{(jk) mod N} with w = exp(-2 pi i/N) (forward table) or
+// its conjugate (backward table), at tab[dir*kTwHalf + off(N) + k*N + j].  A line
+// DFT reads row k with offsets fixed at compile time from a wave-uniform base,
+// so the twiddles arrive through scalar loads into SGPRs: no LDS traffic, no
+// vector instructions and no per-term index arithmetic.
+constexpr int kTwMax = 32;
+constexpr int kTwHalf = (kTwMax - 1) * kTwMax * (2 * kTwMax - 1) / 6 + kTwMax * kTwMax;
+__host__ __device__ constexpr int tw_off(int n) { return (n - 1) * n * (2 * n - 1) / 6; }
+__device__ __forceinline__ const double2* twiddles_of(const double2* tab, int n, int sign) {
+  return tab + (sign > 0 ? kTwHalf : 0) + tw_off(n);
+}
+__global__ void k_fill_twiddles(double2* __restrict__ tab) {
+  for (int n = 1; n <= kTwMax; ++n)
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+      const int k = e / n, j = e % n, m = (j * k) % n;
+      double s, c;
+      sincospi(2.0 * (double)m / (double)n, &s, &c);
+      tab[tw_off(n) + e] = make_double2(c, -s);
+      tab[kTwHalf + tw_off(n) + e] = make_double2(c, s);
+    }
+}
+
+
 // Length-N DFT of the lines of one axis, in place in LDS: out[k] = sum_j v[j]
-// w^{jk}, w = exp(-2 pi i / N) (sign < 0) or its conjugate (sign > 0).  Each
-// thread transforms whole lines (loaded into registers first); the caller
+// T[k][j] with T the forward or backward twiddle table of N.  Each thread
+// transforms whole lines (loaded into registers first); the caller
 // synchronises between axes.
 template <int N>
 __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int axis, int nx, int ny,
-                                                                 int nz, int sign,
-                                                                 const double2* tw) {
+                                                                 int nz, const double2* tw) {
   int nlines, stride;
   if (axis == 0) { nlines = ny * nz; stride = 1; }
   else if (axis == 1) { nlines = nx * nz; stride = nx; }
@@ -50,16 +73,15 @@ __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int
       for (int j = 0; j < N; ++j) v[j] = g[base + j * stride];
 #pragma unroll 1
       for (int k = 0; k < N; ++k) {
+        const double2* row = tw + k * N;
         double re = 0.0, im = 0.0;
-        int idx = 0;
 #pragma unroll
         for (int j = 0; j < N; ++j) {
-          const double2 w = tw[idx];
-          const double wi = sign > 0 ? -w.y : w.y;
-          re += v[j].x * w.x - v[j].y * wi;
-          im += v[j].x * wi + v[j].y * w.x;
-          idx += k;
-          if (idx >= N) idx -= N;
+          const double2 w = row[j];
+          re = fma(v[j].x, w.x, re);
+          re = fma(-v[j].y, w.y, re);
+          im = fma(v[j].x, w.y, im);
+          im = fma(v[j].y, w.x, im);
         }
         g[base + k * stride] = make_double2(re, im);
       }
@@ -72,10 +94,10 @@ __device__ __attribute__((always_inline)) inline void dft_axis_n(double2* g, int
 template <int MAXN>
 __device__ __attribute__((always_inline)) inline void dft_axis(double2* g, int axis, int n, int nx,
                                                                int ny, int nz, int sign,
-                                                               const double2* tw) {
+                                                               const double2* tab) {
   switch (n) {
 #define RP_CASE(N) \
-  case N: if constexpr (N <= MAXN) dft_axis_n<N>(g, axis, nx, ny, nz, sign, tw); break;
+  case N: if constexpr (N <= MAXN) dft_axis_n<N>(g, axis, nx, ny, nz, twiddles_of(tab, N, sign)); break;
     RP_CASE(1) RP_CASE(2) RP_CASE(3) RP_CASE(4) RP_CASE(5) RP_CASE(6) RP_CASE(8)
     RP_CASE(9) RP_CASE(10) RP_CASE(12) RP_CASE(15) RP_CASE(16) RP_CASE(18) RP_CASE(20)
     RP_CASE(24) RP_CASE(25) RP_CASE(27) RP_CASE(30) RP_CASE(32)
@@ -95,35 +117,16 @@ bool dft_size_supported(int n) {
 }
 int max_grid_points() { return kMaxGrid; }
 
-// Twiddle table in HBM for every line length 1..32: w_N^m = exp(-2 pi i m/N)
-// at tw[N(N-1)/2 + m].  The line DFTs index it with wave-uniform offsets, so
-// the twiddles arrive through scalar loads into SGPRs and cost neither LDS
-// bandwidth nor vector instructions.
-constexpr int kTwMax = 32;
-constexpr int kTwEntries = kTwMax * (kTwMax + 1) / 2;
-__device__ __forceinline__ const double2* twiddles_of(const double2* tab, int n) {
-  return tab + n * (n - 1) / 2;
-}
-__global__ void k_fill_twiddles(double2* __restrict__ tab) {
-  for (int n = 1; n <= kTwMax; ++n)
-    for (int m = threadIdx.x; m < n; m += blockDim.x) {
-      double s, c;
-      sincospi(2.0 * (double)m / (double)n, &s, &c);
-      tab[n * (n - 1) / 2 + m] = make_double2(c, -s);
-    }
-}
-
 // z, then y, then x (the order of fft3_fw/fft3_bw, m_fft3d_20150826.f90:1598-
 // 1720), in place.  Ends synchronised.
 template <int MAXN>
 __device__ __attribute__((always_inline)) inline void dft3(double2* g, int nx, int ny, int nz,
-                                                           int sign, const double2* twx,
-                                                           const double2* twy, const double2* twz) {
-  dft_axis<MAXN>(g, 2, nz, nx, ny, nz, sign, twz);
+                                                           int sign, const double2* tab) {
+  dft_axis<MAXN>(g, 2, nz, nx, ny, nz, sign, tab);
   __syncthreads();
-  dft_axis<MAXN>(g, 1, ny, nx, ny, nz, sign, twy);
+  dft_axis<MAXN>(g, 1, ny, nx, ny, nz, sign, tab);
   __syncthreads();
-  dft_axis<MAXN>(g, 0, nx, nx, ny, nz, sign, twx);
+  dft_axis<MAXN>(g, 0, nx, nx, ny, nz, sign, tab);
   __syncthreads();
 }
 
@@ -140,9 +143,6 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  const double2* twx = twiddles_of(twtab, nx);
-  const double2* twy = twiddles_of(twtab, ny);
-  const double2* twz = twiddles_of(twtab, nz);
   const int k = blockIdx.x, b = blockIdx.y;
   for (int r = threadIdx.x; r < nr; r += blockDim.x) g[r] = make_double2(0.0, 0.0);
   __syncthreads();
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
     g[r] = c[i];
   }
   __syncthreads();
-  dft3<MAXN>(g, nx, ny, nz, +1, twx, twy, twz);
+  dft3<MAXN>(g, nx, ny, nz, +1, twtab);
   float2* out = ur + ((size_t)b * ntk + k) * nr;
   for (int r = threadIdx.x; r < nr; r += blockDim.x)
     out[r] = make_float2((float)g[r].x, (float)g[r].y);  // CMPLX(): single precision
@@ -172,9 +172,6 @@ __global__ void __launch_bounds__(256) k_pair_densities(
   extern __shared__ double2 lds[];
   const int nr = nx * ny * nz;
   double2* g = lds;
-  const double2* twx = twiddles_of(twtab, nx);
-  const double2* twy = twiddles_of(twtab, ny);
-  const double2* twz = twiddles_of(twtab, nz);
   const int K = blockIdx.x + k_offset;
   const int p = K / ntk, k = K % ntk;
   const int kq = ikq[k];
@@ -219,7 +216,7 @@ __global__ void __launch_bounds__(256) k_pair_densities(
     }
   }
   __syncthreads();
-  dft3<MAXN>(g, nx, ny, nz, -1, twx, twy, twz);
+  dft3<MAXN>(g, nx, ny, nz, -1, twtab);
   const double scl = 1.0 / (double)nr;
   float2* out = A + (size_t)K * lda;
   for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) {
@@ -285,10 +282,10 @@ __global__ void __launch_bounds__(64) k_q0_heads(
   }
 }
 
-size_t twiddle_entries() { return kTwEntries; }
+size_t twiddle_entries() { return 2 * (size_t)kTwHalf; }
 
 void launch_twiddles(double2* tab, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_twiddles, dim3(1), dim3(64), 0, s, tab);
+  hipLaunchKernelGGL(k_fill_twiddles, dim3(8), dim3(256), 0, s, tab);
   RP_HIP(hipGetLastError());
 }
 
