@@ -6,11 +6,15 @@
 // (3 MFMAs per real product, FP32 accumulation).
 //
 // Operand format: a "packed octet" holds 8 consecutive K of one G as
-// 32 bf16 = [Re hi x8][Re lo x8][Im hi x8][Im lo x8] (64 bytes), exactly the
-// eight k-slots one MFMA lane supplies.  The left operand (conj(a), shared by
-// every frequency) is split once per q by k_presplit into octets in HBM; the
-// right operand d_w(K) a_K(G') is formed and split per workgroup while a chunk
-// is staged, once for all four waves.
+// 32 bf16 = [Re hi x8][Re lo x8][Im hi x8][Im lo x8] (64 bytes); each plane is
+// the eight k-slots one MFMA lane supplies.  Lane group q of a wave supplies
+// octet q of a 32-K chunk, so every MFMA has real k-slots of one kind, and the
+// complex product is assembled from three accumulators:
+//   Re chi += ar br + ai bi,   ImP += ar bi,   ImN += ai br,   Im chi = ImP - ImN
+// (conj(a) b without any sign flip or select in the loop).  The left operand a
+// (shared by every frequency) is split once per q by k_presplit into octets in
+// HBM; the right operand b = d_w(K) a_K(G') is formed and split per workgroup
+// while a chunk is staged, once for all eight waves.
 //
 // Workgroup: 512 threads = 2x4 waves (two per SIMD), tile 128 G x 128 G', one
 // frequency, one K range; each wave owns 4x2 blocks of 16x16.  Per chunk of 32
@@ -118,9 +122,10 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform (SGPR)
   const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: 64 G x 32 G' each
-  const int li = lane & 15, q = lane >> 4, part = q >> 1, hh = q & 1;
+  const int li = lane & 15, q = lane >> 4;
   // this XCD's share of the (frequency, tile) pairs, walked split-major
   const int ntiles = ntile * ntile, npairs = ntiles * nen;
   const int xcd = (int)(blockIdx.x % kXcd), slot = (int)(blockIdx.x / kXcd);
@@ -138,7 +143,8 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
 
   // Staging: thread t owns left octet (k8 = t>>7, row = t&127) and right
   // octet (k8 = t>>7, col = t&127) of every chunk.
-  const int srow = t & 127, sk8 = t >> 7;
+  // sk8 is wave-uniform, so the eight weights of a thread are scalar loads
+  const int srow = t & 127, sk8 = wave >> 1;
   u4v pl[4];
   float2 pr[8], pd[8];
   auto load_chunk = [&](size_t c) {
@@ -168,11 +174,14 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
     for (int x = 0; x < 4; ++x) dr[x] = ov[x];
   };
 
-  f4v acc_re[4][2], acc_im[4][2];
+  // Re chi = sum ar br + ai bi; Im chi = sum ar bi - sum ai br, the two terms of
+  // Im in separate accumulators (ip, in) so that every MFMA takes its k-slots
+  // from one plane of the octets: no selects or sign flips in the loop.
+  f4v acc_re[4][2], acc_ip[4][2], acc_in[4][2];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
+    for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_ip[a][b] = acc_in[a][b] = f4v{0, 0, 0, 0};
 
   double2* o = out + ((size_t)z * nen + w) * (size_t)ng * ng;
   bool first = true;
@@ -189,11 +198,11 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
             double2* p = o + (size_t)hj * ng + gi;
             double2 cur = first ? make_double2(0.0, 0.0) : *p;
             cur.x += (double)acc_re[a][b][v];
-            cur.y += (double)acc_im[a][b][v];
+            cur.y += (double)acc_ip[a][b][v] - (double)acc_in[a][b][v];
             *p = cur;
           }
         }
-        acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
+        acc_re[a][b] = acc_ip[a][b] = acc_in[a][b] = f4v{0, 0, 0, 0};
       }
     first = false;
   };
@@ -203,56 +212,78 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
     stage(smem);
   }
   __syncthreads();
-  const u4v neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
   int since = 0;
   for (size_t c = c0; c < c1; ++c) {
     const int cur = (int)((c - c0) & 1);
     const unsigned char* bcur = smem + cur * kBuf;
     const bool more = c + 1 < c1;
     if (more) load_chunk(c + 1);  // in flight during the MFMAs
+    // lane group q supplies the 8 K of octet q of the chunk (k-slots 8q..8q+7)
+    bf8v Brh[2], Brl[2], Bih[2], Bil[2], Lh[4], Ll[4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int k8 = 2 * s + hh;
-      bf8v Lh[4], Ll[4], Rrh[2], Rrl[2], Rih[2], Ril[2];
+    for (int b = 0; b < 2; ++b) {
+      const u4v* p = (const u4v*)(bcur + (kTM + wn * 32 + b * 16 + li) * kRowBytes + q * 64);
+      Brh[b] = as_bf8(p[0]);
+      Brl[b] = as_bf8(p[1]);
+      Bih[b] = as_bf8(p[2]);
+      Bil[b] = as_bf8(p[3]);
+    }
+    // Re a: acc_re += ar br, acc_ip += ar bi (hi*hi, hi*lo, lo*hi)
 #pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const u4v* p = (const u4v*)(bcur + (wm * 64 + a * 16 + li) * kRowBytes + k8 * 64);
-        Lh[a] = as_bf8(p[2 * part]);
-        Ll[a] = as_bf8(p[2 * part + 1]);
-      }
+    for (int a = 0; a < 4; ++a) {
+      const u4v* p = (const u4v*)(bcur + (wm * 64 + a * 16 + li) * kRowBytes + q * 64);
+      Lh[a] = as_bf8(p[0]);
+      Ll[a] = as_bf8(p[1]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        const u4v* p = (const u4v*)(bcur + (kTM + wn * 32 + b * 16 + li) * kRowBytes + k8 * 64);
-        const u4v rh = p[0], rl = p[1], ih = p[2], il = p[3];
-        // k-slots of the real part: (Re b | Im b); of the imaginary part: (Im b | -Re b)
-        Rrh[b] = as_bf8(part ? ih : rh);
-        Rrl[b] = as_bf8(part ? il : rl);
-        Rih[b] = as_bf8(part ? (rh ^ neg) : ih);
-        Ril[b] = as_bf8(part ? (rl ^ neg) : il);
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Brh[b], acc_re[a][b], 0, 0, 0);
+        acc_ip[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Bih[b], acc_ip[a][b], 0, 0, 0);
       }
-      // three passes over the 8 blocks, so consecutive MFMAs never share an accumulator
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrh[b], acc_re[a][b], 0, 0, 0);
-          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rih[b], acc_im[a][b], 0, 0, 0);
-        }
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Brl[b], acc_re[a][b], 0, 0, 0);
+        acc_ip[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Bil[b], acc_ip[a][b], 0, 0, 0);
+      }
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrl[b], acc_re[a][b], 0, 0, 0);
-          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Ril[b], acc_im[a][b], 0, 0, 0);
-        }
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Brh[b], acc_re[a][b], 0, 0, 0);
+        acc_ip[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Bih[b], acc_ip[a][b], 0, 0, 0);
+      }
+    // Im a: acc_re += ai bi, acc_in += ai br
 #pragma unroll
-      for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rrh[b], acc_re[a][b], 0, 0, 0);
-          acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rih[b], acc_im[a][b], 0, 0, 0);
-        }
+    for (int a = 0; a < 4; ++a) {
+      const u4v* p = (const u4v*)(bcur + (wm * 64 + a * 16 + li) * kRowBytes + q * 64);
+      Lh[a] = as_bf8(p[2]);
+      Ll[a] = as_bf8(p[3]);
     }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Bih[b], acc_re[a][b], 0, 0, 0);
+        acc_in[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Brh[b], acc_in[a][b], 0, 0, 0);
+      }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Bil[b], acc_re[a][b], 0, 0, 0);
+        acc_in[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Brl[b], acc_in[a][b], 0, 0, 0);
+      }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Bih[b], acc_re[a][b], 0, 0, 0);
+        acc_in[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Brh[b], acc_in[a][b], 0, 0, 0);
+      }
     if (more) stage(smem + (cur ^ 1) * kBuf);
     if (++since == kFlushChunks) {
       since = 0;
