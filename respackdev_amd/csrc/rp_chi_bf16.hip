@@ -13,8 +13,9 @@
 //   Re chi += ar br + ai bi,   ImP += ar bi,   ImN += ai br,   Im chi = ImP - ImN
 // (conj(a) b without any sign flip or select in the loop).  The left operand a
 // (shared by every frequency) is split once per q by k_presplit into octets in
-// HBM; the right operand b = d_w(K) a_K(G') is formed and split per workgroup
-// while a chunk is staged, once for all eight waves.
+// HBM; the right operand b = d_w(K) a_K(G') is formed from the same octets
+// and split again per workgroup while a chunk is staged, once for all eight
+// waves.
 //
 // Workgroup: 512 threads = 2x4 waves (two per SIMD), tile 128 G x 128 G', one
 // frequency, one K range; each wave owns 4x2 blocks of 16x16.  Per chunk of 32
@@ -103,6 +104,13 @@ __global__ void __launch_bounds__(256) k_presplit(const float2* __restrict__ A, 
 
 __device__ __forceinline__ bf8v as_bf8(u4v x) { return __builtin_bit_cast(bf8v, x); }
 
+// Element e (0..7) of one plane pair of an octet as hi + lo in FP32.
+__device__ __forceinline__ float octet_value(u4v hi, u4v lo, int e) {
+  const unsigned sh = (e & 1) ? 16u : 0u;
+  const unsigned h = (hi[e >> 1] >> sh) & 0xffffu, l = (lo[e >> 1] >> sh) & 0xffffu;
+  return __builtin_bit_cast(float, h << 16) + __builtin_bit_cast(float, l << 16);
+}
+
 // d (complex FP64, [nen][ldd]) -> single precision copy used by the contraction.
 __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float2* __restrict__ df) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -116,9 +124,9 @@ __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float
 // Columns of A beyond ng are zero, so edge tiles run the full MFMA schedule
 // (no branches between MFMAs).
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t lda,
-             const float2* __restrict__ df, size_t ldd, size_t chunks_per_split, size_t nchunks,
-             int ng, int ntile, int nen, int nsplit, double2* __restrict__ out) {
+k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ df, size_t ldd,
+             size_t chunks_per_split, size_t nchunks, int ng, int ntile, int nen, int nsplit,
+             double2* __restrict__ out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
@@ -142,22 +150,25 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
   if (c1 > nchunks) c1 = nchunks;
 
   // Staging: thread t owns left octet (k8 = t>>7, row = t&127) and right
-  // octet (k8 = t>>7, col = t&127) of every chunk.
-  // sk8 is wave-uniform, so the eight weights of a thread are scalar loads
+  // octet (k8 = t>>7, col = t&127) of every chunk.  Both come from the
+  // presplit octets (the right one rebuilt as hi + lo, which is a to within
+  // 2^-17 relative), so the left and right tiles of an XCD's workgroups share
+  // one set of row blocks in L2.  sk8 is wave-uniform, so the eight weights of
+  // a thread are scalar loads.
   const int srow = t & 127, sk8 = wave >> 1;
-  u4v pl[4];
-  float2 pr[8], pd[8];
+  u4v pl[4], pb[4];
+  float2 pd[8];
   auto load_chunk = [&](size_t c) {
     const u4v* src = Ao + ((c * kK8 + sk8) * lda + g0 + srow) * 4;
+    const u4v* srb = Ao + ((c * kK8 + sk8) * lda + h0 + srow) * 4;
 #pragma unroll
-    for (int x = 0; x < 4; ++x) pl[x] = src[x];
-    const size_t kb = c * kKC + (size_t)sk8 * 8;
-    const float2* sa = A + kb * lda + h0 + srow;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      pr[e] = sa[(size_t)e * lda];
-      pd[e] = dww[kb + e];
+    for (int x = 0; x < 4; ++x) {
+      pl[x] = src[x];
+      pb[x] = srb[x];
     }
+    const size_t kb = c * kKC + (size_t)sk8 * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) pd[e] = dww[kb + e];
   };
   auto stage = [&](unsigned char* buf) {
     u4v* dl = (u4v*)(buf + srow * kRowBytes + sk8 * 64);
@@ -165,8 +176,10 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, const float2* __restrict__ A, size_t ld
     for (int x = 0; x < 4; ++x) dl[x] = pl[x];
     float2 v[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      v[e] = make_float2(pd[e].x * pr[e].x - pd[e].y * pr[e].y, pd[e].x * pr[e].y + pd[e].y * pr[e].x);
+    for (int e = 0; e < 8; ++e) {
+      const float ar = octet_value(pb[0], pb[1], e), ai = octet_value(pb[2], pb[3], e);
+      v[e] = make_float2(pd[e].x * ar - pd[e].y * ai, pd[e].x * ai + pd[e].y * ar);
+    }
     u4v ov[4];
     make_octet(v, ov);
     u4v* dr = (u4v*)(buf + (kTM + srow) * kRowBytes + sk8 * 64);
@@ -330,9 +343,8 @@ int bf16x3_nsplit(int ng, int nen, size_t kpad, int nwg) {
 
 int bf16x3_workgroups(int ncu) { return std::max(kXcd, ncu / kXcd * kXcd); }
 
-void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2* df, size_t ldd,
-                       size_t kpad, int ng, int nen, int nsplit, int nwg, double2* partial,
-                       hipStream_t s) {
+void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd, size_t kpad, int ng,
+                       int nen, int nsplit, int nwg, double2* partial, hipStream_t s) {
   RP_CHECK(lda % kTM == 0, "bf16x3 contraction needs lda a multiple of 128");
   RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
   RP_CHECK(nwg % kXcd == 0 && nwg > 0, "bf16x3 contraction needs a multiple of 8 workgroups");
@@ -342,7 +354,7 @@ void launch_chi_bf16x3(const void* Ao, const float2* A, size_t lda, const float2
   const size_t lds = 2 * (size_t)(kTM + kTN) * kRowBytes;
   RP_HIP(hipFuncSetAttribute((const void*)k_chi_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  hipLaunchKernelGGL(k_chi_bf16x3, dim3(nwg), dim3(512), lds, s, (const u4v*)Ao, A, lda, df, ldd, per,
+  hipLaunchKernelGGL(k_chi_bf16x3, dim3(nwg), dim3(512), lds, s, (const u4v*)Ao, lda, df, ldd, per,
                      nchunks, ng, ntile, nen, nsplit, partial);
   RP_HIP(hipGetLastError());
 }
