@@ -130,6 +130,8 @@ def main() -> int:
     from respackdev_amd import gen
     from respackdev_amd.chiqw import Chiqw, from_system
 
+    if args.workload == "c4_calc_int":
+        return calc_int_bench(args)
     wl = gen.WORKLOADS[args.workload]
     t0 = time.perf_counter()
     s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
@@ -224,6 +226,68 @@ def main() -> int:
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline: oracle on {args.cpu_pairs} pairs with {threads} threads")
         out["cpu_baseline"] = cpu_baseline(w, wl, npair, args.cpu_pairs, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def calc_int_bench(args) -> int:
+    """calc_w3d and calc_j3d at the calc_int size (--workload c4_calc_int; not
+    the default line): 6^3 mesh (NTK = NTQ = 216), 5 d-like Wannier functions,
+    chi cutoff 22.2 Ry (NG ~ 600), 32 frequencies, eps^-1 of every irreducible q
+    resident in HBM.  eps^-1 is synthetic (identity plus a seeded rank-one
+    term per frequency): the kernels do the same work for any values.  A step
+    is one rp_w3d_funcs + rp_w3d_vw (V and W over all q), and one rp_j3d_run for
+    R = 0 (X and J of one lattice vector)."""
+    from respackdev_amd import gen
+    from respackdev_amd.chiqw import from_system
+    from respackdev_amd.w3d import W3d
+
+    wl = gen.WORKLOADS["c4_calc_int"]
+    t0 = time.perf_counter()
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    w = from_system(s)
+    wan, _ = gen.make_wannier(s, wl["nwf"], wl["seed"])
+    ntg = max(c.shape[1] for c in wan)
+    W = np.zeros((len(wan), wl["nwf"], ntg), np.complex128)
+    for k, c in enumerate(wan):
+        W[k, :, :c.shape[1]] = c
+    from respackdev_amd import _lib
+    em = np.zeros(wl["nen"], np.complex128)  # the reference's gen_grid rule, 0..50 eV
+    _lib.check(_lib.lib().rp_gen_grid(wl["nen"], 50.0 / 27.21151, _lib.ptr(em)))
+    sqi = np.array([[gen._fold(x) for x in k] for k in w.ski])
+    g = W3d(0)
+    try:
+        info = g.open(w, wl["ecut_eps"], em)
+        g.set_wannier(W)
+        ng = g.set_q(sqi)
+        rng = np.random.default_rng(wl["seed"])
+        for iq, n in enumerate(ng):
+            u = rng.standard_normal((len(em), n)) + 1j * rng.standard_normal((len(em), n))
+            e = 0.01 / n * u[:, :, None] * np.conj(u[:, None, :])
+            e[:, np.arange(n), np.arange(n)] += 1.0
+            g.set_epsinv(iq, e)
+        info = g.info()
+        log(f"calc_int set up in {time.perf_counter() - t0:.1f} s: NTK {info['ntk']}, NTQ "
+            f"{info['ntq']}, {len(ng)} irreducible q, NG {ng.min()}-{ng.max()}, NWF {info['nwf']}")
+        for i in range(args.warmup):
+            g.funcs(); g.vw(); g.j3d([(0, 0, 0)])
+        tf, tv, tj = [], [], []
+        for i in range(args.steps):
+            t0 = time.perf_counter(); g.funcs(); tf.append(time.perf_counter() - t0)
+            t0 = time.perf_counter(); g.vw(); tv.append(time.perf_counter() - t0)
+            t0 = time.perf_counter(); g.j3d([(0, 0, 0)]); tj.append(time.perf_counter() - t0)
+            log(f"step {i}: funcs {tf[-1]:.3f} s, vw {tv[-1]:.3f} s, j3d(R=0) {tj[-1]:.3f} s")
+    finally:
+        g.close()
+    out = {"metric": "calc_w3d s per run (V and W over all q) and calc_j3d s per lattice vector",
+           "value": float(np.mean(tf) + np.mean(tv)), "unit": "s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
+           "w3d_funcs_s": float(np.mean(tf)), "w3d_vw_s": float(np.mean(tv)),
+           "j3d_per_r_s": float(np.mean(tj)), "dtype": "fp64 (eps^-1 complex64 as the reference)",
+           "data": "synthetic Wannier functions and eps^-1",
+           "config": {"workload": "c4_calc_int", "ntk": info["ntk"], "ntq": info["ntq"],
+                      "nwf": info["nwf"], "nen": info["nen"], "nq_irr": int(len(ng)),
+                      "ng_eps": [int(ng.min()), int(ng.max())]}}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -33,7 +33,11 @@ struct W3d {
   std::vector<double> sq;
   std::vector<int> numirrq, numrotq, trsq, rwq, nge, ngpsi;
   std::vector<std::vector<int>> lg0;
-  std::vector<std::vector<float2>> epsirr;  // complex(4) per irreducible q, (ig,jg,ie)
+  // eps^-1 of every irreducible q lives in HBM only (d_eps, complex(4) as the
+  // reference's epsirr, (ig,jg,ie) at eps_off[iq]); the host keeps which q
+  // are set and the Gamma head element eps^-1_00(w) for the Hybertsen-Louie term
+  std::vector<char> eps_have;
+  std::vector<float2> eps_head;
   std::vector<long long> rho_off, map_off, eps_off;
   std::vector<int4> qinfo;
   std::vector<int> minv;
@@ -90,9 +94,8 @@ void compute_heads(W3d& w) {
   const double qsz = std::pow(6.0 * pi * pi / (double)w.ntq / w.volume, 1.0 / 3.0);
   w.chead = (2.0 / pi) * qsz;
   w.cheadw.assign(w.nen, make_double2(0, 0));
-  const size_t ng = w.nge[w.iqgm];
   for (int ie = 0; ie < w.nen; ++ie) {
-    const float2 e = w.epsirr[w.iqgm][(size_t)w.nog0 + ng * ((size_t)w.nog0 + ng * ie)];
+    const float2 e = w.eps_head[ie];
     w.cheadw[ie] = make_double2((2.0 / pi) * (double)e.x * qsz, (2.0 / pi) * (double)e.y * qsz);
   }
 }
@@ -100,19 +103,12 @@ void compute_heads(W3d& w) {
 void check_ready(const W3d& w, const char* what) {
   RP_CHECK(w.have_wan && w.have_q, std::string(what) + ": Wannier functions or q list missing");
   for (int iq = 0; iq < w.nq_irr; ++iq)
-    RP_CHECK(!w.epsirr[iq].empty(), std::string(what) + ": eps^-1 of irreducible q " +
+    RP_CHECK(w.eps_have[iq], std::string(what) + ": eps^-1 of irreducible q " +
                                         std::to_string(iq + 1) + " missing");
 }
 
-// eps^-1 of every irreducible q and the per-q rotation maps into HBM
+// the per-q rotation maps (eps^-1 itself is already in HBM)
 void upload_eps_maps(W3d& w, hipStream_t s) {
-  w.eps_off.assign(w.nq_irr, 0);
-  size_t tot = 0;
-  for (int iq = 0; iq < w.nq_irr; ++iq) { w.eps_off[iq] = (long long)tot; tot += w.epsirr[iq].size(); }
-  w.d_eps.reserve(tot);
-  for (int iq = 0; iq < w.nq_irr; ++iq)
-    RP_HIP(hipMemcpyAsync(w.d_eps.p + w.eps_off[iq], w.epsirr[iq].data(),
-                          sizeof(float2) * w.epsirr[iq].size(), hipMemcpyHostToDevice, s));
   w.d_eps_off.upload(w.eps_off, s);
   w.d_map_off.upload(w.map_off, s);
   w.d_qinfo.upload(w.qinfo, s);
@@ -436,7 +432,17 @@ int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps
       w.rho_off[iq] = ro;
       ro += (long long)w.ngpsi[iq] * w3d_max_wannier();
     }
-    w.epsirr.assign(nq_irr, {});
+    // HBM for eps^-1 of every irreducible q, filled by rp_w3d_set_epsinv
+    w.eps_off.assign(nq_irr, 0);
+    size_t tot = 0;
+    for (int iq = 0; iq < nq_irr; ++iq) {
+      w.eps_off[iq] = (long long)tot;
+      tot += (size_t)w.nge[iq] * w.nge[iq] * w.nen;
+    }
+    RP_HIP(hipSetDevice(h->device));
+    w.d_eps.reserve(tot);
+    w.eps_have.assign(nq_irr, 0);
+    w.eps_head.assign(w.nen, make_float2(0.f, 0.f));
     if (ng_eps)
       for (int iq = 0; iq < nq_irr; ++iq) ng_eps[iq] = w.nge[iq];
     w.have_q = true;
@@ -449,11 +455,16 @@ int rp_w3d_set_epsinv(rp_handle h, int32_t iq, const double* epsinv) {
     W3d& w = w3d_of(h, "rp_w3d_set_epsinv");
     RP_CHECK(w.have_q, "rp_w3d_set_epsinv: q list not set (rp_w3d_set_q)");
     RP_CHECK(iq >= 0 && iq < w.nq_irr && epsinv, "rp_w3d_set_epsinv: bad arguments");
-    const size_t n = (size_t)w.nge[iq] * w.nge[iq] * w.nen;
+    const size_t ng = w.nge[iq], n = ng * ng * w.nen;
     // epsirr is complex(4) in the reference (calc_intW.F:586-630)
     std::vector<float2> e(n);
     for (size_t i = 0; i < n; ++i) e[i] = make_float2((float)epsinv[2 * i], (float)epsinv[2 * i + 1]);
-    w.epsirr[iq] = std::move(e);
+    if (iq == w.iqgm)
+      for (int ie = 0; ie < w.nen; ++ie)
+        w.eps_head[ie] = e[(size_t)w.nog0 + ng * ((size_t)w.nog0 + ng * ie)];
+    RP_HIP(hipSetDevice(h->device));
+    RP_HIP(hipMemcpy(w.d_eps.p + w.eps_off[iq], e.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
+    w.eps_have[iq] = 1;
     w.have_funcs = false;
   });
 }
