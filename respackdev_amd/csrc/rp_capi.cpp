@@ -575,13 +575,14 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
       h->d_part.reserve((size_t)nsplit * per);
       h->d_Ao.reserve(h->kpad * h->lda * 8);
       h->d_wf.reserve((size_t)nen * h->kpad);
+      h->d_sync.reserve(bf16x3_sync_words());
       launch_weights_to_f32(h->d_w.p, (size_t)nen * h->kpad, h->d_wf.p, s);
       for (int ix = 0; ix < nc; ++ix) {
         const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
         launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
         RP_HIP(hipEventRecord(h->gev[2 * ix], s));
         launch_chi_bf16x3(h->d_Ao.p, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit, nwg,
-                          h->d_part.p, s);
+                          h->d_part.p, h->d_sync.p, s);
         RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
         launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
       }

@@ -55,6 +55,9 @@ constexpr int kK8 = kKC / 8;          // octets per row per chunk
 constexpr int kRowBytes = kK8 * 64 + 16;  // padded LDS row (bytes)
 constexpr int kFlushChunks = 128;     // FP32 accumulation span: 4096 K
 constexpr int kXcd = 8;               // XCDs of gfx950 (workgroup b -> XCD b % 8)
+constexpr int kSyncChunks = 32;       // chunks between rendezvous of an XCD's workgroups
+constexpr int kSyncSpin = 512;        // bound on the polls of one rendezvous
+constexpr int kSyncStride = 32;       // counters 128 bytes apart (one per XCD)
 }  // namespace
 
 __device__ __forceinline__ unsigned short bf16_bits(float x) {
@@ -126,7 +129,7 @@ __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ df, size_t ldd,
              size_t chunks_per_split, size_t nchunks, int ng, int ntile, int nen, int nsplit,
-             double2* __restrict__ out) {
+             double2* __restrict__ out, unsigned* __restrict__ sync) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
@@ -140,6 +143,16 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
   const int nslot = (int)(gridDim.x / kXcd);
   const int mypairs = (npairs - xcd + kXcd - 1) / kXcd;
   const int nitems = mypairs * nsplit;
+  // Rendezvous of the XCD's workgroups every kSyncChunks chunks (split phase:
+  // arrive, keep working for one chunk, then wait).  Without it the workgroups
+  // drift apart by more chunks than L2 holds (measured: L2 hit rate 45% instead
+  // of 84% on real pair densities) and each fetches the shared operand blocks
+  // itself.  The wait is bounded (kSyncSpin polls), so a missing workgroup can
+  // only cost time, never block: every wave reaches the end of the kernel.
+  unsigned* const xsync = sync + xcd * kSyncStride;
+  unsigned epoch = 0;
+  int phase = 0;  // chunks since the last arrival; waits at phase == 1 after an arrival
+  bool waiting = false;
   for (int item = slot; item < nitems; item += nslot) {
   const int z = item / mypairs, u = (item % mypairs) * kXcd + xcd;
   const int w = u / ntiles, tile = u % ntiles;
@@ -302,6 +315,23 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
       since = 0;
       flush();
     }
+    if (t == 0) {
+      if (waiting) {
+        const unsigned target = epoch * (unsigned)nslot;
+        for (int i = 0; i < kSyncSpin; ++i) {
+          if (__hip_atomic_load(xsync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+          __builtin_amdgcn_s_sleep(4);
+        }
+      }
+      if (++phase == kSyncChunks) {
+        __hip_atomic_fetch_add(xsync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ++epoch;
+        phase = 0;
+        waiting = true;
+      } else {
+        waiting = false;
+      }
+    }
     __syncthreads();
   }
   if (since || first) flush();
@@ -343,11 +373,16 @@ int bf16x3_nsplit(int ng, int nen, size_t kpad, int nwg) {
 
 int bf16x3_workgroups(int ncu) { return std::max(kXcd, ncu / kXcd * kXcd); }
 
+size_t bf16x3_sync_words() { return (size_t)kXcd * kSyncStride; }
+
 void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd, size_t kpad, int ng,
-                       int nen, int nsplit, int nwg, double2* partial, hipStream_t s) {
+                       int nen, int nsplit, int nwg, double2* partial, unsigned* sync,
+                       hipStream_t s) {
   RP_CHECK(lda % kTM == 0, "bf16x3 contraction needs lda a multiple of 128");
   RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
   RP_CHECK(nwg % kXcd == 0 && nwg > 0, "bf16x3 contraction needs a multiple of 8 workgroups");
+  RP_CHECK(sync != nullptr, "bf16x3 contraction needs its rendezvous counters");
+  RP_HIP(hipMemsetAsync(sync, 0, sizeof(unsigned) * bf16x3_sync_words(), s));
   const int ntile = (ng + kTM - 1) / kTM;
   const size_t nchunks = kpad / kKC;
   const size_t per = (nchunks + nsplit - 1) / nsplit;
@@ -355,7 +390,7 @@ void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd,
   RP_HIP(hipFuncSetAttribute((const void*)k_chi_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
   hipLaunchKernelGGL(k_chi_bf16x3, dim3(nwg), dim3(512), lds, s, (const u4v*)Ao, lda, df, ldd, per,
-                     nchunks, ng, ntile, nen, nsplit, partial);
+                     nchunks, ng, ntile, nen, nsplit, partial, sync);
   RP_HIP(hipGetLastError());
 }
 

@@ -100,6 +100,7 @@ struct rp_ctx {
   DevBuf<float2> d_A, d_A3;
   DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand
   DevBuf<float2> d_wf;         // single-precision copy of the weights
+  DevBuf<unsigned> d_sync;     // per-XCD rendezvous counters of the split-BF16 contraction
   DevBuf<double2> d_w, d_part, d_chi, d_eps;
   DevBuf<double> d_qn, d_wd;
 

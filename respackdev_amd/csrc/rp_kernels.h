@@ -48,8 +48,10 @@ void launch_presplit(const float2* A, size_t lda, size_t kpad, void* Ao, hipStre
 void launch_weights_to_f32(const double2* dw, size_t n, float2* df, hipStream_t s);
 int bf16x3_nsplit(int ng, int nen, size_t kpad, int nwg);
 int bf16x3_workgroups(int ncu);
+size_t bf16x3_sync_words();
 void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd, size_t kpad, int ng,
-                       int nen, int nsplit, int nwg, double2* partial, hipStream_t s);
+                       int nen, int nsplit, int nwg, double2* partial, unsigned* sync,
+                       hipStream_t s);
 
 // rp_dyson.hip
 void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
