@@ -336,6 +336,10 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
   }
   if (since || first) flush();
   }  // item
+  // A workgroup out of items releases every later rendezvous of its XCD (its
+  // peers would otherwise wait out the spin bound at each one); 2^22 exceeds
+  // any epoch * nslot the counter reaches (nslot <= 256, epochs < 2^14).
+  if (t == 0) __hip_atomic_fetch_add(xsync, 1u << 22, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 size_t bf16x3_lda_multiple() { return kTM; }
