@@ -1,0 +1,126 @@
+"""A synthetic crystal given two ways: as the irreducible k list of a 4_1 screw
+axis (the four rotations C4z^j, each with the fractional translation
+(0, 0, j/4), nnp = 4) plus time reversal, and as the full mesh with the identity
+only.  The full-mesh wavefunctions are made from the irreducible ones by the
+oracle's make_C0 (chiqw_sub.F:264-322), which is pinned against the compiled
+reference for rotations with fractional translations, in the unfolding order
+of chiqw.F:437-480.  chi computed from either form must agree: that exercises
+the symmetry path (nsym > 1, rotations, fractional translations, umklapp RW)
+of whatever computes it.
+"""
+import numpy as np
+
+import oracle
+from oracle import ptr
+from respackdev_amd import gen
+from respackdev_amd.chiqw import WfnData
+
+# C4 about z acting on fractional k (k'_i = sum_j rg(i,j) k_j) and its powers
+_C4 = np.array([[0, -1, 0], [1, 0, 0], [0, 0, 1]])
+ROT = [np.linalg.matrix_power(_C4, j) for j in range(4)]
+TAU = [(0, 0, j) for j in range(4)]   # units of 1/nnp
+NNP = 4
+
+
+def _rg_rows(rots):
+    """rg(i,j) with i fastest (the dat.symmetry / rp_chiqw_open layout)."""
+    return np.ascontiguousarray([r.T.reshape(-1) for r in rots], np.int32)
+
+
+def _fold3(k):
+    return np.array([gen._fold(x) for x in k])
+
+
+def irreducible(nk):
+    """One k per orbit of {+-R^j k} on the Gamma-centred nk^3 mesh."""
+    keep = []
+    seen = set()
+
+    def key(k):
+        return tuple(np.round(_fold3(k) * nk).astype(int) % nk)
+
+    for n1 in range(nk):
+        for n2 in range(nk):
+            for n3 in range(nk):
+                k = _fold3(np.array([n1, n2, n3]) / nk)
+                if key(k) in seen:
+                    continue
+                keep.append(k)
+                for r in ROT:
+                    for s in (1, -1):
+                        seen.add(key(s * (r @ k)))
+    return np.array(keep)
+
+
+def make(nk=4, nocc=2, nvir=4, ecut_psi=4.5, seed=77, a=7.0):
+    """Returns (wA, wB): WfnData with the screw symmetry (irreducible k) and
+    the identity-only full mesh of the same crystal."""
+    lat = np.eye(3) * a
+    b = 2.0 * np.pi * np.linalg.inv(lat).T
+    ski = irreducible(nk)
+    rng = np.random.Generator(np.random.Philox(seed))
+    ntb = nocc + nvir
+    kgs, coefs, eigs = [], [], []
+    for k in ski:
+        kg = oracle.make_kg0(b, ecut_psi, k)
+        p = (k[None, :] + kg) @ b
+        e2 = np.einsum("ij,ij->i", p, p)
+        z = rng.standard_normal((len(kg), ntb)) + 1j * rng.standard_normal((len(kg), ntb))
+        z *= np.exp(-e2 / 8.0)[:, None]
+        q, _ = np.linalg.qr(z)
+        kgs.append(kg)
+        coefs.append(np.ascontiguousarray(q.T))
+        eo = np.sort(rng.uniform(-10.0, 0.0, nocc))
+        ev = np.sort(rng.uniform(1.0, 40.0, nvir))
+        eigs.append(np.concatenate([eo, ev]) / gen.AU_EV)
+    rg = _rg_rows(ROT)
+    pg = np.array(TAU, np.int32)
+
+    def wfn_data(klist, kglist, coeflist, eiglist, rgv, pgv, nnp):
+        nkl = len(klist)
+        ngi = np.array([len(x) for x in kglist], np.int32)
+        ntg = int(ngi.max())
+        kgi = np.zeros((nkl, ntg, 3), np.int32)
+        wfn = np.zeros((nkl, ntb, ntg), np.complex128)
+        for i in range(nkl):
+            kgi[i, :ngi[i]] = kglist[i]
+            wfn[i, :, :ngi[i]] = coeflist[i]
+        return WfnData(lattice=lat, rg=rgv, pg=pgv, nnp=nnp, ski=np.ascontiguousarray(klist),
+                       ngi=ngi, kgi=kgi, eig=np.ascontiguousarray(eiglist), wfn=wfn,
+                       ef=0.5 / gen.AU_EV, ecut=ecut_psi)
+
+    wA = wfn_data(ski, kgs, coefs, np.array(eigs), rg, pg, NNP)
+
+    # the full mesh in the reference's unfolding order, each k with its own sphere
+    n = 2 * len(ski) * len(ROT)
+    sk0 = np.zeros((n, 3)); numirr = np.zeros(n, np.int32); numrot = np.zeros(n, np.int32)
+    trs = np.zeros(n, np.int32); rw = np.zeros((n, 3), np.int32)
+    ntk = oracle.lib().ro_unfold_k(len(ski), len(ROT), ptr(np.ascontiguousarray(ski)), ptr(rg),
+                                   ptr(sk0), ptr(numirr), ptr(numrot), ptr(trs), ptr(rw))
+    assert ntk == nk ** 3, ntk
+    L = oracle.lib()
+    kgB, coefB, eigB = [], [], []
+    for j in range(ntk):
+        ik, op, t = int(numirr[j]), int(numrot[j]), int(trs[j])
+        kgt = np.ascontiguousarray(oracle.make_kg0(b, ecut_psi, sk0[j]), np.int32)
+        src = kgs[ik]
+        lmax = int(max(np.abs(src).max(), np.abs(kgt).max())) + 3
+        pack = np.zeros((2 * lmax + 1,) * 3, np.int32)
+        pack[src[:, 2] + lmax, src[:, 1] + lmax, src[:, 0] + lmax] = np.arange(1, len(src) + 1)
+        rinv = np.linalg.inv(ROT[op])
+        # every rotated G must land in the irreducible sphere (checked before the C call)
+        back = (np.rint(rinv) @ (t * kgt + rw[j]).T).T.astype(int)
+        assert np.all(np.abs(back) <= lmax - 1)
+        assert np.all(pack[back[:, 2] + lmax, back[:, 1] + lmax, back[:, 0] + lmax] > 0)
+        rginv = np.ascontiguousarray(rinv.T.reshape(-1), np.float64)   # rginv(i,j) at i+3j
+        c = np.zeros((ntb, len(kgt)), np.complex128)
+        for ib in range(ntb):
+            out = np.zeros(len(kgt), np.complex128)
+            L.ro_make_c0(t, len(kgt), ptr(kgt), ptr(np.ascontiguousarray(rw[j])), ptr(rginv),
+                         ptr(np.ascontiguousarray(pg[op])), NNP, lmax, lmax, lmax, ptr(pack),
+                         ptr(np.ascontiguousarray(coefs[ik][ib])), ptr(out))
+            c[ib] = out
+        kgB.append(kgt); coefB.append(c); eigB.append(eigs[ik])
+    ident = np.eye(3, dtype=np.int32).reshape(1, 9).copy()
+    wB = wfn_data(sk0[:ntk], kgB, coefB, np.array(eigB), ident, np.zeros((1, 3), np.int32), 1)
+    return wA, wB

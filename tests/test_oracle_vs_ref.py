@@ -288,3 +288,38 @@ def test_make_c0_identity_and_time_reversal():
         oracle.ref().rpref_make_c0(n, trs, n, ptr(kgt), ptr(rw), ptr(rginv), ptr(pg), 1, L1, L2, L3,
                                    ptr(pack), ptr(occ), ptr(r))
         np.testing.assert_array_equal(a, r)
+
+
+@pytest.mark.parametrize("op", [1, 2, 3])
+@pytest.mark.parametrize("itrs", [1, -1])
+def test_make_c0_rotations_with_fractional_translations(op, itrs):
+    """make_C0 (chiqw_sub.F:264-322) for cubic operations other than the
+    identity: C4 about z, C3 about (111) and the inversion, each with a
+    fractional translation (units of 1/nnp, nnp = 4), at Gamma (whose G sphere
+    every cubic operation maps onto itself) and with an umklapp RW."""
+    rot = [np.eye(3, dtype=int),
+           np.array([[0, 1, 0], [-1, 0, 0], [0, 0, 1]]),
+           np.array([[0, 0, 1], [1, 0, 0], [0, 1, 0]]),
+           -np.eye(3, dtype=int)]
+    tau = [(0, 0, 0), (1, 0, 2), (0, 3, 1), (2, 2, 2)]
+    kg = oracle.make_kg0(B7, 6.0, np.zeros(3))
+    n = len(kg)
+    rw = np.array([1, -1, 0], np.int32)
+    # the G list at the unfolded k: G + RW must rotate back into the sphere
+    kgt = np.ascontiguousarray(itrs * (kg - rw) if itrs == 1 else -(kg - rw), np.int32)
+    lmax = int(np.abs(kg).max()) + 2
+    pack = np.zeros((2 * lmax + 1,) * 3, np.int32)
+    for i, g in enumerate(kg):
+        pack[g[2] + lmax, g[1] + lmax, g[0] + lmax] = i + 1
+    rng = np.random.default_rng(20 + op)
+    occ = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    rginv = np.ascontiguousarray(np.linalg.inv(rot[op]).T.reshape(-1), np.float64)  # rginv(i,j) at i+3j
+    pg = np.array(tau[op], np.int32)
+    a = np.zeros(n, np.complex128); r = np.zeros(n, np.complex128)
+    oracle.lib().ro_make_c0(itrs, n, ptr(kgt), ptr(rw), ptr(rginv), ptr(pg), 4, lmax, lmax, lmax,
+                            ptr(pack), ptr(occ), ptr(a))
+    oracle.ref().rpref_make_c0(n, itrs, n, ptr(kgt), ptr(rw), ptr(rginv), ptr(pg), 4, lmax, lmax, lmax,
+                               ptr(pack), ptr(occ), ptr(r))
+    np.testing.assert_array_equal(a, r)
+    # every coefficient came from the irreducible list (no index fell outside the sphere)
+    assert np.all(np.abs(a) > 0)
