@@ -132,6 +132,8 @@ def main() -> int:
 
     if args.workload == "c4_calc_int":
         return calc_int_bench(args)
+    if args.workload == "c3_batch":
+        return c3_bench(args)
     wl = gen.WORKLOADS[args.workload]
     t0 = time.perf_counter()
     s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
@@ -226,6 +228,77 @@ def main() -> int:
         threads = min(os.cpu_count() or 1, 16)
         log(f"cpu baseline: oracle on {args.cpu_pairs} pairs with {threads} threads")
         out["cpu_baseline"] = cpu_baseline(w, wl, npair, args.cpu_pairs, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def c3_bench(args) -> int:
+    """The batched configuration (--workload c3_batch; not the default line):
+    every q of a 3^3 mesh on one opened system (3^3 k, 8 occupied + 200 empty
+    bands, ~1000 plane waves) with per-q chi cutoffs 17.5-24.5 Ry (NG ~420-700)
+    and per-q windows of 150-200 empty bands, back to back with the
+    wavefunctions resident.  A step is the whole q list; value is seconds per
+    q of the mixed path, with the FP64 path and the per-q errors beside it."""
+    from respackdev_amd import gen
+    from respackdev_amd.chiqw import Chiqw, from_system
+
+    wl = gen.WORKLOADS["c3_batch"]
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    w = from_system(s)
+    cases = gen.c3_windows(wl["nk"])
+    c = Chiqw(0)
+    try:
+        info = c.open(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        log(f"c3: NTK {info['ntk']}, {len(cases)} q, {info['n_occ']} occ + {info['n_vir']} empty")
+        res = {}
+        for prec in ("fp64", args.mixed):
+            for i in range(args.warmup):
+                c.set_window(cases[0][1], cases[0][2])
+                c.run_q(cases[0][0], prec, want_eps=False)
+            walls, dev, ngs, flops = [], [], [], 0.0
+            keep = []
+            for step in range(args.steps):
+                t0 = time.perf_counter()
+                devs = 0.0
+                for q, ecut, nvir in cases:
+                    c.set_window(ecut, nvir)
+                    r = c.run_q(q, prec, want_eps=False)
+                    devs += r["timings"]["device_total"]
+                    if step == 0:
+                        ng = int(len(r["lg"]))
+                        ngs.append(ng)
+                        flops += 8.0 * ng * ng * info["nen"] * c.npair * info["ntk"] * (3 if q == (0.0, 0.0, 0.0) else 1)
+                walls.append(time.perf_counter() - t0)
+                dev.append(devs)
+                log(f"{prec} step {step}: {walls[-1]:.3f} s wall, {devs:.3f} s device for {len(cases)} q")
+            res[prec] = dict(wall=float(np.mean(walls)), device=float(np.mean(dev)), ngs=ngs,
+                             flops=flops)
+        # errors of the mixed path against FP64, q by q (one pair of results held at a time)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        m = args.mixed
+        err_chi = err_eps = 0.0
+        for q, ecut, nvir in cases:
+            c.set_window(ecut, nvir)
+            r64 = c.run_q(q, "fp64", want_chi=True, want_eps=True)
+            rmx = c.run_q(q, m, want_chi=True, want_eps=True)
+            err_chi = max(err_chi, rel(rmx["chi"], r64["chi"]))
+            err_eps = max(err_eps, rel(rmx["epsinv"], r64["epsinv"]))
+    finally:
+        c.close()
+    nq = len(cases)
+    out = {"metric": "chi(q,w) s per q-point over a batch of uneven q (3^3 mesh)",
+           "value": res[m]["wall"] / nq, "unit": "s/q-point", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "higher_is_better": False, "dtype": m,
+           "fp64_s_per_q": res["fp64"]["wall"] / nq,
+           "speedup_vs_fp64": res["fp64"]["wall"] / res[m]["wall"],
+           "device_s_per_q": res[m]["device"] / nq,
+           "chi_rel_err_vs_fp64_max": err_chi, "epsinv_rel_err_vs_fp64_max": err_eps,
+           "contraction_tflop_total": res[m]["flops"] / 1e12,
+           "data": "synthetic (seeded Gaussian plane-wave coefficients, Gram-Schmidt per k)",
+           "config": {"workload": "c3_batch: a=7.0 bohr sc, 3^3 k (NTK 27), 8 occ + 200 empty, "
+                                  "every q of the mesh, per-q chi cutoff 17.5-24.5 Ry and "
+                                  "150-200 empty bands", "nq": nq,
+                      "ng_range": [min(res[m]["ngs"]), max(res[m]["ngs"])]}}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -33,7 +33,7 @@ module respack_gpu
   public :: rp_algn235, rp_gen_grid, rp_make_kg0, rp_ttrhdrn_mkidx
   public :: rp_chiqw_open, rp_chiqw_set_crpa, rp_chiqw_info, rp_chiqw_kmesh
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
-  public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair
+  public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
   public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run
 
@@ -159,6 +159,12 @@ module respack_gpu
       import :: c_int, c_int32_t, c_ptr
       type(c_ptr), value :: h
       integer(c_int32_t) :: npair
+    end function
+    integer(c_int) function rp_chiqw_set_window(h, ecut_for_eps, n_vir) bind(C, name="rp_chiqw_set_window")
+      import :: c_ptr, c_int, c_double, c_int32_t
+      type(c_ptr), value :: h
+      real(c_double), value :: ecut_for_eps
+      integer(c_int32_t), value :: n_vir
     end function
     ! calc_w3d (calc_intW.F)
     integer(c_int) function rp_w3d_open(h, lattice, nsym, nnp, rg, pg, nk_irr, ski, ngi, ntg, &

@@ -129,6 +129,12 @@ int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_o
                double* chi_out, double* timings);
 /* Device-resident number of pairs (after the Max_excitation_energy filter). */
 int rp_chiqw_npair(rp_handle h, int32_t* npair);
+/* Per-q window for the q points prepared after this call (the batched
+ * configuration's uneven G counts and band windows): ecut_for_eps (Ry) replaces
+ * the chi cutoff of rp_chiqw_open (chiqw.F:786-799 with a per-q Ecut_for_eps)
+ * and n_vir keeps only the lowest n_vir empty bands in the pair list (a per-q
+ * N_CALC_BAND, chiqw.F:1267-1295); 0 restores the value of rp_chiqw_open. */
+int rp_chiqw_set_window(rp_handle h, double ecut_for_eps, int32_t n_vir);
 
 /* ------------------------------------------------------------------------
  * calc_int drop-in, calc_w3d (replaces calc_intW.F:8-985): Wannier-basis bare

@@ -106,10 +106,15 @@ def setup(w, ecut_for_eps=0.0, num_freq_grid=70, n_calc_band=0, shift_ef=0.0,
                 dmnr=dmnr, ntg=ntg)
 
 
-def run_q(w, q, nthreads=0, prob=None, **kw):
-    """Returns chi [dir][ie][jg][ig], epsinv (same), and pair-0 M and x."""
+def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
+    """Returns chi [dir][ie][jg][ig], epsinv (same), and pair-0 M and x.
+    ecut_eps_q / n_vir_window: the per-q window of the batched configuration
+    (chi cutoff for this q, lowest empty bands kept in the pair list; the
+    frequency grid stays the one of the whole band set)."""
     L = lib()
     s = setup(w, **kw)
+    if ecut_eps_q > 0.0:
+        s["ecut_eps"] = ecut_eps_q
     b = s["b"]
     ntk, ntg = s["ntk"], s["ntg"]
     sq = np.zeros(3)
@@ -129,8 +134,9 @@ def run_q(w, q, nthreads=0, prob=None, **kw):
         ikq[k] = L.ro_search_kq(ntk, ptr(s["sk0"]), sq[0], sq[1], sq[2], k, ptr(sg))
         shift[k] = sg
     pocc, pvir = [], []
+    nvir = min(n_vir_window, s["n_vir"]) if n_vir_window > 0 else s["n_vir"]
     for jb in range(s["n_occ"]):
-        for ib in range(s["n_vir"]):
+        for ib in range(nvir):
             vb = s["n_docc"] + ib
             if abs(s["eave"][vb] - s["eave"][jb]) >= s["delta_ex"]:
                 continue

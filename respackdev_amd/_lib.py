@@ -24,7 +24,7 @@ EXPORTS = [
     "rp_init", "rp_finalize", "rp_last_error", "rp_algn235", "rp_gen_grid", "rp_make_kg0",
     "rp_ttrhdrn_mkidx", "rp_chiqw_open", "rp_chiqw_set_crpa", "rp_chiqw_info", "rp_chiqw_kmesh",
     "rp_chiqw_prepare_q", "rp_pairdens_batch", "rp_tetra_weights_batch", "rp_chi_accumulate",
-    "rp_dyson", "rp_chiqw_q", "rp_chiqw_npair",
+    "rp_dyson", "rp_chiqw_q", "rp_chiqw_npair", "rp_chiqw_set_window",
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
     "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run",
 ]
@@ -69,6 +69,7 @@ def lib():
         L.rp_dyson.argtypes = [P, P]
         L.rp_chiqw_q.argtypes = [P, P, I32, P, P, P]
         L.rp_chiqw_npair.argtypes = [P, P]
+        L.rp_chiqw_set_window.argtypes = [P, D, I32]
         L.rp_w3d_open.argtypes = [P, P, I32, I32, P, P, I32, P, P, I32, P, D, I32, P]
         L.rp_w3d_info.argtypes = [P, P, P]
         L.rp_w3d_kmesh.argtypes = [P, P, P]

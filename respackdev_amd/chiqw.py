@@ -231,6 +231,11 @@ class Chiqw:
         check(_lib.lib().rp_dyson(self._h, ptr(out)))
         return out
 
+    def set_window(self, ecut_for_eps: float = 0.0, n_vir: int = 0):
+        """Per-q chi cutoff (Ry) and empty-band window for the q points prepared
+        next (0: the values of open)."""
+        check(_lib.lib().rp_chiqw_set_window(self._h, float(ecut_for_eps), int(n_vir)))
+
     def run_q(self, q, precision: str = "fp64", want_chi: bool = False, want_eps: bool = True):
         """One q through all four stages.  want_eps / want_chi copy eps^-1 / chi
         to the host; without them the results stay in HBM."""

@@ -421,7 +421,7 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       h->qready = h->have_A = h->have_w = h->have_chi = false;
       std::memcpy(h->q, q, sizeof q);
       h->q0 = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;  // chiqw.F:1257
-      h->lg = make_kg0(h->b, h->a, h->ecut_eps, q);
+      h->lg = make_kg0(h->b, h->a, h->ecut_eps_win > 0.0 ? h->ecut_eps_win : h->ecut_eps, q);
       h->ng = (int)(h->lg.size() / 3);
       RP_CHECK(h->ng > 0, "empty chi G sphere");
       RP_CHECK(h->ng <= h->ntg, "NG_for_eps exceeds NTG (LG0 is sized NTG in the reference)");
@@ -443,8 +443,9 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       h->pocc.clear();
       h->pvir.clear();
       int nintra = 0;
+      const int nvir = h->n_vir_win > 0 ? std::min(h->n_vir_win, h->n_vir) : h->n_vir;
       for (int jb = 0; jb < h->n_occ; ++jb)
-        for (int ib = 0; ib < h->n_vir; ++ib) {
+        for (int ib = 0; ib < nvir; ++ib) {
           const int vb = h->n_docc + ib;
           if (std::fabs(h->eave[vb] - h->eave[jb]) >= h->delta_ex) continue;
           if (h->q0) {
@@ -483,6 +484,17 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
     h->prep_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (ng) *ng = h->ng;
     if (lgout) std::memcpy(lgout, h->lg.data(), sizeof(int) * std::min<size_t>(h->lg.size(), (size_t)3 * lg_cap));
+  });
+}
+
+int rp_chiqw_set_window(rp_handle h, double ecut_for_eps, int32_t n_vir) {
+  return guarded([&] {
+    RP_CHECK(h && h->open, "rp_chiqw_set_window: system not open");
+    RP_CHECK(ecut_for_eps >= 0.0 && n_vir >= 0, "rp_chiqw_set_window: negative window");
+    RP_CHECK(ecut_for_eps <= h->ecut_psi, "rp_chiqw_set_window: chi cutoff above the wavefunction cutoff");
+    if (ecut_for_eps != h->ecut_eps_win || n_vir != h->n_vir_win) h->qready = false;
+    h->ecut_eps_win = ecut_for_eps;
+    h->n_vir_win = n_vir;
   });
 }
 

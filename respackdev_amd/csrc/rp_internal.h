@@ -72,6 +72,9 @@ struct rp_ctx {
   double a[3][3], b[3][3], volume = 0;
   int nk_irr = 0, ntg = 0, ntb = 0, nsym = 0, nnp = 1;
   double ef = 0, delt = 0, dmna = 0, dmnr = 0, delta_ex = 0, ecut_psi = 0, ecut_eps = 0, emax = 0;
+  // per-q window (rp_chiqw_set_window): chi cutoff and number of empty bands in the pair list
+  double ecut_eps_win = 0;
+  int n_vir_win = 0;
   int nw[3] = {0, 0, 0};
   int ntk = 0, nkb[3] = {0, 0, 0};
   std::vector<double> sk0;

@@ -287,7 +287,21 @@ WORKLOADS = {
     # calc_w3d parity case: 3^3 mesh (time-reversal partners present), 3 Wannier functions
     "w3d_tiny": dict(a=7.0, nk=3, nocc=2, nvir=3, ecut_psi=4.5, seed=11,
                      ecut_eps=2.0, nen=14, q=(0.0, 0.0, 1.0 / 3.0), nwf=3),
+    # batching (C3): every q of a 3^3 mesh, per-q chi cutoff 17.5-24.5 Ry (NG ~420-700)
+    # and per-q windows of 150-200 empty bands (C3_WINDOWS)
+    "c3_batch": dict(a=7.0, nk=3, nocc=8, nvir=200, ecut_psi=30.97, seed=1002,
+                     ecut_eps=24.5, nen=32, q=(0.0, 0.0, 1.0 / 3.0)),
     # calc_int scale (C4): 6^3 mesh, 5 d-like Wannier functions, NG ~600
     "c4_calc_int": dict(a=7.0, nk=6, nocc=20, nvir=180, ecut_psi=30.97, seed=1003,
                         ecut_eps=22.2, nen=32, q=(0.0, 0.0, 1.0 / 6.0), nwf=5),
 }
+
+
+def c3_windows(nk: int = 3):
+    """The batched configuration's q list (every point of the nk^3 mesh, folded)
+    with its per-q chi cutoff (Ry) and empty-band window."""
+    qs = [(_fold(n1 / nk), _fold(n2 / nk), _fold(n3 / nk))
+          for n1 in range(nk) for n2 in range(nk) for n3 in range(nk)]
+    nq = len(qs)
+    return [(q, 17.5 + 7.0 * i / (nq - 1), 150 + int(round(50.0 * i / (nq - 1))))
+            for i, q in enumerate(qs)]
