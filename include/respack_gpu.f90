@@ -34,6 +34,8 @@ module respack_gpu
   public :: rp_chiqw_open, rp_chiqw_set_crpa, rp_chiqw_info, rp_chiqw_kmesh
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
+  public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
+  public :: rp_mlp_gradient, rp_mlp_train
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
   public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run
 
@@ -237,6 +239,49 @@ module respack_gpu
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s
+    end function
+    integer(c_int) function rp_mlp_open(h, nin, width, depth, nout, seed) bind(C, name="rp_mlp_open")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nin, width, depth, nout
+      integer(c_int64_t), value :: seed
+    end function
+    integer(c_int) function rp_mlp_nparams(h, n) bind(C, name="rp_mlp_nparams")
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: h
+      integer(c_int64_t) :: n
+    end function
+    integer(c_int) function rp_mlp_get_params(h, params) bind(C, name="rp_mlp_get_params")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: h
+      real(c_float) :: params(*)
+    end function
+    integer(c_int) function rp_mlp_set_params(h, params) bind(C, name="rp_mlp_set_params")
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: h
+      real(c_float) :: params(*)
+    end function
+    integer(c_int) function rp_mlp_forward(h, n, x, y) bind(C, name="rp_mlp_forward")
+      import :: c_ptr, c_int, c_int64_t, c_float
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: n
+      real(c_float) :: x(*), y(*)
+    end function
+    integer(c_int) function rp_mlp_gradient(h, n, x, t, grad, loss) bind(C, name="rp_mlp_gradient")
+      import :: c_ptr, c_int, c_int64_t, c_float, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: n
+      real(c_float) :: x(*), t(*), grad(*)
+      real(c_double) :: loss
+    end function
+    integer(c_int) function rp_mlp_train(h, n, x, t, steps, batch, lr, seed, losses) bind(C, name="rp_mlp_train")
+      import :: c_ptr, c_int, c_int32_t, c_int64_t, c_float, c_double
+      type(c_ptr), value :: h
+      integer(c_int64_t), value :: n, seed
+      real(c_float) :: x(*), t(*)
+      integer(c_int32_t), value :: steps, batch
+      real(c_double), value :: lr
+      real(c_double) :: losses(*)
     end function
   end interface
 

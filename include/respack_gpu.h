@@ -186,6 +186,31 @@ int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads);
 int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, double* jmat,
                double* heads);
 
+/* ------------------------------------------------------------------------
+ * Band-tail closure network.  No counterpart in the reference: a small MLP
+ * (GELU hidden layers, linear output, BF16 weights with FP32 accumulation)
+ * that predicts chi_full - chi_truncated for diagonal and near-diagonal
+ * entries from features of the truncated chi; the full-band method stays the
+ * fallback and the judge.  Parameters are one FP32 vector: per layer l the
+ * weights W_l[out][in] then the biases b_l[out].
+ * ------------------------------------------------------------------------ */
+/* depth hidden layers of the given width (all layers at most 128 wide);
+ * Glorot-uniform weights from seed, zero biases. */
+int rp_mlp_open(rp_handle h, int32_t nin, int32_t width, int32_t depth, int32_t nout, uint64_t seed);
+int rp_mlp_nparams(rp_handle h, int64_t* n);
+int rp_mlp_get_params(rp_handle h, float* params);
+/* Also resets the Adam state. */
+int rp_mlp_set_params(rp_handle h, const float* params);
+/* y[n][nout] = network(x[n][nin]) (host arrays). */
+int rp_mlp_forward(rp_handle h, int64_t n, const float* x, float* y);
+/* Gradient of the mean-squared error over the n samples with targets
+ * t[n][nout] with respect to every parameter (same layout), and the loss. */
+int rp_mlp_gradient(rp_handle h, int64_t n, const float* x, const float* t, float* grad, double* loss);
+/* steps Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples drawn
+ * with replacement from the n samples (uploaded once); losses[steps] optional. */
+int rp_mlp_train(rp_handle h, int64_t n, const float* x, const float* t, int32_t steps, int32_t batch,
+                 double lr, uint64_t seed, double* losses);
+
 #ifdef __cplusplus
 }
 #endif

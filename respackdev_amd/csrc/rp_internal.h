@@ -52,6 +52,7 @@ double ecut_for_psi(const double b[3][3], int nk_irr, const double* ski, const i
                     const int* kgi);
 
 struct W3d;  // calc_int state (rp_w3d.cpp)
+struct Mlp;  // band-tail closure network (rp_mlp_state.h)
 
 }  // namespace rp
 
@@ -60,6 +61,7 @@ using rp::DevBuf;
 
 struct rp_ctx {
   std::unique_ptr<rp::W3d> w3d;  // calc_int (Wannier-basis V/W), when opened
+  std::unique_ptr<rp::Mlp> mlp;  // band-tail closure network, when opened
   int device = 0;
   int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
@@ -107,7 +109,7 @@ struct rp_ctx {
   DevBuf<double2> d_w, d_part, d_chi, d_eps;
   DevBuf<double> d_qn, d_wd;
 
-  rp_ctx();   // defined where rp::W3d is complete (rp_w3d.cpp)
+  rp_ctx();   // defined where rp::W3d and rp::Mlp are complete (rp_w3d.cpp)
   ~rp_ctx();
   void release_events() {
     for (auto& e : ev)

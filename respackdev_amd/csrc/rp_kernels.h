@@ -89,4 +89,21 @@ void launch_w_fourier(const double2* func, const double2* funcw, const double* s
                       int nen, const int na[3], double volume, double chead, const double2* cheadw,
                       double2* vmat, double2* wmat, hipStream_t s);
 
+// band-tail closure network (rp_mlp.hip)
+int mlp_max_width();
+int mlp_chunk();
+void launch_to_bf16(const float* src, size_t n, unsigned short* dst, hipStream_t s);
+void launch_linear_fwd(const float* A, int n, int nin, const unsigned short* W, const float* b, int nout,
+                       int act, float* Z, float* Aout, hipStream_t s);
+void launch_linear_bwd_data(const float* dZ, int n, int nout, const unsigned short* W, int nin,
+                            const float* Zprev, int act, float* dA, hipStream_t s);
+void launch_linear_bwd_w(const float* dZ, const float* A, int n, int nin, int nout, float* part, float* gW,
+                         float* gb, hipStream_t s);
+void launch_mse(const float* Y, const float* T, size_t m, float scale, float* dY, double* lossp, int nblocks,
+                hipStream_t s);
+void launch_adam(float* p, const float* g, float* m1, float* m2, size_t n, float lr_t, float b1, float b2,
+                 float c2, float eps, hipStream_t s);
+void launch_gather_rows(const float* x, const float* t, const int* idx, int nb, int nin, int nout, float* xb,
+                        float* tb, hipStream_t s);
+
 }  // namespace rp

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "rp_internal.h"
+#include "rp_mlp_state.h"
 
 namespace rp {
 
