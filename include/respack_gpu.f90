@@ -35,7 +35,7 @@ module respack_gpu
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
   public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
-  public :: rp_mlp_gradient, rp_mlp_train
+  public :: rp_mlp_gradient, rp_mlp_train, rp_chi_set
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
   public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run
 
@@ -282,6 +282,11 @@ module respack_gpu
       integer(c_int32_t), value :: steps, batch
       real(c_double), value :: lr
       real(c_double) :: losses(*)
+    end function
+    integer(c_int) function rp_chi_set(h, chi) bind(C, name="rp_chi_set")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: chi(*)
     end function
   end interface
 

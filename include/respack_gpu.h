@@ -118,6 +118,10 @@ int rp_tetra_weights_batch(rp_handle h, double* d_out);
  * in the given RP_PREC_*.  chi_out (optional): complex, Fortran order
  * chiqw(ig,jg,ie) (times 3 directions at q = 0). */
 int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out);
+/* Replaces the chi of the prepared q (the layout rp_chi_accumulate writes:
+ * complex (ig, jg, ie[, dir])) before rp_dyson, e.g. with the band-tail
+ * closure applied.  No counterpart in the reference. */
+int rp_chi_set(rp_handle h, const double* chi_in);
 /* Dyson step calc_eps_rpa / calc_eps_rpa_q_0 (chiqw_sub.F:386-568) on the GPU
  * in FP64: eps^-1, Fortran order (ig,jg,ie) (times 3 at q = 0). */
 int rp_dyson(rp_handle h, double* epsinv_out);

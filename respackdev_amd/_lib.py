@@ -26,7 +26,7 @@ EXPORTS = [
     "rp_chiqw_prepare_q", "rp_pairdens_batch", "rp_tetra_weights_batch", "rp_chi_accumulate",
     "rp_dyson", "rp_chiqw_q", "rp_chiqw_npair", "rp_chiqw_set_window",
     "rp_mlp_open", "rp_mlp_nparams", "rp_mlp_get_params", "rp_mlp_set_params", "rp_mlp_forward",
-    "rp_mlp_gradient", "rp_mlp_train",
+    "rp_mlp_gradient", "rp_mlp_train", "rp_chi_set",
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
     "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run",
 ]
@@ -80,6 +80,7 @@ def lib():
         L.rp_mlp_forward.argtypes = [P, I64, P, P]
         L.rp_mlp_gradient.argtypes = [P, I64, P, P, P, P]
         L.rp_mlp_train.argtypes = [P, I64, P, P, I32, I32, D, U64, P]
+        L.rp_chi_set.argtypes = [P, P]
         L.rp_w3d_open.argtypes = [P, P, I32, I32, P, P, I32, P, P, I32, P, D, I32, P]
         L.rp_w3d_info.argtypes = [P, P, P]
         L.rp_w3d_kmesh.argtypes = [P, P, P]

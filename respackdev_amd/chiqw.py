@@ -225,6 +225,14 @@ class Chiqw:
         check(_lib.lib().rp_chi_accumulate(self._h, _lib.PREC[precision], ptr(out)))
         return out
 
+    def dyson_from(self, chi) -> np.ndarray:
+        """eps^-1 of the prepared q from a given chi ([dir][ie][jg][ig]; e.g.
+        with the band-tail closure applied) through the GPU Dyson step."""
+        chi = np.ascontiguousarray(chi, np.complex128)
+        assert chi.shape == (self.ndir, self.info["nen"], self.ng, self.ng)
+        check(_lib.lib().rp_chi_set(self._h, ptr(chi)))
+        return self.dyson()
+
     def dyson(self, copy: bool = True):
         nen, ng = self.info["nen"], self.ng
         out = np.zeros((self.ndir, nen, ng, ng), np.complex128) if copy else None

@@ -626,6 +626,18 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
   });
 }
 
+int rp_chi_set(rp_handle h, const double* chi_in) {
+  return guarded([&] {
+    RP_CHECK(h && h->qready, "rp_chi_set: no q prepared");
+    RP_CHECK(chi_in, "rp_chi_set: null input");
+    RP_HIP(hipSetDevice(h->device));
+    const size_t n = (size_t)h->ng * h->ng * h->nen * (h->q0 ? 3 : 1);
+    h->d_chi.reserve(n);
+    RP_HIP(hipMemcpy(h->d_chi.p, chi_in, sizeof(double2) * n, hipMemcpyHostToDevice));
+    h->have_chi = true;
+  });
+}
+
 int rp_dyson(rp_handle h, double* epsinv_out) {
   return guarded([&] {
     RP_CHECK(h && h->qready && h->have_chi, "rp_dyson: chi missing");

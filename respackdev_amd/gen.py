@@ -101,7 +101,8 @@ def _sphere(kf: np.ndarray, bvec: np.ndarray, ecut: float, gmax: int) -> tuple[n
 
 
 def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
-                ecut_psi: float = 13.9, seed: int = 1000, ef_ev: float = 0.5) -> System:
+                ecut_psi: float = 13.9, seed: int = 1000, ef_ev: float = 0.5,
+                gap_ev: float = 1.0, emax_ev: float = 40.0) -> System:
     lat = np.eye(3) * a
     bvec = 2.0 * np.pi * np.linalg.inv(lat).T  # rows b1, b2, b3
     ski = irreducible_kmesh(nk)
@@ -133,7 +134,7 @@ def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
         sysm.kg.append(kg)
         sysm.coef.append(np.ascontiguousarray(q.T))
         eo = np.sort(rng.uniform(-10.0, 0.0, nocc))
-        ev = np.sort(rng.uniform(1.0, 40.0, nvir))
+        ev = np.sort(rng.uniform(gap_ev, emax_ev, nvir))
         eig[ik] = np.concatenate([eo, ev]) / AU_EV
     sysm.eig = eig
     return sysm
