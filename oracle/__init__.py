@@ -60,6 +60,7 @@ def lib():
         L.ro_dft3.argtypes = [I32, I32, I32, I32, P, P]
         L.ro_pairdens.argtypes = [I32, P, P, I32, P, P, I32, I32, I32, I32, P, P, P]
         L.ro_vmab.argtypes = [I32, P, P, P, P, P, P, P]
+        L.ro_vmaa.argtypes = [I32, P, P, P, P, P, P, P, P]
         L.ro_zinv.restype = I32
         L.ro_zinv.argtypes = [I32, P]
         L.ro_eps_rpa.restype = I32

@@ -106,6 +106,41 @@ def setup(w, ecut_for_eps=0.0, num_freq_grid=70, n_calc_band=0, shift_ef=0.0,
                 dmnr=dmnr, ntg=ntg)
 
 
+def plasma_frequency(s, intra):
+    """Drude plasma frequency per direction at q = 0 (chiqw.F:1631-1733 and
+    1991-1993): for every intraband pair (a partially occupied band n with
+    itself) and full-mesh k, p_k = -i vm with vm = calc_VMaa(n, n, k); the
+    tetrahedron weights of ttrhdrn_simple for f = 1 and
+    g = CMPLX(E_n(k) - E_F, -delt) (single precision, no KIND); then
+    wpl = 8 sum Im(x_k) p_k^2 / (VOLUME NTK) and wd = sqrt(Re wpl)."""
+    L = lib()
+    b = s["b"]
+    ntk, nkb = s["ntk"], [int(x) for x in s["nkb"]]
+    index_kpt, imt = s["index_kpt"], s["imt"]
+    pwpl = np.zeros(3, np.complex128)
+    for n in intra:
+        pk = np.zeros((ntk, 3), np.complex128)
+        gk = np.zeros(ntk, np.complex128)
+        for k in range(ntk):
+            m = int(s["ng0"][k])
+            kg = np.ascontiguousarray(s["kg0"][k, :m])
+            c = np.ascontiguousarray(s["coef"][n, k, :m])
+            vm = np.zeros(3, np.complex128)
+            skt = np.ascontiguousarray(s["sk0"][k])
+            L.ro_vmaa(m, ptr(kg), ptr(c), ptr(c), ptr(b[0]), ptr(b[1]), ptr(b[2]), ptr(skt), ptr(vm))
+            pk[k] = -1j * vm
+            gk[k] = complex(np.float32(s["eband"][n, k] - s["ef"]), np.float32(-s["delt"]))
+        f3 = np.ones(len(index_kpt), np.complex128)
+        g3 = np.ascontiguousarray(gk[index_kpt])
+        xo = np.zeros(len(index_kpt), np.complex128)
+        L.ro_ttrhdrn_simple(s["dmna"], s["dmnr"], nkb[0], nkb[1], nkb[2], ptr(imt), ptr(f3),
+                            ptr(g3), ptr(xo))
+        p3 = pk[index_kpt]
+        pwpl += (xo.imag[:, None] * p3 * p3).sum(axis=0)
+    wpl = 8.0 * pwpl / s["volume"] / ntk
+    return np.sqrt(wpl.real)
+
+
 def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
     """Returns chi [dir][ie][jg][ig], epsinv (same), and pair-0 M and x.
     ecut_eps_q / n_vir_window: the per-q window of the batched configuration
@@ -133,7 +168,7 @@ def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
         sg = np.zeros(3, np.int32)
         ikq[k] = L.ro_search_kq(ntk, ptr(s["sk0"]), sq[0], sq[1], sq[2], k, ptr(sg))
         shift[k] = sg
-    pocc, pvir = [], []
+    pocc, pvir, intra = [], [], []
     nvir = min(n_vir_window, s["n_vir"]) if n_vir_window > 0 else s["n_vir"]
     for jb in range(s["n_occ"]):
         for ib in range(nvir):
@@ -141,9 +176,13 @@ def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
             if abs(s["eave"][vb] - s["eave"][jb]) >= s["delta_ex"]:
                 continue
             if q0 and s["tindx"][vb] == s["tindx"][jb] and s["tindx"][jb] != 0:
+                intra.append(jb)
                 continue
             pocc.append(jb)
             pvir.append(vb)
+    # cRPA (a projector is given) drops the intraband term (chiqw.F:1626-1629)
+    if prob is not None:
+        intra = []
     pocc = _i32(pocc); pvir = _i32(pvir)
     npair = len(pocc)
     nb = s["coef"].shape[0]
@@ -172,6 +211,7 @@ def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
                       ptr(b[2]), nkb[0], nkb[1], nkb[2], ptr(s["imt"]), ptr(s["index_kpt"]),
                       s["ef"], s["delt"], s["dmna"], s["dmnr"], nen, ptr(em), s["volume"],
                       ptr(chi), nthreads)
+    wd = plasma_frequency(s, intra) if q0 else np.zeros(3)
     eps = np.zeros_like(chi)
     for ix in range(chi.shape[0]):
         for ie in range(nen):
@@ -180,9 +220,9 @@ def run_q(w, q, nthreads=0, prob=None, ecut_eps_q=0.0, n_vir_window=0, **kw):
             if not q0:
                 rc = L.ro_eps_rpa(ng, ptr(lg), ptr(sq), ptr(b[0]), ptr(b[1]), ptr(b[2]), ptr(c), ptr(e))
             else:
-                rc = L.ro_eps_rpa_q0(ng, ptr(lg), ptr(b[0]), ptr(b[1]), ptr(b[2]), g0, 0.0,
+                rc = L.ro_eps_rpa_q0(ng, ptr(lg), ptr(b[0]), ptr(b[1]), ptr(b[2]), g0, float(wd[ix]),
                                      s["delt"], float(em[ie].real), ptr(c), ptr(e))
             assert rc == 0
             eps[ix, ie] = e
     return dict(chi=chi, epsinv=eps, m0=m0, x0=x0, lg=lg, em=em, npair=npair, pocc=pocc,
-                pvir=pvir, ikq=ikq, shift=shift, setup=s)
+                pvir=pvir, ikq=ikq, shift=shift, setup=s, intra=intra, wd=wd)

@@ -102,7 +102,15 @@ def _sphere(kf: np.ndarray, bvec: np.ndarray, ecut: float, gmax: int) -> tuple[n
 
 def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
                 ecut_psi: float = 13.9, seed: int = 1000, ef_ev: float = 0.5,
-                gap_ev: float = 1.0, emax_ev: float = 40.0) -> System:
+                gap_ev: float = 1.0, emax_ev: float = 40.0, n_cross: int = 0,
+                rough: bool = False) -> System:
+    """n_cross > 0 makes a metal: the top n_cross occupied and the lowest
+    n_cross empty bands cross E_F.  Their energies are, per k and sorted,
+    E_F + d_n + w_n (cos 2pi k1 + cos 2pi k2 + cos 2pi k3)/3 with d_n in
+    +-0.6 eV and w_n in [1.5, 2.5] eV (smooth bands), or uniform in
+    E_F +- 2 eV (rough=True: a Fermi surface that cuts many tetrahedra).  The
+    other bands then keep 2.5 eV from E_F, so the band classes stay in index
+    order: fully occupied, partially occupied, empty."""
     lat = np.eye(3) * a
     bvec = 2.0 * np.pi * np.linalg.inv(lat).T  # rows b1, b2, b3
     ski = irreducible_kmesh(nk)
@@ -123,6 +131,11 @@ def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
                   ef=ef_ev / AU_EV, lattice=lat, bvec=bvec, ski=ski)
     ntb = nocc + nvir
     eig = np.empty((len(ski), ntb))
+    if n_cross > 0:
+        if not (0 < n_cross <= min(nocc, nvir)):
+            raise ValueError("n_cross must be at most min(nocc, nvir)")
+        d_n = rng.uniform(-0.6, 0.6, 2 * n_cross)
+        w_n = rng.uniform(1.5, 2.5, 2 * n_cross)
     for ik, kf in enumerate(ski):
         kg, e2 = _sphere(kf, bvec, ecut_psi, gmax)
         ng = len(kg)
@@ -133,9 +146,47 @@ def make_system(a: float = 7.0, nk: int = 4, nocc: int = 8, nvir: int = 24,
         q, _ = np.linalg.qr(z)
         sysm.kg.append(kg)
         sysm.coef.append(np.ascontiguousarray(q.T))
-        eo = np.sort(rng.uniform(-10.0, 0.0, nocc))
-        ev = np.sort(rng.uniform(gap_ev, emax_ev, nvir))
+        if n_cross == 0:
+            eo = np.sort(rng.uniform(-10.0, 0.0, nocc))
+            ev = np.sort(rng.uniform(gap_ev, emax_ev, nvir))
+        else:
+            eo = np.sort(rng.uniform(-10.0, ef_ev - 2.5, nocc))
+            ev = np.sort(rng.uniform(max(gap_ev, ef_ev + 2.5), emax_ev, nvir))
+            if rough:
+                ec = np.sort(rng.uniform(ef_ev - 2.0, ef_ev + 2.0, 2 * n_cross))
+            else:
+                c = np.cos(2 * np.pi * np.asarray(kf)).sum() / 3.0
+                ec = np.sort(ef_ev + d_n + w_n * c)
+            eo[nocc - n_cross:] = ec[:n_cross]
+            ev[:n_cross] = ec[n_cross:]
         eig[ik] = np.concatenate([eo, ev]) / AU_EV
+    sysm.eig = eig
+    return sysm
+
+
+def free_electron_system(a: float = 7.0, nk: int = 6, nbands: int = 12, ecut_psi: float = 6.0,
+                         ef_ev: float = 9.0) -> System:
+    """The empty lattice: per k the nbands lowest plane waves, each band one
+    unit coefficient on its G, with E = |k+G|^2 / 2 (Ha).  A metal whose
+    Drude plasma frequency is known in closed form, w_p^2 = 4 pi n with
+    n = k_F^3 / (3 pi^2) and k_F = sqrt(2 E_F)."""
+    lat = np.eye(3) * a
+    bvec = 2.0 * np.pi * np.linalg.inv(lat).T
+    ski = irreducible_kmesh(nk)
+    gmax = int(np.ceil(np.sqrt(ecut_psi) / (2 * np.pi / a))) + 2
+    sysm = System(a=a, nk=nk, nocc=nbands // 2, nvir=nbands - nbands // 2, ecut_psi=ecut_psi,
+                  seed=0, ef=ef_ev / AU_EV, lattice=lat, bvec=bvec, ski=ski)
+    eig = np.empty((len(ski), nbands))
+    for ik, kf in enumerate(ski):
+        kg, e2 = _sphere(kf, bvec, ecut_psi, gmax)
+        if len(kg) < nbands:
+            raise ValueError(f"{len(kg)} plane waves cannot hold {nbands} bands")
+        order = np.argsort(e2, kind="stable")[:nbands]
+        c = np.zeros((nbands, len(kg)), np.complex128)
+        c[np.arange(nbands), order] = 1.0
+        sysm.kg.append(kg)
+        sysm.coef.append(c)
+        eig[ik] = e2[order] / 2.0
     sysm.eig = eig
     return sysm
 
@@ -288,6 +339,12 @@ WORKLOADS = {
     # calc_w3d parity case: 3^3 mesh (time-reversal partners present), 3 Wannier functions
     "w3d_tiny": dict(a=7.0, nk=3, nocc=2, nvir=3, ecut_psi=4.5, seed=11,
                      ecut_eps=2.0, nen=14, q=(0.0, 0.0, 1.0 / 3.0), nwf=3),
+    # metals: bands crossing E_F (partially occupied; Fermi-surface tetrahedron cuts,
+    # intraband Drude term at q = 0)
+    "metal_tiny": dict(a=7.0, nk=3, nocc=3, nvir=4, ecut_psi=4.5, seed=21, n_cross=1,
+                       ecut_eps=2.0, nen=14, q=(0.0, 0.0, 1.0 / 3.0)),
+    "metal_c1": dict(a=7.0, nk=4, nocc=8, nvir=24, ecut_psi=13.9, seed=1004, n_cross=2,
+                     ecut_eps=6.9, nen=16, q=(0.0, 0.0, 0.25)),
     # batching (C3): every q of a 3^3 mesh, per-q chi cutoff 17.5-24.5 Ry (NG ~420-700)
     # and per-q windows of 150-200 empty bands (C3_WINDOWS)
     "c3_batch": dict(a=7.0, nk=3, nocc=8, nvir=200, ecut_psi=30.97, seed=1002,

@@ -266,6 +266,21 @@ def test_vmab_matches_reference():
     np.testing.assert_allclose(v1, v2, rtol=1e-12)
 
 
+def test_vmaa_matches_reference():
+    b, kg_k, ck, kg_kq, ckq, lg, sg = _tiny_pair_inputs(seed=6)
+    n = len(kg_k)
+    b1, b2, b3 = (np.ascontiguousarray(b[i]) for i in range(3))
+    skt = np.array([0.25, -1.0 / 3.0, 0.5])
+    for c2 in (ck, np.ascontiguousarray(np.resize(ckq, n))):  # same band (intraband) and a pair
+        v1 = np.zeros(3, np.complex128); v2 = np.zeros(3, np.complex128)
+        oracle.lib().ro_vmaa(n, ptr(kg_k), ptr(ck), ptr(c2), ptr(b1), ptr(b2), ptr(b3), ptr(skt),
+                             ptr(v1))
+        ng0 = np.array([n], np.int32)
+        oracle.ref().rpref_vmaa(1, n, ptr(ng0), ptr(kg_k), ptr(ck), ptr(c2), 1, ptr(b1), ptr(b2),
+                                ptr(b3), ptr(skt), ptr(v2))
+        np.testing.assert_allclose(v1, v2, rtol=1e-12)
+
+
 def test_make_c0_identity_and_time_reversal():
     b = B7
     k = np.array([0.0, 0.25, 0.5])
