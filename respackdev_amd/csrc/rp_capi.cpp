@@ -292,9 +292,6 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     }
     h->n_occ = h->n_docc + h->n_pocc;
     h->n_vir = h->n_uocc + h->n_pocc;
-    RP_CHECK(h->n_pocc == 0,
-             "partially occupied bands (tetrahedra cut by the Fermi level) are not handled by "
-             "the GPU tetrahedron kernel");
     // ---- frequency grid (chiqw.F:1184-1192)
     double mx = -1e300, mn = 1e300;
     for (int ib = 0; ib < h->n_calc; ++ib) { mx = std::max(mx, bmax[ib]); mn = std::min(mn, bmin[ib]); }
@@ -353,6 +350,7 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     h->d_index.upload(h->index_kpt, s);
     h->d_em.upload((const double2*)h->em.data(), (size_t)h->nen, s);
     h->d_bvec.upload(bflat, 9, s);
+    h->d_sk0.upload(h->sk0, s);
     RP_HIP(hipStreamSynchronize(s));
     h->open = true;
   });
@@ -442,7 +440,7 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       // pair list (chiqw.F:1267-1295; q = 0: chiqw.F:1582-1619)
       h->pocc.clear();
       h->pvir.clear();
-      int nintra = 0;
+      h->intra.clear();
       const int nvir = h->n_vir_win > 0 ? std::min(h->n_vir_win, h->n_vir) : h->n_vir;
       for (int jb = 0; jb < h->n_occ; ++jb)
         for (int ib = 0; ib < nvir; ++ib) {
@@ -450,13 +448,15 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
           if (std::fabs(h->eave[vb] - h->eave[jb]) >= h->delta_ex) continue;
           if (h->q0) {
             const int f1 = h->tindx[vb], f2 = h->tindx[jb];
-            if (f1 == f2 && f1 != 0) { ++nintra; continue; }
+            // intraband: the Drude term instead of a pair (none in cRPA, chiqw.F:1626-1629)
+            if (f1 == f2 && f1 != 0) {
+              if (h->flg_crpa == 0) h->intra.push_back(jb);
+              continue;
+            }
           }
           h->pocc.push_back(jb);
           h->pvir.push_back(vb);
         }
-      RP_CHECK(!(h->q0 && nintra > 0 && h->flg_crpa == 0),
-               "q = 0 with partially occupied bands (intraband Drude term) is not supported");
       h->npair = (int)h->pocc.size();
       RP_CHECK(h->npair > 0, "no band pairs");
       // |q+G| for the Dyson step (chiqw_sub.F:417-432)
@@ -477,6 +477,7 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       h->d_shift.upload(h->shift, s);
       h->d_pocc.upload(h->pocc, s);
       h->d_pvir.upload(h->pvir, s);
+      if (!h->intra.empty()) h->d_intra.upload(h->intra, s);
       h->d_qn.upload(h->qn, s);
       RP_HIP(hipStreamSynchronize(s));
       h->qready = true;
@@ -509,7 +510,8 @@ static void check_status(rp_handle h, const char* what) {
   int st = 0;
   RP_HIP(hipMemcpyAsync(&st, h->d_status.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   RP_HIP(hipStreamSynchronize(h->stream));
-  if (st & 1) throw Error(std::string(what) + ": too many sub-tetrahedra in a Fermi-surface cut");
+  if (st & 1) throw Error(std::string(what) + ": a tetrahedron cut by E_F without partially occupied bands");
+  if (st & 8) throw Error(std::string(what) + ": too many sub-tetrahedra in a Fermi-surface cut");
   if (st & 2) throw Error(std::string(what) + ": bad tetrahedron degeneracy class (errexec)");
   if (st & 4) throw Error(std::string(what) + ": singular dielectric matrix (zero pivot)");
 }
@@ -560,7 +562,8 @@ int rp_tetra_weights_batch(rp_handle h, double* d_out) {
     RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
     launch_tetra_weights(h->ntk, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p, h->d_pocc.p,
                          h->d_pvir.p, h->d_ikq.p, h->d_eband.p, h->d_pband.p, h->ef, h->delt, h->dmna,
-                         h->dmnr, h->nen, h->d_em.p, h->kpad, h->npair, h->d_status.p, h->d_w.p, s);
+                         h->dmnr, h->nen, h->d_em.p, h->kpad, h->npair, h->n_pocc > 0 ? 1 : 0,
+                         h->d_status.p, h->d_w.p, s);
     check_status(h, "tetrahedron weights");
     if (d_out)
       RP_HIP(hipMemcpy2DAsync(d_out, sizeof(double2) * K, h->d_w.p, sizeof(double2) * h->kpad,
@@ -649,9 +652,18 @@ int rp_dyson(rp_handle h, double* epsinv_out) {
     h->d_gjwork.reserve(gauss_jordan_work_ints(nb));
     h->d_ipiv.reserve((size_t)nb * n);
     h->d_wd.reserve(3);
-    const double wd0[3] = {0, 0, 0};  // no intraband pairs (cRPA, or gapped fRPA): wpl = 0
-    RP_HIP(hipMemcpyAsync(h->d_wd.p, wd0, sizeof wd0, hipMemcpyHostToDevice, s));
     RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
+    if (h->q0 && !h->intra.empty()) {
+      // Drude plasma frequency of the intraband bands (fRPA metals at q = 0)
+      h->d_drude.reserve(3 * h->intra.size());
+      launch_drude(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_intra.p,
+                   (int)h->intra.size(), h->d_sk0.p, h->d_bvec.p, h->d_eband.p, h->ef, h->delt,
+                   h->dmna, h->dmnr, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p,
+                   h->volume, h->d_status.p, h->d_drude.p, h->d_wd.p, s);
+    } else {
+      const double wd0[3] = {0, 0, 0};  // no intraband pairs (cRPA, or an insulator): wpl = 0
+      RP_HIP(hipMemcpyAsync(h->d_wd.p, wd0, sizeof wd0, hipMemcpyHostToDevice, s));
+    }
     launch_eps_build(h->d_chi.p, h->d_qn.p, n, nb, h->nen, h->q0 ? 1 : 0, h->g0, h->d_wd.p, h->d_em.p,
                      h->delt, h->d_eps.p, s);
     launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_gjwork.p, h->d_status.p, s);

@@ -90,7 +90,7 @@ struct rp_ctx {
   DevBuf<double2> d_coef;
   DevBuf<int> d_kg0, d_ng0, d_imt, d_index;
   DevBuf<float2> d_ur;
-  DevBuf<double> d_eband, d_bvec;
+  DevBuf<double> d_eband, d_bvec, d_sk0;
   DevBuf<double2> d_pband, d_em;
   DevBuf<double2> d_tw;  // line-DFT twiddle table
   // per q
@@ -98,10 +98,12 @@ struct rp_ctx {
   double q[3] = {0, 0, 0};
   int ng = 0, g0 = -1, npair = 0;
   std::vector<int> lg, ikq, shift, pocc, pvir;
+  std::vector<int> intra;  // q = 0, fRPA: partially occupied bands paired with themselves
   std::vector<double> qn;
   size_t lda = 0, kpad = 0;
   double prep_seconds = 0;
-  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_status, d_ipiv, d_gjwork;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra, d_status, d_ipiv, d_gjwork;
+  DevBuf<double2> d_drude;  // per intraband band and direction: sum_k Im x(k) p(k)^2
   DevBuf<float2> d_A, d_A3;
   DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand
   DevBuf<float2> d_wf;         // single-precision copy of the weights

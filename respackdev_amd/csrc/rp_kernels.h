@@ -31,8 +31,14 @@ int tetra_freqs_per_block(int nk3, int nen);
 void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const int* index_kpt,
                           const int* pocc, const int* pvir, const int* ikq, const double* eband,
                           const double2* pband, double ef, double delt, double dmna, double dmnr,
-                          int nen, const double2* em, size_t ldd, int npair, int* status,
+                          int nen, const double2* em, size_t ldd, int npair, int cuts, int* status,
                           double2* dout, hipStream_t s);
+// q = 0 Drude plasma frequency per direction from the intraband bands -> wd[3]
+void launch_drude(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                  const int* bands, int nbands, const double* sk0, const double* bvec,
+                  const double* eband, double ef, double delt, double dmna, double dmnr, int n1,
+                  int n2, int n3, const int* imt, const int* index_kpt, double volume,
+                  int* status, double2* part, double* wd, hipStream_t s);
 
 // rp_chi_gemm.hip
 int chi_tile();

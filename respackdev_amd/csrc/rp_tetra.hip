@@ -10,9 +10,8 @@
 // frequency needs only the four vertex logarithms: Log(a/b) = Log a - Log b,
 // brought back to the principal branch.  Degenerate items (rare) use the
 // reference's closed forms as written.  Corner contributions go to LDS with
-// FP64 atomics.  Only tetrahedra not cut by E_F are handled (every system
-// without partially occupied bands, where the reference's ttct never splits);
-// the host rejects systems with partially occupied bands before launch.
+// FP64 atomics.  Tetrahedra cut by E_F (metals: partially occupied bands)
+// are left to k_tetra_cut, which runs after this kernel on the same output.
 //
 // The output is the contraction weight d[w][K] = x(k,w) (1 - P_occ P_vir)/NTK
 // with K = pair*NTK + k (the factors of chiqw.F:1404-1406).
@@ -190,15 +189,16 @@ __device__ __forceinline__ Item load_item(int it, int ntet, const int* __restric
 
 // grid: (npair, ceil(nen / nf)) blocks.  eband[band][k] energies (Ha, full
 // mesh, FP64); the tetrahedron sees them rounded to single precision (CMPLX,
-// chiqw.F:1353,1356).  Tetrahedra cut by E_F raise status bit 1, a class the
-// reference rejects (errexec) bit 2.
+// chiqw.F:1353,1356).  Tetrahedra cut by E_F are skipped when cuts_elsewhere
+// (k_tetra_cut adds them), else they raise status bit 1; a class the
+// reference rejects (errexec) raises bit 2.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_tetra_weights(
     int ntk, int n1, int n2, int n3, const int* __restrict__ imt,
     const int* __restrict__ index_kpt, const int* __restrict__ pocc,
     const int* __restrict__ pvir, const int* __restrict__ ikq, const double* __restrict__ eband,
     const double2* __restrict__ pband, double ef, double delt, double dmna, double dmnr, int nen,
-    int nf, const double2* __restrict__ em, size_t ldd, int* __restrict__ status,
-    double2* __restrict__ dout) {
+    int nf, const double2* __restrict__ em, size_t ldd, int cuts_elsewhere,
+    int* __restrict__ status, double2* __restrict__ dout) {
   extern __shared__ double2 lds[];
   const int nk3 = n1 * n2 * n3;
   const int ntet = 6 * nk3, nitem = 2 * ntet;
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const double a0 = e1[c0], a1 = e1[c1], a2 = e1[c2], a3 = e1[c3];
     const double b0 = e2[c0], b1 = e2[c1], b2 = e2[c2], b3 = e2[c3];
     if (straddles(ef, b0, b1, b2, b3) || straddles(ef, a0, a1, a2, a3)) {
-      atomicOr(status, 1);
+      if (!cuts_elsewhere) atomicOr(status, 1);
       continue;
     }
     // ttct keeps the tetrahedron iff some e2 vertex is above and some e1 vertex below E_F
@@ -390,6 +390,419 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   }
 }
 
+// ------------------------------------------------- tetrahedra cut by E_F
+// Vertex terms of calc (m_tetrahedron_20170325.F:209-300) for one
+// tetrahedron with vertex values g0..g3 of class `code` at frequency e, the
+// factor fk left out: xo(v) += fk * t_v.  The closed forms are those of the
+// reference, each evaluated directly (the uncut kernel shares logarithms).
+__device__ __forceinline__ void add_at(int i, zd v, zd& t0, zd& t1, zd& t2, zd& t3) {
+  t0.x += i == 0 ? v.x : 0.0;
+  t0.y += i == 0 ? v.y : 0.0;
+  t1.x += i == 1 ? v.x : 0.0;
+  t1.y += i == 1 ? v.y : 0.0;
+  t2.x += i == 2 ? v.x : 0.0;
+  t2.y += i == 2 ? v.y : 0.0;
+  t3.x += i == 3 ? v.x : 0.0;
+  t3.y += i == 3 ? v.y : 0.0;
+}
+
+__device__ __forceinline__ void calc_terms(int code, zd g0, zd g1, zd g2, zd g3, zd e, zd& t0,
+                                           zd& t1, zd& t2, zd& t3) {
+  t0 = t1 = t2 = t3 = mkz(0.0, 0.0);
+  const int kind = code & 15;
+  if (kind == kNormal) {  // case(0)
+    const int i0 = vtx(code, 1, 0), i1 = vtx(code, 1, 1), i2 = vtx(code, 1, 2), i3 = vtx(code, 1, 3);
+    const zd A0 = sel4(i0, g0, g1, g2, g3), A1 = sel4(i1, g0, g1, g2, g3);
+    const zd A2 = sel4(i2, g0, g1, g2, g3), A3 = sel4(i3, g0, g1, g2, g3);
+    const zd c1 = f_sub2normal(A1, A0, A2, A3, e);
+    const zd c2 = f_sub2normal(A0, A1, A2, A3, e);
+    const zd c3 = f_sub2normal(A0, A2, A1, A3, e);
+    const zd c4 = f_sub1normal(A0, A3, A1, A2, e);
+    add_at(i0, c1, t0, t1, t2, t3);
+    add_at(i1, c2, t0, t1, t2, t3);
+    add_at(i2, c3, t0, t1, t2, t3);
+    add_at(i3, c4 - c3 - c2 - c1, t0, t1, t2, t3);
+    return;
+  }
+  const int b0 = vtx(code, 0, 0), b1 = vtx(code, 0, 1), b2 = vtx(code, 0, 2), b3 = vtx(code, 0, 3);
+  const int o0 = vtx(code, 1, 0), o1 = vtx(code, 1, 1);
+  const zd G30 = sel4(b0, g0, g1, g2, g3), G31 = sel4(b1, g0, g1, g2, g3);
+  const zd G32 = sel4(b2, g0, g1, g2, g3), G33 = sel4(b3, g0, g1, g2, g3);
+  const zd G40 = sel4(o0, g0, g1, g2, g3), G41 = sel4(o1, g0, g1, g2, g3);
+  if (kind == kPair) {  // case(2)
+    const zd q1 = (G30 + G31) / 2.0;
+    const zd c1 = f_sub2g10(q1, G40, G41, e);
+    const zd c2 = f_sub2g30(q1, G40, G41, e);
+    const zd c3 = f_sub1g30(q1, G41, G40, e);
+    add_at(b0, c1, t0, t1, t2, t3);
+    add_at(b1, c1, t0, t1, t2, t3);
+    add_at(o0, c2, t0, t1, t2, t3);
+    add_at(o1, c3 - c2 - c1 - c1, t0, t1, t2, t3);
+  } else if (kind == kTriple) {  // case(3)
+    const zd q1 = (G30 + G31 + G32) / 3.0;
+    const zd c1 = f_sub2g30g10(q1, G40, e);
+    const zd c2 = f_sub1g30g20(q1, G40, e);
+    add_at(b0, c1, t0, t1, t2, t3);
+    add_at(b1, c1, t0, t1, t2, t3);
+    add_at(b2, c1, t0, t1, t2, t3);
+    add_at(o0, c2 - c1 - c1 - c1, t0, t1, t2, t3);
+  } else if (kind == kTwoPairs) {  // case(4), i3 = 2
+    const zd q1 = (G30 + G31) / 2.0, q2 = (G32 + G33) / 2.0;
+    const zd c1 = f_sub2g30g20(q1, q2, e);
+    const zd c2 = f_sub1g30g21(q1, q2, e);
+    add_at(b0, c1, t0, t1, t2, t3);
+    add_at(b1, c1, t0, t1, t2, t3);
+    add_at(b2, (c2 - c1 - c1) * 0.5, t0, t1, t2, t3);
+    add_at(b3, (c2 - c1 - c1) * 0.5, t0, t1, t2, t3);
+  } else {  // case(4), i3 = 3..6
+    const zd q1 = (G30 + G31 + G32 + G33) / 4.0;
+    const zd c1 = f_sub2simple(q1, e);
+    add_at(b0, c1, t0, t1, t2, t3);
+    add_at(b1, c1, t0, t1, t2, t3);
+    add_at(b2, c1, t0, t1, t2, t3);
+    add_at(b3, c1, t0, t1, t2, t3);
+  }
+}
+
+// A node of ttct's split tree (m_tetrahedron_20170325.F:143-207): row v holds
+// the weights of the four original corners in vertex v (the reference's
+// dwgt(:, v, t)), vol the volume fraction, marks the vertices already moved
+// onto the cutting plane (bvtx).  Only constant indices after unrolling, and
+// selects for data-dependent ones, so the node stays in registers.
+struct CutNode {
+  double r[4][4];
+  double vol;
+  unsigned marks;
+};
+
+__device__ __forceinline__ void node_values(const CutNode& n, const double (&dv)[4],
+                                            double (&val)[4]) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v)
+    val[v] = n.r[v][0] * dv[0] + n.r[v][1] * dv[1] + n.r[v][2] * dv[2] + n.r[v][3] * dv[3];
+}
+
+// First pair i1 < i2 of unmoved vertices strictly on opposite sides of the
+// plane, in the reference's loop order.
+__device__ __forceinline__ bool find_cut(const CutNode& n, const double (&val)[4], double dval,
+                                         int& i1, int& i2) {
+  bool found = false;
+  i1 = i2 = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = a + 1; b < 4; ++b) {
+      const bool ok = !found && !((n.marks >> a) & 1u) && !((n.marks >> b) & 1u) &&
+                      (val[a] - dval) * (dval - val[b]) > 0.0;
+      if (ok) {
+        found = true;
+        i1 = a;
+        i2 = b;
+      }
+    }
+  return found;
+}
+
+// Split along edge (i1, i2) where it meets the plane.  child 1 is the
+// reference's appended tetrahedron (vertex i1 moved, volume * d1), child 0
+// the one split in place (vertex i2 moved, volume * d2).
+__device__ __forceinline__ void split_node(CutNode& n, const double (&val)[4], double dval, int i1,
+                                           int i2, unsigned child) {
+  const double v1 = sel4d(i1, val[0], val[1], val[2], val[3]);
+  const double v2 = sel4d(i2, val[0], val[1], val[2], val[3]);
+  const double d1 = (v2 - dval) / (v2 - v1), d2 = (dval - v1) / (v2 - v1);
+  const int tgt = child ? i1 : i2;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const double w = sel4d(i1, n.r[0][j], n.r[1][j], n.r[2][j], n.r[3][j]) * d1 +
+                     sel4d(i2, n.r[0][j], n.r[1][j], n.r[2][j], n.r[3][j]) * d2;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) n.r[v][j] = v == tgt ? w : n.r[v][j];
+  }
+  n.vol *= child ? d1 : d2;
+  n.marks |= 1u << tgt;
+}
+
+// ttct's keep test for a node that is not split: some unmoved vertex on the
+// imod side of the plane.
+__device__ __forceinline__ bool node_kept(const CutNode& n, const double (&val)[4], double dval,
+                                          double imod) {
+  bool k = false;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) k = k || (!((n.marks >> v) & 1u) && imod * (val[v] - dval) > 0.0);
+  return k;
+}
+
+// The tetrahedra of a pair that straddle E_F (in the empty band e2 or the
+// occupied band e1), as ttrhdrn_chi0 treats them (m_tetrahedron_20170325.F:
+// 114-139): ttct cuts by e2 = E_F keeping the e2 > E_F side, then every piece
+// by e1 = E_F keeping e1 < E_F, and calc runs on each piece with f = vol*dsgn
+// and g_v = -(sum_j (e2_j - e1_j) dwgt(j,v) - i delt) dsgn, its vertex terms
+// spread back to the corners with dwgt.
+//
+// A plane moves one more vertex onto itself with every cut and needs two
+// unmoved vertices to cut again, so a tree has at most 3 cut levels per plane:
+// at most 2^6 pieces.  Each thread owns one tetrahedron and walks the leaves
+// of its tree depth first, rebuilding each from the root along its path of
+// child bits, so the pieces never need storage; both signs dsgn share the
+// pieces and the degeneracy class (chk1 depends on |g_p - g_q| and |g_p + g_q|).
+//
+// grid as k_tetra_weights, which must have run first: the pieces are added
+// to the weights it wrote, with the same 1/NTK and cRPA factor.  Status bit 2
+// as there; bit 8 if a tree had more leaves than the bound (cannot happen).
+__global__ void __launch_bounds__(256) k_tetra_cut(
+    int ntk, int n1, int n2, int n3, const int* __restrict__ imt,
+    const int* __restrict__ index_kpt, const int* __restrict__ pocc,
+    const int* __restrict__ pvir, const int* __restrict__ ikq, const double* __restrict__ eband,
+    const double2* __restrict__ pband, double ef, double delt, double dmna, double dmnr, int nen,
+    int nf, const double2* __restrict__ em, size_t ldd, int* __restrict__ status,
+    double2* __restrict__ dout) {
+  extern __shared__ double2 lds[];
+  const int nk3 = n1 * n2 * n3;
+  const int ntet = 6 * nk3;
+  double2* x = lds;                              // [nk3][nf]
+  double* e1 = (double*)(x + (size_t)nk3 * nf);  // [nk3]
+  double* e2 = e1 + nk3;                         // [nk3]
+  int* list = (int*)(e2 + nk3);                  // [ntet]
+  __shared__ int count;
+  const int p = blockIdx.x;
+  const int f0 = blockIdx.y * nf, nfb = min(nf, nen - f0);
+  const int jo = pocc[p], iv = pvir[p];
+  for (int i = threadIdx.x; i < nk3 * nf; i += blockDim.x) x[i] = make_double2(0, 0);
+  for (int gi = threadIdx.x; gi < nk3; gi += blockDim.x) {
+    const int k = index_kpt[gi];
+    e1[gi] = (double)(float)eband[(size_t)jo * ntk + k];
+    e2[gi] = (double)(float)eband[(size_t)iv * ntk + ikq[k]];
+  }
+  if (threadIdx.x == 0) count = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < ntet; t += blockDim.x) {
+    const int c0 = imt[4 * t], c1 = imt[4 * t + 1], c2 = imt[4 * t + 2], c3 = imt[4 * t + 3];
+    if (straddles(ef, e2[c0], e2[c1], e2[c2], e2[c3]) ||
+        straddles(ef, e1[c0], e1[c1], e1[c2], e1[c3]))
+      list[atomicAdd(&count, 1)] = t;
+  }
+  __syncthreads();
+  const int ncut = count;
+  if (ncut == 0) return;  // uniform over the block
+  for (int j = threadIdx.x; j < ncut; j += blockDim.x) {
+    const int t = list[j];
+    const int c0 = imt[4 * t], c1 = imt[4 * t + 1], c2 = imt[4 * t + 2], c3 = imt[4 * t + 3];
+    const double de2[4] = {e2[c0], e2[c1], e2[c2], e2[c3]};
+    const double de1[4] = {e1[c0], e1[c1], e1[c2], e1[c3]};
+    double2* y0 = x + (size_t)c0 * nf;
+    double2* y1 = x + (size_t)c1 * nf;
+    double2* y2 = x + (size_t)c2 * nf;
+    double2* y3 = x + (size_t)c3 * nf;
+    unsigned path = 0;
+    bool more = true;
+#pragma unroll 1
+    for (int leaf = 0; leaf < 64 && more; ++leaf) {
+      CutNode n;
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) n.r[v][c] = v == c ? 1.0 : 0.0;
+      n.vol = 1.0;
+      n.marks = 0;
+      int depth = 0;
+      bool keep = true;
+#pragma unroll
+      for (int plane = 0; plane < 2; ++plane) {
+        if (keep) {
+          const double(&dv)[4] = plane == 0 ? de2 : de1;
+          const double imod = plane == 0 ? 1.0 : -1.0;
+          n.marks = 0;
+          bool leafy = false;
+          double val[4];
+#pragma unroll 1
+          for (int step = 0; step < 4 && !leafy; ++step) {
+            node_values(n, dv, val);
+            int i1, i2;
+            if (find_cut(n, val, ef, i1, i2)) {
+              split_node(n, val, ef, i1, i2, (path >> depth) & 1u);
+              ++depth;
+            } else {
+              leafy = true;
+              keep = node_kept(n, val, ef, imod);
+            }
+          }
+          if (!leafy) {  // more than three cuts in one plane: impossible
+            atomicOr(status, 8);
+            keep = false;
+          }
+        }
+      }
+      if (keep) {
+        double s[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          s[v] = (de2[0] - de1[0]) * n.r[v][0] + (de2[1] - de1[1]) * n.r[v][1] +
+                 (de2[2] - de1[2]) * n.r[v][2] + (de2[3] - de1[3]) * n.r[v][3];
+        const zd g[4] = {mkz(-s[0], delt), mkz(-s[1], delt), mkz(-s[2], delt), mkz(-s[3], delt)};
+        const int code = classify(dmna, dmnr, g);
+        if ((code & 15) == kBad) {
+          atomicOr(status, 2);
+        } else {
+#pragma unroll 1
+          for (int f = 0; f < nfb; ++f) {
+            const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
+            zd T0 = mkz(0, 0), T1 = mkz(0, 0), T2 = mkz(0, 0), T3 = mkz(0, 0);
+#pragma unroll 1
+            for (int sg = 0; sg < 2; ++sg) {  // dsgn = +1, then -1: g -> -g, f -> -f
+              const double ds = sg == 0 ? 1.0 : -1.0;
+              zd t0, t1, t2, t3;
+              calc_terms(code, g[0] * ds, g[1] * ds, g[2] * ds, g[3] * ds, e, t0, t1, t2, t3);
+              T0 = T0 + t0 * ds;
+              T1 = T1 + t1 * ds;
+              T2 = T2 + t2 * ds;
+              T3 = T3 + t3 * ds;
+            }
+            T0 = T0 * n.vol;
+            T1 = T1 * n.vol;
+            T2 = T2 * n.vol;
+            T3 = T3 * n.vol;
+            // corner c: sum_v T_v dwgt(c, v)
+            lds_add(y0 + f, T0 * n.r[0][0] + T1 * n.r[1][0] + T2 * n.r[2][0] + T3 * n.r[3][0]);
+            lds_add(y1 + f, T0 * n.r[0][1] + T1 * n.r[1][1] + T2 * n.r[2][1] + T3 * n.r[3][1]);
+            lds_add(y2 + f, T0 * n.r[0][2] + T1 * n.r[1][2] + T2 * n.r[2][2] + T3 * n.r[3][2]);
+            lds_add(y3 + f, T0 * n.r[0][3] + T1 * n.r[1][3] + T2 * n.r[2][3] + T3 * n.r[3][3]);
+          }
+        }
+      }
+      // next leaf in depth-first order: flip the deepest 0 bit of the path
+      int b = depth - 1;
+      while (b >= 0 && ((path >> b) & 1u)) --b;
+      if (b < 0) more = false;
+      else path = (path & ((1u << b) - 1u)) | (1u << b);
+    }
+    if (more) atomicOr(status, 8);
+  }
+  __syncthreads();
+  const double inv_ntk = 1.0 / (double)ntk;
+  for (int i = threadIdx.x; i < nk3 * nfb; i += blockDim.x) {
+    const int gi = i / nfb, f = i % nfb;
+    const int k = index_kpt[gi];
+    const double2 po = pband[(size_t)jo * ntk + k], pv = pband[(size_t)iv * ntk + ikq[k]];
+    const zd fac = mkz(1.0, 0.0) - mkz(po.x, po.y) * mkz(pv.x, pv.y);
+    const double2 xv = x[(size_t)gi * nf + f];
+    const zd v = mkz(xv.x, xv.y) * inv_ntk * fac;
+    double2* o = dout + (size_t)(f0 + f) * ldd + (size_t)p * ntk + k;
+    *o = make_double2(o->x + v.x, o->y + v.y);
+  }
+}
+
+// Drude sums at q = 0 (chiqw.F:1631-1733), one intraband band per block:
+// p(k) = -i calc_VMaa(n, n, k) = sum_G (k+G) conj(C) C (Cartesian), the
+// ttrhdrn_simple weights x(k) for f = 1 and g = CMPLX(E_n(k) - E_F, -delt)
+// (single precision) at em = 0, then part[block][d] = sum_k Im x(k) p_d(k)^2.
+// coef/kg/ng0 as k_q0_heads; sk0 the full-mesh k (fractional).
+__global__ void __launch_bounds__(256) k_drude(
+    const double2* __restrict__ coef, const int* __restrict__ kg, const int* __restrict__ ng0,
+    int ntk, int ntg, const int* __restrict__ bands, const double* __restrict__ sk0,
+    const double* __restrict__ bvec, const double* __restrict__ eband, double ef, double delt,
+    double dmna, double dmnr, int n1, int n2, int n3, const int* __restrict__ imt,
+    const int* __restrict__ index_kpt, int* __restrict__ status, double2* __restrict__ part) {
+  extern __shared__ double2 lds[];
+  const int nk3 = n1 * n2 * n3;
+  double2* pk = lds;           // [ntk][3]
+  double2* xo = pk + 3 * ntk;  // [nk3]
+  __shared__ double red[6][4];
+  const int n = bands[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  for (int k = wave; k < ntk; k += nwave) {
+    const double2* c = coef + ((size_t)n * ntk + k) * ntg;
+    const int* kk = kg + (size_t)k * ntg * 3;
+    const double k0 = sk0[3 * k], k1 = sk0[3 * k + 1], k2 = sk0[3 * k + 2];
+    double s[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    for (int i = lane; i < ng0[k]; i += 64) {
+      const double2 a = c[i];
+      const double pr = a.x * a.x + a.y * a.y, pi = a.x * a.y - a.y * a.x;  // conj(C) C
+      const double u0 = k0 + (double)kk[3 * i], u1 = k1 + (double)kk[3 * i + 1],
+                   u2 = k2 + (double)kk[3 * i + 2];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double gv = u0 * bvec[d] + u1 * bvec[3 + d] + u2 * bvec[6 + d];
+        s[d][0] += gv * pr;
+        s[d][1] += gv * pi;
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        s[d][0] += __shfl_xor(s[d][0], off);
+        s[d][1] += __shfl_xor(s[d][1], off);
+      }
+    // vm = i S and p = -i vm = S
+    if (lane == 0) {
+      pk[3 * k] = make_double2(s[0][0], s[0][1]);
+      pk[3 * k + 1] = make_double2(s[1][0], s[1][1]);
+      pk[3 * k + 2] = make_double2(s[2][0], s[2][1]);
+    }
+  }
+  for (int i = threadIdx.x; i < nk3; i += blockDim.x) xo[i] = make_double2(0, 0);
+  __syncthreads();
+  const zd e0 = mkz(0.0, 0.0);
+  for (int t = threadIdx.x; t < 6 * nk3; t += blockDim.x) {
+    const int c0 = imt[4 * t], c1 = imt[4 * t + 1], c2 = imt[4 * t + 2], c3 = imt[4 * t + 3];
+    const double mdl = (double)(float)(-delt);
+    const zd g[4] = {mkz((double)(float)(eband[(size_t)n * ntk + index_kpt[c0]] - ef), mdl),
+                     mkz((double)(float)(eband[(size_t)n * ntk + index_kpt[c1]] - ef), mdl),
+                     mkz((double)(float)(eband[(size_t)n * ntk + index_kpt[c2]] - ef), mdl),
+                     mkz((double)(float)(eband[(size_t)n * ntk + index_kpt[c3]] - ef), mdl)};
+    const int code = classify(dmna, dmnr, g);
+    if ((code & 15) == kBad) {
+      atomicOr(status, 2);
+      continue;
+    }
+    zd t0, t1, t2, t3;
+    calc_terms(code, g[0], g[1], g[2], g[3], e0, t0, t1, t2, t3);
+    lds_add(xo + c0, t0);
+    lds_add(xo + c1, t1);
+    lds_add(xo + c2, t2);
+    lds_add(xo + c3, t3);
+  }
+  __syncthreads();
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (int gi = threadIdx.x; gi < nk3; gi += blockDim.x) {
+    const int k = index_kpt[gi];
+    const double im = xo[gi].y;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double2 q = pk[3 * k + d];
+      acc[2 * d] += im * (q.x * q.x - q.y * q.y);
+      acc[2 * d + 1] += im * (2.0 * q.x * q.y);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off);
+    if (lane == 0) red[c][wave] = acc[c];
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const int d = threadIdx.x;
+    double re = 0, im = 0;
+    for (int w = 0; w < nwave; ++w) {
+      re += red[2 * d][w];
+      im += red[2 * d + 1][w];
+    }
+    part[3 * blockIdx.x + d] = make_double2(re, im);
+  }
+}
+
+// wpl = 8 sum_bands part / (VOLUME NTK), wd = sqrt(Re wpl) (chiqw.F:1991-1993).
+__global__ void k_drude_finish(const double2* __restrict__ part, int nb, double volume, int ntk,
+                               double* __restrict__ wd) {
+  const int d = threadIdx.x;
+  if (d >= 3) return;
+  double re = 0.0;
+  for (int b = 0; b < nb; ++b) re += part[3 * b + d].x;
+  wd[d] = sqrt(8.0 * re / volume / (double)ntk);
+}
+
 size_t tetra_lds_bytes(int nk3, int nf) {
   return sizeof(double2) * (size_t)nk3 * nf + 2 * sizeof(double) * (size_t)nk3 +
          sizeof(int2) * 12 * (size_t)nk3;
@@ -409,7 +822,7 @@ int tetra_freqs_per_block(int nk3, int nen) {
 void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const int* index_kpt,
                           const int* pocc, const int* pvir, const int* ikq, const double* eband,
                           const double2* pband, double ef, double delt, double dmna, double dmnr,
-                          int nen, const double2* em, size_t ldd, int npair, int* status,
+                          int nen, const double2* em, size_t ldd, int npair, int cuts, int* status,
                           double2* dout, hipStream_t s) {
   const int nk3 = n1 * n2 * n3;
   const int nf = tetra_freqs_per_block(nk3, nen);
@@ -418,9 +831,35 @@ void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const
   if (lds > 65536)
     RP_HIP(hipFuncSetAttribute((const void*)k_tetra_weights,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  hipLaunchKernelGGL(k_tetra_weights, dim3(npair, (nen + nf - 1) / nf), dim3(256), lds, s, ntk,
-                     n1, n2, n3, imt, index_kpt, pocc, pvir, ikq, eband, pband, ef, delt, dmna,
-                     dmnr, nen, nf, em, ldd, status, dout);
+  const dim3 grid(npair, (nen + nf - 1) / nf);
+  hipLaunchKernelGGL(k_tetra_weights, grid, dim3(256), lds, s, ntk, n1, n2, n3, imt, index_kpt,
+                     pocc, pvir, ikq, eband, pband, ef, delt, dmna, dmnr, nen, nf, em, ldd,
+                     cuts ? 1 : 0, status, dout);
+  RP_HIP(hipGetLastError());
+  if (!cuts) return;
+  // x[nk3][nf], e1, e2, the cut list (<= lds of the uncut kernel)
+  const size_t lcut = sizeof(double2) * (size_t)nk3 * nf + 2 * sizeof(double) * (size_t)nk3 +
+                      sizeof(int) * 6 * (size_t)nk3;
+  if (lcut > 65536)
+    RP_HIP(hipFuncSetAttribute((const void*)k_tetra_cut,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lcut));
+  hipLaunchKernelGGL(k_tetra_cut, grid, dim3(256), lcut, s, ntk, n1, n2, n3, imt, index_kpt, pocc,
+                     pvir, ikq, eband, pband, ef, delt, dmna, dmnr, nen, nf, em, ldd, status, dout);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_drude(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
+                  const int* bands, int nbands, const double* sk0, const double* bvec,
+                  const double* eband, double ef, double delt, double dmna, double dmnr, int n1,
+                  int n2, int n3, const int* imt, const int* index_kpt, double volume,
+                  int* status, double2* part, double* wd, hipStream_t s) {
+  const int nk3 = n1 * n2 * n3;
+  const size_t lds = sizeof(double2) * (3 * (size_t)ntk + nk3);
+  RP_CHECK(lds <= 64 * 1024, "k mesh too large for the Drude kernel");
+  hipLaunchKernelGGL(k_drude, dim3(nbands), dim3(256), lds, s, coef, kg, ng0, ntk, ntg, bands, sk0,
+                     bvec, eband, ef, delt, dmna, dmnr, n1, n2, n3, imt, index_kpt, status, part);
+  RP_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_drude_finish, dim3(1), dim3(64), 0, s, part, nbands, volume, ntk, wd);
   RP_HIP(hipGetLastError());
 }
 

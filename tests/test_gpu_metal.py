@@ -1,0 +1,100 @@
+"""Metals on the GPU (partially occupied bands): tetrahedra cut by E_F
+(k_tetra_cut, ttct in ttrhdrn_chi0) and the q = 0 intraband Drude term
+(k_drude, chiqw.F:1631-1733), against the oracle on the same synthetic
+metals.  The weights of a pair whose bands cross E_F within FP64 closed-form
+rounding; at q != 0 chi and eps^-1 from the FP64 path within 1e-6 relative
+Frobenius, as for insulators; the split-BF16 path within 1e-5 of FP64.
+
+q = 0 of a metal: pairs of two partially occupied bands have small gaps, so
+their G = 0 entries -i<v|p|o>/(e_v - e_o) are large and dominate the chi head,
+and their weights come from small Fermi-surface pieces whose closed forms
+cancel at large omega.  There the reference itself carries few digits: on
+metal_tiny the compiled reference ttrhdrn_chi0 and the oracle disagree on
+single weights by up to 2.6e-3 above 5 eV
+(tests/test_oracle_vs_ref.py::test_fermi_surface_weights_on_a_metal), and on
+the sum over dsgn = +-1 of the pair of the two crossing bands by 1e-6 at
+9 eV, 2e-2 at 49 eV and 0.17 at 98 eV (the GPU is closer to the oracle).  The bars at q = 0 are therefore
+eps^-1 within 1e-6 per frequency below 20 eV, within 1e-3 over all
+frequencies, and chi within 1e-5; eps^-1 carries the Drude head.
+"""
+import numpy as np
+import pytest
+
+from respackdev_amd import gen
+from respackdev_amd.chiqw import Chiqw, from_system
+
+pytestmark = pytest.mark.gpu
+
+
+def relf(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _system(name, rough=False):
+    wl = gen.WORKLOADS[name]
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed",
+                                             "n_cross")}, rough=rough)
+    return wl, from_system(s)
+
+
+@pytest.mark.parametrize("rough", [False, True])
+def test_cut_weights_match_oracle(rough):
+    from oracle import pipeline
+    wl, w = _system("metal_tiny", rough)
+    g = Chiqw(0)
+    try:
+        info = g.open(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        assert info["n_occ"] > info["n_docc"]  # partially occupied bands present
+        g.prepare_q(wl["q"])
+        o = pipeline.run_q(w, wl["q"], ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        ntk = info["ntk"]
+        # pair 0 runs from the lowest band into the first partially occupied one: its
+        # empty-band energies cross E_F, so its tetrahedra are cut
+        assert o["pvir"][0] == info["n_docc"]
+        d = g.weights()
+        want = o["x0"].T / ntk
+        scale = np.abs(want).max()
+        low = info["em"].real < 30 / 27.21151
+        err = np.abs(d[:, :ntk] - want)
+        print(f"rough={rough}: weights max err {err.max() / scale:.2e} "
+              f"(low band {err[low].max() / scale:.2e})", flush=True)
+        np.testing.assert_allclose(d[low, :ntk], want[low], rtol=0, atol=1e-8 * scale)
+        np.testing.assert_allclose(d[:, :ntk], want, rtol=0, atol=1e-6 * scale)
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("name,q,rough", [("metal_tiny", None, False),
+                                          ("metal_tiny", None, True),
+                                          ("metal_tiny", (0.0, 0.0, 0.0), False),
+                                          ("metal_c1", None, False),
+                                          ("metal_c1", (0.0, 0.0, 0.0), False)])
+def test_metal_chi_and_epsinv(name, q, rough):
+    from oracle import pipeline
+    wl, w = _system(name, rough)
+    q = wl["q"] if q is None else q
+    g = Chiqw(0)
+    try:
+        g.open(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        r = g.run_q(q, "fp64", want_chi=True)
+        o = pipeline.run_q(w, q, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        ec, ee = relf(r["chi"], o["chi"]), relf(r["epsinv"], o["epsinv"])
+        print(f"{name} q={q} rough={rough}: chi {ec:.3e} epsinv {ee:.3e} "
+              f"(intraband bands {len(o['intra'])}, wd {np.round(o['wd'], 5)})", flush=True)
+        assert r["chi"].shape == o["chi"].shape
+        if not np.all(np.asarray(q) == 0.0):
+            assert ec <= 1e-6 and ee <= 1e-6
+        else:
+            assert len(o["intra"]) > 0 and np.all(o["wd"] > 0)
+            low = o["em"].real < 20.0 / 27.21151
+            per = [relf(r["epsinv"][:, ie], o["epsinv"][:, ie]) for ie in range(len(o["em"]))]
+            print("  eps^-1 per frequency:", np.array2string(np.array(per), precision=1), flush=True)
+            assert max(np.array(per)[low]) <= 1e-6
+            assert ec <= 1e-5 and ee <= 1e-3
+        if name == "metal_c1":
+            m = g.run_q(q, "bf16x3", want_chi=True)
+            em, eme = relf(m["chi"], r["chi"]), relf(m["epsinv"], r["epsinv"])
+            print(f"  bf16x3 vs fp64: chi {em:.3e} epsinv {eme:.3e}", flush=True)
+            assert em <= 1e-5 and eme <= 1e-5
+    finally:
+        g.close()
