@@ -279,6 +279,26 @@ def write_dir_wan(wan: list, centres: np.ndarray, root: str) -> str:
     return d
 
 
+def write_dir_wan_crpa(ns, nb, umat, root: str) -> str:
+    """dir-wan/dat.ns-nb (per full-mesh k: bands below the window, bands in
+    it) and dat.umat (NWF, then per k and window band the NWF complex
+    <psi_b|w> as Fortran list-directed (re,im) pairs), as the reference's
+    wannier program writes them (wannier.F:567-572, 2193-2217) and
+    calc_chiqw reads them (chiqw.F:870-895).  umat: (NTK, NWF, Mb)."""
+    d = os.path.join(root, "dir-wan")
+    os.makedirs(d, exist_ok=True)
+    umat = np.asarray(umat, np.complex128)
+    with open(os.path.join(d, "dat.ns-nb"), "w") as f:
+        for a, b in zip(ns, nb):
+            f.write(f" {int(a):8d} {int(b):8d}\n")
+    with open(os.path.join(d, "dat.umat"), "w") as f:
+        f.write(f" {umat.shape[1]:8d}\n")
+        for k in range(umat.shape[0]):
+            for ib in range(int(nb[k])):
+                f.write(" " + " ".join(f"({v.real:.17E},{v.imag:.17E})" for v in umat[k, :, ib]) + "\n")
+    return d
+
+
 def _fortran_record(fh, payload: bytes) -> None:
     n = struct.pack("<i", len(payload))
     fh.write(n)

@@ -19,14 +19,16 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "respackdev_amd", "bin", "calc_chiqw")
 
 
-def _case(tmp_path, calc_k, prec=0, name="tiny"):
+def _case(tmp_path, calc_k, prec=0, name="tiny", crpa=0):
     wl = gen.WORKLOADS[name]
-    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    keys = ["a", "nk", "nocc", "nvir", "ecut_psi", "seed"] + (["n_cross"] if "n_cross" in wl else [])
+    s = gen.make_system(**{k: wl[k] for k in keys})
     gen.write_dir_wfn(s, str(tmp_path))
     nml = ("&param_chiqw\n"
            f" Ecut_for_eps = {wl['ecut_eps']!r}\n"
            f" Num_freq_grid = {wl['nen']}\n"
            " flg_calc_type = 2\n"
+           f" flg_cRPA = {crpa}\n"
            f" n_calc_q = {len(calc_k)}\n"
            f" Contraction_precision = {prec}\n"
            "/\n" + " ".join(str(k) for k in calc_k) + "\n")
@@ -120,3 +122,74 @@ def test_host_dir_eps_matches_oracle(tmp_path):
     r2 = _run(tmp_path, nml)
     assert r2.returncode == 0
     assert "dat.log.400 exists" in r2.stdout
+
+
+def _eps_files(eps_dir, iq, q0, ng, nen):
+    d = eps_dir / f"q{iq:03d}"
+    if q0:
+        return np.stack([_epsqw(d / f"dat.epsqw.{600 + ix}", ng, nen) for ix in range(3)])
+    return _epsqw(d / "dat.epsqw.600", ng, nen)[None]
+
+
+@pytest.mark.gpu
+def test_host_crpa_reads_dir_wan(tmp_path):
+    """flg_cRPA = 1: the host reads dir-wan/dat.ns-nb and dat.umat (written in
+    the reference's wannier format) and gives the oracle's cRPA eps^-1 for
+    P(n,k) = sum_w |U|^2 (chiqw.F:870-905), at q != 0 and q = 0."""
+    from oracle import pipeline
+    wl0, w, _ = _case(tmp_path, [1], name="w3d_tiny")
+    iq = _kindex(w, wl0["q"])
+    ig = _kindex(w, (0.0, 0.0, 0.0))
+    wl, w, nml = _case(tmp_path, [iq, ig], name="w3d_tiny", crpa=1)
+    st = pipeline.setup(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+    ntk, nbands, nocc = st["ntk"], st["coef"].shape[0], st["n_occ"]
+    rng = np.random.default_rng(8)
+    ns = (nocc - 1 - rng.integers(0, 2, ntk)).astype(np.int32)
+    nb = np.minimum(rng.integers(2, 4, ntk), nbands - ns).astype(np.int32)
+    nwf = 2
+    u = (rng.standard_normal((ntk, nwf, int(nb.max()))) +
+         1j * rng.standard_normal((ntk, nwf, int(nb.max())))) * 0.5
+    prob = np.zeros((nbands, ntk), np.complex128)
+    for k in range(ntk):
+        for b in range(nb[k]):
+            prob[ns[k] + b, k] = np.sum(np.abs(u[k, :, b]) ** 2)
+    gen.write_dir_wan_crpa(ns, nb, u, str(tmp_path))
+    r = _run(tmp_path, nml)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "cRPA" in r.stdout
+    for qq, idx in ((wl["q"], iq), ((0.0, 0.0, 0.0), ig)):
+        o = pipeline.run_q(w, qq, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"], prob=prob)
+        plain = pipeline.run_q(w, qq, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        ng = o["lg"].shape[0]
+        eps = _eps_files(tmp_path / "dir-eps", idx, np.all(np.asarray(qq) == 0), ng, wl["nen"])
+        err = np.linalg.norm(eps - o["epsinv"]) / np.linalg.norm(o["epsinv"])
+        moved = np.linalg.norm(plain["epsinv"] - o["epsinv"]) / np.linalg.norm(o["epsinv"])
+        print(f"q={qq} cRPA eps^-1 rel err {err:.3e} (cRPA moves it by {moved:.3f})", flush=True)
+        assert moved > 1e-3 and err <= 1e-6
+
+
+@pytest.mark.gpu
+def test_host_metal(tmp_path):
+    """A metal through the host: partially occupied bands, Fermi-surface cuts
+    at q != 0, the Drude head at q = 0 (bars as in tests/test_gpu_metal.py)."""
+    from oracle import pipeline
+    wl0, w, _ = _case(tmp_path, [1], name="metal_tiny")
+    iq = _kindex(w, wl0["q"])
+    ig = _kindex(w, (0.0, 0.0, 0.0))
+    wl, w, nml = _case(tmp_path, [iq, ig], name="metal_tiny")
+    r = _run(tmp_path, nml)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    o = pipeline.run_q(w, wl["q"], ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+    eps = _eps_files(tmp_path / "dir-eps", iq, False, o["lg"].shape[0], wl["nen"])
+    err = np.linalg.norm(eps - o["epsinv"]) / np.linalg.norm(o["epsinv"])
+    print(f"metal q={wl['q']} eps^-1 rel err {err:.3e}", flush=True)
+    assert err <= 1e-6
+    o0 = pipeline.run_q(w, (0.0, 0.0, 0.0), ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+    e0 = _eps_files(tmp_path / "dir-eps", ig, True, o0["lg"].shape[0], wl["nen"])
+    low = o0["em"].real < 20.0 / 27.21151
+    per = [np.linalg.norm(e0[:, ie] - o0["epsinv"][:, ie]) / np.linalg.norm(o0["epsinv"][:, ie])
+           for ie in range(wl["nen"])]
+    print(f"metal q=0 eps^-1 per frequency {np.array2string(np.array(per), precision=1)}", flush=True)
+    assert max(np.array(per)[low]) <= 1e-6 and max(per) <= 1e-3
