@@ -41,6 +41,37 @@ def test_eps_rpa_formula_and_inverse():
     np.testing.assert_allclose(out, np.linalg.inv(eps), rtol=1e-11, atol=1e-13)
 
 
+def test_eps_rpa_q0_is_the_small_q_limit():
+    """The q = 0 head/wing assembly (ro_eps_rpa_q0, chiqw_sub.F:459-568) is
+    the q -> 0 limit of the q != 0 one once the G = 0 matrix elements carry
+    their 1/|q| (the head of chi scaled by |q|^2, the wings by |q|): at a tiny
+    q along any direction both give the same eps^-1 (wd = 0).  A wing with two
+    1/|q+G| factors, or a head with one, fails by orders of magnitude."""
+    rng = np.random.default_rng(3)
+    lg = oracle.make_kg0(B7, 3.0, (0, 0, 0))
+    n = len(lg)
+    g0 = int(np.where((lg == 0).all(1))[0][0])
+    chi = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 1e-3
+    chi[g0, g0] = 0.02 + 0.003j
+    b = [np.ascontiguousarray(B7[i]) for i in range(3)]
+    out0 = np.zeros((n, n), np.complex128, order="F")
+    chif = np.asfortranarray(chi)
+    assert oracle.lib().ro_eps_rpa_q0(n, ptr(lg), ptr(b[0]), ptr(b[1]), ptr(b[2]), g0, 0.0,
+                                      0.1 / 27.21151, 0.3, chif.ctypes.data,
+                                      out0.ctypes.data) == 0
+    for d in (np.array([0.0, 0.0, 1.0]), np.array([0.3, -0.5, 0.8])):
+        q = 1e-7 * d / np.linalg.norm(d @ B7)  # |q| = 1e-7 bohr^-1
+        qa = np.linalg.norm(q @ B7)
+        c = chi.copy()
+        c[g0, :] *= qa
+        c[:, g0] *= qa
+        out = np.zeros((n, n), np.complex128, order="F")
+        assert oracle.lib().ro_eps_rpa(n, ptr(lg), ptr(np.ascontiguousarray(q)), ptr(b[0]),
+                                       ptr(b[1]), ptr(b[2]), np.asfortranarray(c).ctypes.data,
+                                       out.ctypes.data) == 0
+        np.testing.assert_allclose(out, out0, rtol=0, atol=1e-6 * np.abs(out0).max())
+
+
 def test_flat_band_tetrahedron_closed_form():
     # all four vertices degenerate -> x(k,w) = 1/(w - D + i d) - 1/(w + D - i d)
     nkb = (4, 4, 4)
