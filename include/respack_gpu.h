@@ -140,6 +140,38 @@ int rp_chiqw_npair(rp_handle h, int32_t* npair);
  * N_CALC_BAND, chiqw.F:1267-1295); 0 restores the value of rp_chiqw_open. */
 int rp_chiqw_set_window(rp_handle h, double ecut_for_eps, int32_t n_vir);
 
+/* Batched configuration, captured (no counterpart in the reference, which runs
+ * one q per iteration of its q loop, chiqw.F:662-687; for callers that run
+ * many q of one system).  rp_chiqw_batch_plan prepares all nq q points
+ * (qs[3*nq], fractional) with per-q ecut_for_eps[nq] and n_vir[nq] windows as
+ * rp_chiqw_set_window (either may be NULL: the values of rp_chiqw_open), runs
+ * every stage of every q once with its status checked (tuning the contraction
+ * per shape when rp_chiqw_autotune is on), and captures the launch sequence of
+ * all q and all frequencies into one HIP graph.  The batch owns its buffers;
+ * the handle's own q state and buffers are left as they were.  A new plan
+ * replaces the old one. */
+int rp_chiqw_batch_plan(rp_handle h, int32_t nq, const double* qs, const double* ecut_for_eps,
+                        const int32_t* n_vir, int32_t precision);
+/* Replays the graph (all q recomputed from the resident wavefunctions);
+ * seconds: the device time of the replay. */
+int rp_chiqw_batch_run(rp_handle h, double* seconds);
+/* Entry iq of the last replay: NG, number of directions (3 at q = 0, else 1),
+ * the chi G list (lg[3*NG], capacity lg_cap triplets; may be NULL) and
+ * eps^-1 as rp_dyson's output (may be NULL). */
+int rp_chiqw_batch_result(rp_handle h, int32_t iq, int32_t* ng, int32_t* ndir, int32_t* lg,
+                          int32_t lg_cap, double* epsinv_out);
+int rp_chiqw_batch_free(rp_handle h);
+/* Autotuner of the contraction (no counterpart in the reference): when on,
+ * the first contraction of each shape (precision, NG, nen, padded K,
+ * directions) times the launch candidates around the built-in heuristic on
+ * that q's data and keeps the fastest for every later q of the shape. */
+int rp_chiqw_autotune(rp_handle h, int32_t enable);
+int rp_chiqw_tune_count(rp_handle h, int32_t* n);
+/* Entry idx: iout[10] = precision, NG, nen, directions, padded K, candidates,
+ * chosen K split, chosen workgroups, heuristic K split, heuristic workgroups;
+ * dout[2] = chosen and heuristic time (ms, one contraction). */
+int rp_chiqw_tune_info(rp_handle h, int32_t idx, int32_t* iout, double* dout);
+
 /* ------------------------------------------------------------------------
  * calc_int drop-in, calc_w3d (replaces calc_intW.F:8-985): Wannier-basis bare
  * and screened Coulomb integrals V_ij(R) and W_ij(R, w).

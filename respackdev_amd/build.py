@@ -26,7 +26,7 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
             "-Wno-unused-function", "-Wno-unused-variable",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["rp_setup.cpp", "rp_capi.cpp", "rp_w3d.cpp", "rp_pairdens.hip", "rp_tetra.hip",
+SOURCES = ["rp_setup.cpp", "rp_capi.cpp", "rp_w3d.cpp", "rp_batch.cpp", "rp_pairdens.hip", "rp_tetra.hip",
            "rp_chi_gemm.hip", "rp_chi_bf16.hip", "rp_dyson.hip", "rp_w3d.hip",
            "rp_mlp.cpp", "rp_mlp.hip"]
 

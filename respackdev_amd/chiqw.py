@@ -239,6 +239,55 @@ class Chiqw:
         check(_lib.lib().rp_dyson(self._h, ptr(out)))
         return out
 
+    def autotune(self, enable: bool = True) -> None:
+        check(_lib.lib().rp_chiqw_autotune(self._h, 1 if enable else 0))
+
+    def tune_table(self) -> list[dict]:
+        n = np.zeros(1, np.int32)
+        check(_lib.lib().rp_chiqw_tune_count(self._h, ptr(n)))
+        out = []
+        for i in range(int(n[0])):
+            iv = np.zeros(10, np.int32)
+            dv = np.zeros(2)
+            check(_lib.lib().rp_chiqw_tune_info(self._h, i, ptr(iv), ptr(dv)))
+            out.append(dict(precision=int(iv[0]), ng=int(iv[1]), nen=int(iv[2]), ndir=int(iv[3]),
+                            kpad=int(iv[4]), candidates=int(iv[5]), nsplit=int(iv[6]),
+                            nwg=int(iv[7]), heuristic_nsplit=int(iv[8]),
+                            heuristic_nwg=int(iv[9]), ms=float(dv[0]),
+                            heuristic_ms=float(dv[1])))
+        return out
+
+    def batch_plan(self, qs, ecut_for_eps=None, n_vir=None, precision: str = "bf16x3") -> None:
+        """Prepares every q (per-q windows optional), checks each stage once
+        and captures the launch sequence of all q into one HIP graph."""
+        qq = np.ascontiguousarray(qs, np.float64).reshape(-1, 3)
+        ec = None if ecut_for_eps is None else np.ascontiguousarray(ecut_for_eps, np.float64)
+        nv = None if n_vir is None else np.ascontiguousarray(n_vir, np.int32)
+        self._batch_n = len(qq)
+        check(_lib.lib().rp_chiqw_batch_plan(self._h, len(qq), ptr(qq),
+                                             None if ec is None else ptr(ec),
+                                             None if nv is None else ptr(nv),
+                                             _lib.PREC[precision]))
+
+    def batch_run(self) -> float:
+        t = np.zeros(1)
+        check(_lib.lib().rp_chiqw_batch_run(self._h, ptr(t)))
+        return float(t[0])
+
+    def batch_result(self, iq: int, want_eps: bool = True):
+        ng = np.zeros(1, np.int32)
+        nd = np.zeros(1, np.int32)
+        check(_lib.lib().rp_chiqw_batch_result(self._h, iq, ptr(ng), ptr(nd), None, 0, None))
+        ng, nd = int(ng[0]), int(nd[0])
+        lg = np.zeros((ng, 3), np.int32)
+        eps = np.zeros((nd, self.info["nen"], ng, ng), np.complex128) if want_eps else None
+        check(_lib.lib().rp_chiqw_batch_result(self._h, iq, None, None, ptr(lg), ng,
+                                               None if eps is None else ptr(eps)))
+        return dict(lg=lg, epsinv=eps)
+
+    def batch_free(self) -> None:
+        check(_lib.lib().rp_chiqw_batch_free(self._h))
+
     def set_window(self, ecut_for_eps: float = 0.0, n_vir: int = 0):
         """Per-q chi cutoff (Ry) and empty-band window for the q points prepared
         next (0: the values of open)."""

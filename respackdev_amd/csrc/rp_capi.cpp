@@ -506,7 +506,11 @@ int rp_chiqw_npair(rp_handle h, int32_t* npair) {
   });
 }
 
-static void check_status(rp_handle h, const char* what) {
+}  // extern "C"
+
+namespace rp {
+
+void check_status(rp_ctx* h, const char* what) {
   int st = 0;
   RP_HIP(hipMemcpyAsync(&st, h->d_status.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   RP_HIP(hipStreamSynchronize(h->stream));
@@ -516,28 +520,212 @@ static void check_status(rp_handle h, const char* what) {
   if (st & 4) throw Error(std::string(what) + ": singular dielectric matrix (zero pivot)");
 }
 
+// ------------------------------------------------------------ stage bodies
+// Enqueue only, on h->stream: no host synchronisation, no status read-back and
+// no host copies, so that the same code runs eagerly and under graph capture.
+// While capturing, every buffer must already be large enough (a reallocation
+// would free memory the captured nodes use) and the status word is cleared
+// once for the whole graph instead of per stage.
+template <class T>
+static void grow(const rp_ctx* h, DevBuf<T>& b, size_t m, const char* what) {
+  if (h->capturing) {
+    RP_CHECK(m == 0 || (b.p && b.n >= m), std::string("captured launch needs a larger ") + what);
+    return;
+  }
+  b.reserve(m);
+}
+
+void issue_pairdens(rp_ctx* h) {
+  hipStream_t s = h->stream;
+  const size_t K = (size_t)h->npair * h->ntk;
+  grow(h, h->d_A, h->kpad * h->lda, "pair-density buffer");
+  if (h->kpad > K) launch_zero(h->d_A.p + K * h->lda, (h->kpad - K) * h->lda * sizeof(float2), s);
+  launch_pair_densities(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
+                        h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_tw.p,
+                        h->d_A.p, s);
+  if (h->q0) {
+    grow(h, h->d_A3, 3 * h->kpad * h->lda, "q = 0 pair-density buffer");
+    for (int ix = 0; ix < 3; ++ix)
+      if (h->kpad > K)
+        launch_zero(h->d_A3.p + (ix * h->kpad + K) * h->lda, (h->kpad - K) * h->lda * sizeof(float2), s);
+    // three copies laid out [ix][kpad][lda]
+    launch_q0_heads(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_pocc.p, h->d_pvir.p,
+                    h->d_eband.p, h->d_bvec.p, h->delt, h->g0, h->ng, h->lda, h->kpad, (int)K,
+                    h->d_A.p, h->d_A3.p, s);
+  }
+}
+
+void issue_weights(rp_ctx* h) {
+  hipStream_t s = h->stream;
+  grow(h, h->d_w, (size_t)h->nen * h->kpad, "weight buffer");
+  launch_zero(h->d_w.p, sizeof(double2) * h->nen * h->kpad, s);
+  if (!h->capturing) launch_zero(h->d_status.p, sizeof(int), s);
+  launch_tetra_weights(h->ntk, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p, h->d_pocc.p,
+                       h->d_pvir.p, h->d_ikq.p, h->d_eband.p, h->d_pband.p, h->ef, h->delt, h->dmna,
+                       h->dmnr, h->nen, h->d_em.p, h->kpad, h->npair, h->n_pocc > 0 ? 1 : 0,
+                       h->d_status.p, h->d_w.p, s);
+}
+
+ChiLaunch chi_launch_heuristic(const rp_ctx* h, int prec) {
+  ChiLaunch c;
+  if (prec == RP_PREC_BF16X3) {
+    // 128 x 128 tiles, one persistent workgroup per CU; K split so that the
+    // items fill every workgroup evenly
+    c.nwg = bf16x3_workgroups(h->ncu);
+    c.nsplit = bf16x3_nsplit(h->ng, h->nen, h->kpad, c.nwg);
+  } else {
+    const int ntile = (h->ng + chi_tile() - 1) / chi_tile();
+    const size_t kchunks = h->kpad / chi_kchunk();
+    const long wgs = (long)ntile * ntile * h->nen;
+    c.nsplit = (int)std::min<long>((long)kchunks, std::max<long>(1, (2048 + wgs - 1) / wgs));
+  }
+  return c;
+}
+
+void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events) {
+  hipStream_t s = h->stream;
+  const int ng = h->ng, nen = h->nen, nc = h->q0 ? 3 : 1;
+  const size_t per = (size_t)ng * ng * nen;
+  grow(h, h->d_chi, (size_t)nc * per, "chi buffer");
+  grow(h, h->d_part, (size_t)c.nsplit * per, "partial-sum buffer");
+  if (prec == RP_PREC_BF16X3) {
+    grow(h, h->d_Ao, h->kpad * h->lda * 8, "split operand buffer");
+    grow(h, h->d_wf, (size_t)nen * h->kpad, "single-precision weight buffer");
+    grow(h, h->d_sync, bf16x3_sync_words(), "rendezvous counters");
+    launch_weights_to_f32(h->d_w.p, (size_t)nen * h->kpad, h->d_wf.p, s);
+  }
+  for (int ix = 0; ix < nc; ++ix) {
+    const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
+    if (prec == RP_PREC_BF16X3) launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
+    if (events) RP_HIP(hipEventRecord(h->gev[2 * ix], s));
+    if (prec == RP_PREC_BF16X3)
+      launch_chi_bf16x3(h->d_Ao.p, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, c.nsplit, c.nwg,
+                        h->d_part.p, h->d_sync.p, s);
+    else
+      launch_chi_gemm(prec, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, c.nsplit, h->d_part.p, s);
+    if (events) RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
+    launch_chi_reduce(h->d_part.p, per, c.nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
+  }
+}
+
+void issue_dyson(rp_ctx* h) {
+  hipStream_t s = h->stream;
+  const int n = h->ng, nb = h->nbatch();
+  const size_t per = (size_t)n * n;
+  grow(h, h->d_eps, (size_t)nb * per, "eps buffer");
+  grow(h, h->d_gjwork, gauss_jordan_work_ints(nb), "pivot work buffer");
+  grow(h, h->d_ipiv, (size_t)nb * n, "pivot buffer");
+  grow(h, h->d_wd, 3, "plasma-frequency buffer");
+  if (!h->capturing) launch_zero(h->d_status.p, sizeof(int), s);
+  if (h->q0 && !h->intra.empty()) {
+    // Drude plasma frequency of the intraband bands (fRPA metals at q = 0)
+    grow(h, h->d_drude, 3 * h->intra.size(), "Drude buffer");
+    launch_drude(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_intra.p,
+                 (int)h->intra.size(), h->d_sk0.p, h->d_bvec.p, h->d_eband.p, h->ef, h->delt,
+                 h->dmna, h->dmnr, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p,
+                 h->volume, h->d_status.p, h->d_drude.p, h->d_wd.p, s);
+  } else {
+    launch_zero(h->d_wd.p, 3 * sizeof(double), s);  // no intraband pairs (cRPA, or an insulator)
+  }
+  launch_eps_build(h->d_chi.p, h->d_qn.p, n, nb, h->nen, h->q0 ? 1 : 0, h->g0, h->d_wd.p, h->d_em.p,
+                   h->delt, h->d_eps.p, s);
+  launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_gjwork.p, h->d_status.p, s);
+}
+
+// ---------------------------------------------------------------- autotune
+// Candidates around the heuristic: for the split-BF16 kernel the K splits
+// whose work rounds per XCD slot (the heuristic's model) are within 15% of the
+// best, plus half and double the heuristic's; for the FP64/FP32 kernel the
+// splits that give 1024, 2048 and 4096 workgroups.  Each is timed on the
+// current q's data (one warm-up, then the best of two), and the fastest is
+// kept for every later q of the same shape.
+static std::vector<ChiLaunch> chi_candidates(const rp_ctx* h, int prec) {
+  std::vector<ChiLaunch> out;
+  const ChiLaunch base = chi_launch_heuristic(h, prec);
+  auto add = [&](ChiLaunch c) {
+    for (const auto& o : out)
+      if (o.nsplit == c.nsplit && o.nwg == c.nwg) return;
+    out.push_back(c);
+  };
+  add(base);
+  if (prec == RP_PREC_BF16X3) {
+    const size_t nchunks = h->kpad / 32;
+    const int ntile = (h->ng + 127) / 128;
+    const long mypairs = ((long)ntile * ntile * h->nen + 7) / 8;
+    const long nslot = std::max(1, base.nwg / 8);
+    std::vector<std::pair<double, int>> r;
+    for (int sp = 1; sp <= 64 && (size_t)sp <= nchunks; ++sp)
+      r.push_back({(double)((mypairs * sp + nslot - 1) / nslot) / sp, sp});
+    std::sort(r.begin(), r.end());
+    for (const auto& e : r)
+      if (e.first <= 1.15 * r[0].first && out.size() < 4) add({e.second, base.nwg});
+    if (base.nsplit > 1) add({base.nsplit / 2, base.nwg});
+    if ((size_t)base.nsplit * 2 <= nchunks && base.nsplit * 2 <= 64) add({base.nsplit * 2, base.nwg});
+  } else {
+    const int ntile = (h->ng + chi_tile() - 1) / chi_tile();
+    const size_t kchunks = h->kpad / chi_kchunk();
+    const long wgs = (long)ntile * ntile * h->nen;
+    for (long target : {1024L, 2048L, 4096L})
+      add({(int)std::min<long>((long)kchunks, std::max<long>(1, (target + wgs - 1) / wgs)), 0});
+  }
+  return out;
+}
+
+static double time_launch(rp_ctx* h, int prec, const ChiLaunch& c) {
+  issue_accumulate(h, prec, c, false);  // warm-up (and buffer growth)
+  double best = 1e300;
+  for (int rep = 0; rep < 2; ++rep) {
+    RP_HIP(hipEventRecord(h->ev[4], h->stream));
+    issue_accumulate(h, prec, c, false);
+    RP_HIP(hipEventRecord(h->ev[5], h->stream));
+    RP_HIP(hipEventSynchronize(h->ev[5]));
+    float ms = 0;
+    RP_HIP(hipEventElapsedTime(&ms, h->ev[4], h->ev[5]));
+    best = std::min(best, (double)ms);
+  }
+  return best;
+}
+
+ChiLaunch chi_launch(rp_ctx* h, int prec) {
+  const int ndir = h->q0 ? 3 : 1;
+  for (const auto& t : h->tune)
+    if (t.prec == prec && t.ng == h->ng && t.nen == h->nen && t.kpad == h->kpad && t.ndir == ndir)
+      return t.best;
+  const ChiLaunch base = chi_launch_heuristic(h, prec);
+  if (!h->autotune || h->capturing || !(h->have_A && h->have_w)) return base;
+  TuneEntry t;
+  t.prec = prec;
+  t.ng = h->ng;
+  t.nen = h->nen;
+  t.kpad = h->kpad;
+  t.ndir = ndir;
+  t.heuristic = base;
+  const auto cand = chi_candidates(h, prec);
+  t.ncand = (int)cand.size();
+  t.best_ms = 1e300;
+  for (const auto& c : cand) {
+    const double ms = time_launch(h, prec, c);
+    if (c.nsplit == base.nsplit && c.nwg == base.nwg) t.heuristic_ms = ms;
+    if (ms < t.best_ms) {
+      t.best_ms = ms;
+      t.best = c;
+    }
+  }
+  h->tune.push_back(t);
+  return t.best;
+}
+
+}  // namespace rp
+
+extern "C" {
+
 int rp_pairdens_batch(rp_handle h, float* m_out) {
   return guarded([&] {
     RP_CHECK(h && h->qready, "rp_pairdens_batch: no q prepared");
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const size_t K = (size_t)h->npair * h->ntk;
-    h->d_A.reserve(h->kpad * h->lda);
-    if (h->kpad > K)
-      RP_HIP(hipMemsetAsync(h->d_A.p + K * h->lda, 0, (h->kpad - K) * h->lda * sizeof(float2), s));
-    launch_pair_densities(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
-                          h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_tw.p, h->d_A.p, s);
-    if (h->q0) {
-      h->d_A3.reserve(3 * h->kpad * h->lda);
-      for (int ix = 0; ix < 3; ++ix)
-        if (h->kpad > K)
-          RP_HIP(hipMemsetAsync(h->d_A3.p + (ix * h->kpad + K) * h->lda, 0,
-                                (h->kpad - K) * h->lda * sizeof(float2), s));
-      // three copies laid out [ix][kpad][lda]
-      launch_q0_heads(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_pocc.p, h->d_pvir.p,
-                      h->d_eband.p, h->d_bvec.p, h->delt, h->g0, h->ng, h->lda, h->kpad, (int)K,
-                      h->d_A.p, h->d_A3.p, s);
-    }
+    issue_pairdens(h);
     if (m_out) {
       const int nc = h->q0 ? 3 : 1;
       for (int ix = 0; ix < nc; ++ix) {
@@ -557,13 +745,7 @@ int rp_tetra_weights_batch(rp_handle h, double* d_out) {
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const size_t K = (size_t)h->npair * h->ntk;
-    h->d_w.reserve((size_t)h->nen * h->kpad);
-    RP_HIP(hipMemsetAsync(h->d_w.p, 0, sizeof(double2) * h->nen * h->kpad, s));
-    RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
-    launch_tetra_weights(h->ntk, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p, h->d_pocc.p,
-                         h->d_pvir.p, h->d_ikq.p, h->d_eband.p, h->d_pband.p, h->ef, h->delt, h->dmna,
-                         h->dmnr, h->nen, h->d_em.p, h->kpad, h->npair, h->n_pocc > 0 ? 1 : 0,
-                         h->d_status.p, h->d_w.p, s);
+    issue_weights(h);
     check_status(h, "tetrahedron weights");
     if (d_out)
       RP_HIP(hipMemcpy2DAsync(d_out, sizeof(double2) * K, h->d_w.p, sizeof(double2) * h->kpad,
@@ -579,44 +761,12 @@ int rp_chi_accumulate(rp_handle h, int32_t precision, double* chi_out) {
     RP_CHECK(precision >= RP_PREC_FP64 && precision <= RP_PREC_BF16X3, "unknown precision");
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
-    const int ng = h->ng, nen = h->nen, nc = h->q0 ? 3 : 1;
-    const size_t per = (size_t)ng * ng * nen;
-    h->d_chi.reserve((size_t)nc * per);
-    if (precision == RP_PREC_BF16X3) {
-      // 128 x 128 tiles, one persistent workgroup per CU; K split so that the
-      // items fill every workgroup evenly
-      const int nwg = bf16x3_workgroups(h->ncu);
-      const int nsplit = bf16x3_nsplit(ng, nen, h->kpad, nwg);
-      h->d_part.reserve((size_t)nsplit * per);
-      h->d_Ao.reserve(h->kpad * h->lda * 8);
-      h->d_wf.reserve((size_t)nen * h->kpad);
-      h->d_sync.reserve(bf16x3_sync_words());
-      launch_weights_to_f32(h->d_w.p, (size_t)nen * h->kpad, h->d_wf.p, s);
-      for (int ix = 0; ix < nc; ++ix) {
-        const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
-        launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
-        RP_HIP(hipEventRecord(h->gev[2 * ix], s));
-        launch_chi_bf16x3(h->d_Ao.p, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, nsplit, nwg,
-                          h->d_part.p, h->d_sync.p, s);
-        RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
-        launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
-      }
-    } else {
-      const int ntile = (ng + chi_tile() - 1) / chi_tile();
-      const size_t kchunks = h->kpad / chi_kchunk();
-      long wgs = (long)ntile * ntile * nen;
-      int nsplit = (int)std::min<long>((long)kchunks, std::max<long>(1, (2048 + wgs - 1) / wgs));
-      h->d_part.reserve((size_t)nsplit * per);
-      for (int ix = 0; ix < nc; ++ix) {
-        const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
-        RP_HIP(hipEventRecord(h->gev[2 * ix], s));
-        launch_chi_gemm(precision, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, nsplit, h->d_part.p, s);
-        RP_HIP(hipEventRecord(h->gev[2 * ix + 1], s));
-        launch_chi_reduce(h->d_part.p, per, nsplit, 1.0 / h->volume, h->d_chi.p + (size_t)ix * per, s);
-      }
-    }
+    const int nc = h->q0 ? 3 : 1;
+    const ChiLaunch c = chi_launch(h, precision);
+    issue_accumulate(h, precision, c, true);
     if (chi_out)
-      RP_HIP(hipMemcpyAsync(chi_out, h->d_chi.p, sizeof(double2) * nc * per, hipMemcpyDeviceToHost, s));
+      RP_HIP(hipMemcpyAsync(chi_out, h->d_chi.p, sizeof(double2) * nc * (size_t)h->ng * h->ng * h->nen,
+                            hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
     h->gemm_seconds = 0;
     h->gemm_launches = nc;
@@ -646,30 +796,11 @@ int rp_dyson(rp_handle h, double* epsinv_out) {
     RP_CHECK(h && h->qready && h->have_chi, "rp_dyson: chi missing");
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
-    const int n = h->ng, nb = h->nbatch();
-    const size_t per = (size_t)n * n;
-    h->d_eps.reserve((size_t)nb * per);
-    h->d_gjwork.reserve(gauss_jordan_work_ints(nb));
-    h->d_ipiv.reserve((size_t)nb * n);
-    h->d_wd.reserve(3);
-    RP_HIP(hipMemsetAsync(h->d_status.p, 0, sizeof(int), s));
-    if (h->q0 && !h->intra.empty()) {
-      // Drude plasma frequency of the intraband bands (fRPA metals at q = 0)
-      h->d_drude.reserve(3 * h->intra.size());
-      launch_drude(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, h->ntk, h->ntg, h->d_intra.p,
-                   (int)h->intra.size(), h->d_sk0.p, h->d_bvec.p, h->d_eband.p, h->ef, h->delt,
-                   h->dmna, h->dmnr, h->nkb[0], h->nkb[1], h->nkb[2], h->d_imt.p, h->d_index.p,
-                   h->volume, h->d_status.p, h->d_drude.p, h->d_wd.p, s);
-    } else {
-      const double wd0[3] = {0, 0, 0};  // no intraband pairs (cRPA, or an insulator): wpl = 0
-      RP_HIP(hipMemcpyAsync(h->d_wd.p, wd0, sizeof wd0, hipMemcpyHostToDevice, s));
-    }
-    launch_eps_build(h->d_chi.p, h->d_qn.p, n, nb, h->nen, h->q0 ? 1 : 0, h->g0, h->d_wd.p, h->d_em.p,
-                     h->delt, h->d_eps.p, s);
-    launch_gauss_jordan(h->d_eps.p, n, nb, h->d_ipiv.p, h->d_gjwork.p, h->d_status.p, s);
+    issue_dyson(h);
     check_status(h, "Dyson step");
     if (epsinv_out)
-      RP_HIP(hipMemcpyAsync(epsinv_out, h->d_eps.p, sizeof(double2) * nb * per, hipMemcpyDeviceToHost, s));
+      RP_HIP(hipMemcpyAsync(epsinv_out, h->d_eps.p, sizeof(double2) * h->nbatch() * (size_t)h->ng * h->ng,
+                            hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
   });
 }

@@ -386,7 +386,7 @@ void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd,
   RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
   RP_CHECK(nwg % kXcd == 0 && nwg > 0, "bf16x3 contraction needs a multiple of 8 workgroups");
   RP_CHECK(sync != nullptr, "bf16x3 contraction needs its rendezvous counters");
-  RP_HIP(hipMemsetAsync(sync, 0, sizeof(unsigned) * bf16x3_sync_words(), s));
+  launch_zero(sync, sizeof(unsigned) * bf16x3_sync_words(), s);
   const int ntile = (ng + kTM - 1) / kTM;
   const size_t nchunks = kpad / kKC;
   const size_t per = (nchunks + nsplit - 1) / nsplit;

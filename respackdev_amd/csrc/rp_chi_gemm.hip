@@ -20,6 +20,8 @@
 //          accumulators flushed into FP64 every kFlush chunks.
 // The K range may be split over gridDim.z; partial tiles are summed in fixed
 // order by k_chi_reduce (deterministic), which also divides by the volume.
+#include <algorithm>
+
 #include "rp_common.h"
 #include "rp_kernels.h"
 
@@ -289,6 +291,27 @@ void launch_chi_gemm(int prec, const float2* A, size_t lda, const double2* dw, s
     default:
       throw Error("unknown precision mode");
   }
+  RP_HIP(hipGetLastError());
+}
+
+// Zero fill by vector stores.  Used instead of hipMemsetAsync wherever the
+// launch sequence may be captured into a graph (memset nodes are avoided there).
+__global__ void k_zero(uint4* __restrict__ p, size_t n16, unsigned* __restrict__ tail,
+                       size_t ntail) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride)
+    p[i] = make_uint4(0u, 0u, 0u, 0u);
+  if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0u;
+}
+
+void launch_zero(void* p, size_t bytes, hipStream_t s) {
+  if (bytes == 0) return;
+  RP_CHECK(bytes % 4 == 0 && ((uintptr_t)p % 16) == 0, "zero fill needs 16-byte aligned words");
+  const size_t n16 = bytes / 16, ntail = (bytes % 16) / 4;
+  int blocks = (int)std::min<size_t>((n16 + 255) / 256, 4096);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_zero, dim3(blocks), dim3(256), 0, s, (uint4*)p, n16,
+                     (unsigned*)((char*)p + n16 * 16), ntail);
   RP_HIP(hipGetLastError());
 }
 

@@ -54,6 +54,63 @@ double ecut_for_psi(const double b[3][3], int nk_irr, const double* ski, const i
 struct W3d;  // calc_int state (rp_w3d.cpp)
 struct Mlp;  // band-tail closure network (rp_mlp_state.h)
 
+// Launch parameters of the chi contraction: K split and (split-BF16 only)
+// persistent workgroups.
+struct ChiLaunch {
+  int nsplit = 1;
+  int nwg = 0;
+};
+
+// One autotuned contraction shape and the launch chosen for it.
+struct TuneEntry {
+  int prec = 0, ng = 0, nen = 0, ndir = 1;
+  size_t kpad = 0;
+  ChiLaunch best, heuristic;
+  double best_ms = 0, heuristic_ms = 0;
+  int ncand = 0;
+};
+
+// The per-q state of a handle, parked while another q is current (batched
+// runs keep one per q; swap_q exchanges it with the handle's).
+struct QSlot {
+  bool qready = false, q0 = false, have_A = false, have_w = false, have_chi = false;
+  double q[3] = {0, 0, 0};
+  int ng = 0, g0 = -1, npair = 0;
+  std::vector<int> lg, ikq, shift, pocc, pvir, intra;
+  std::vector<double> qn;
+  size_t lda = 0, kpad = 0;
+  double ecut_eps_win = 0;
+  int n_vir_win = 0;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra;
+  DevBuf<double> d_qn;
+  DevBuf<double2> d_eps;  // eps^-1 of this q (a batched run's output)
+  ChiLaunch launch;
+};
+
+// Work buffers of the stages (shared by all q of one launch sequence).
+struct Scratch {
+  DevBuf<float2> d_A, d_A3, d_wf;
+  DevBuf<unsigned char> d_Ao;
+  DevBuf<unsigned> d_sync;
+  DevBuf<double2> d_w, d_part, d_chi, d_drude;
+  DevBuf<double> d_wd;
+  DevBuf<int> d_status, d_ipiv, d_gjwork;
+};
+
+// A batched configuration: every q prepared, its launch sequence captured in
+// one HIP graph over buffers that only the graph uses.
+struct Batch {
+  std::vector<QSlot> slots;
+  Scratch scratch;
+  int prec = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+};
+
 }  // namespace rp
 
 // ------------------------------------------------------------------ context
@@ -110,6 +167,12 @@ struct rp_ctx {
   DevBuf<unsigned> d_sync;     // per-XCD rendezvous counters of the split-BF16 contraction
   DevBuf<double2> d_w, d_part, d_chi, d_eps;
   DevBuf<double> d_qn, d_wd;
+  // launch sequence being captured into a graph: no host syncs or status
+  // read-back, buffers must already be large enough
+  bool capturing = false;
+  int autotune = 0;              // time the contraction's launch candidates per shape
+  std::vector<rp::TuneEntry> tune;
+  std::unique_ptr<rp::Batch> batch;
 
   rp_ctx();   // defined where rp::W3d and rp::Mlp are complete (rp_w3d.cpp)
   ~rp_ctx();
@@ -123,3 +186,17 @@ struct rp_ctx {
   int nbatch() const { return q0 ? 3 * nen : nen; }
 };
 
+namespace rp {
+// Stage bodies (rp_capi.cpp): enqueue only, on h->stream.
+void check_status(rp_ctx* h, const char* what);
+void issue_pairdens(rp_ctx* h);
+void issue_weights(rp_ctx* h);
+void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events);
+void issue_dyson(rp_ctx* h);
+ChiLaunch chi_launch_heuristic(const rp_ctx* h, int prec);
+// The launch for the current q: the tuned one for its shape, tuning it now
+// when autotuning is on and the inputs are ready, else the heuristic.
+ChiLaunch chi_launch(rp_ctx* h, int prec);
+void swap_q(rp_ctx* h, QSlot& s);
+void swap_scratch(rp_ctx* h, Scratch& s);
+}  // namespace rp

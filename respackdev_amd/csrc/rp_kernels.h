@@ -47,6 +47,8 @@ void launch_chi_gemm(int prec, const float2* A, size_t lda, const double2* dw, s
                      size_t kpad, int ng, int nen, int nsplit, double2* partial, hipStream_t s);
 void launch_chi_reduce(const double2* partial, size_t n, int nsplit, double inv_vol, double2* chi,
                        hipStream_t s);
+// zero `bytes` (a multiple of 4) at p (16-byte aligned) with a fill kernel
+void launch_zero(void* p, size_t bytes, hipStream_t s);
 
 // rp_chi_bf16.hip
 size_t bf16x3_lda_multiple();
