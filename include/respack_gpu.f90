@@ -34,6 +34,8 @@ module respack_gpu
   public :: rp_chiqw_open, rp_chiqw_set_crpa, rp_chiqw_info, rp_chiqw_kmesh
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
+  public :: rp_chiqw_batch_plan, rp_chiqw_batch_run, rp_chiqw_batch_result, rp_chiqw_batch_free
+  public :: rp_chiqw_autotune, rp_chiqw_tune_count, rp_chiqw_tune_info
   public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
   public :: rp_mlp_gradient, rp_mlp_train, rp_chi_set
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
@@ -167,6 +169,49 @@ module respack_gpu
       type(c_ptr), value :: h
       real(c_double), value :: ecut_for_eps
       integer(c_int32_t), value :: n_vir
+    end function
+    ! batched configuration under graph capture; contraction autotuner (no
+    ! counterpart in the reference)
+    integer(c_int) function rp_chiqw_batch_plan(h, nq, qs, ecut_for_eps, n_vir, precision) &
+        bind(C, name="rp_chiqw_batch_plan")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nq, precision
+      type(c_ptr), value :: qs, ecut_for_eps, n_vir
+    end function
+    integer(c_int) function rp_chiqw_batch_run(h, seconds) bind(C, name="rp_chiqw_batch_run")
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: h
+      real(c_double) :: seconds
+    end function
+    integer(c_int) function rp_chiqw_batch_result(h, iq, ng, ndir, lg, lg_cap, epsinv_out) &
+        bind(C, name="rp_chiqw_batch_result")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: iq, lg_cap
+      integer(c_int32_t) :: ng, ndir
+      type(c_ptr), value :: lg, epsinv_out
+    end function
+    integer(c_int) function rp_chiqw_batch_free(h) bind(C, name="rp_chiqw_batch_free")
+      import :: c_ptr, c_int
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function rp_chiqw_autotune(h, enable) bind(C, name="rp_chiqw_autotune")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: enable
+    end function
+    integer(c_int) function rp_chiqw_tune_count(h, n) bind(C, name="rp_chiqw_tune_count")
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t) :: n
+    end function
+    integer(c_int) function rp_chiqw_tune_info(h, idx, iout, dout) bind(C, name="rp_chiqw_tune_info")
+      import :: c_ptr, c_int, c_int32_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: idx
+      integer(c_int32_t) :: iout(10)
+      real(c_double) :: dout(2)
     end function
     ! calc_w3d (calc_intW.F)
     integer(c_int) function rp_w3d_open(h, lattice, nsym, nnp, rg, pg, nk_irr, ski, ngi, ntg, &

@@ -12,6 +12,9 @@
 ! MPI_* namelist entries are accepted and ignored (one process drives one GPU).
 ! Contraction_precision is an addition: 0 = FP64 (default, the reference's
 ! arithmetic), 1 = FP32, 2 = split-BF16 (both checked to 1e-5 against FP64).
+! Batch_q_graph is another: 1 runs every remaining q as one batch captured
+! into a HIP graph (all eps^-1 held on the device until written), with the
+! contraction autotuned per shape; 0 (default) runs the q one after another.
 program calc_chiqw
   use iso_c_binding
   use respack_gpu
@@ -28,11 +31,13 @@ program calc_chiqw
   integer :: file_num_chi_base_start = 500000, file_num_eps_base_start = 600000
   integer :: n_calc_q = 0
   integer :: Contraction_precision = 0
+  integer :: Batch_q_graph = 0
   namelist /param_chiqw/ N_CALC_BAND, Ecut_for_eps, shift_ef, Max_excitation_energy, &
     Green_func_delt, ttrhdrn_dmna, ttrhdrn_dmnr, MPI_io_rank, MPI_num_proc_per_qcomm, &
     MPI_num_qcomm, Num_freq_grid, Lower_bound_energy_window, Upper_bound_energy_window, &
     flg_cRPA, flg_calc_type, file_num_log_start, file_num_chi_start, file_num_eps_start, &
-    file_num_chi_base_start, file_num_eps_base_start, n_calc_q, Contraction_precision
+    file_num_chi_base_start, file_num_eps_base_start, n_calc_q, Contraction_precision, &
+    Batch_q_graph
 
   type(wfn_data) :: w
   type(c_ptr) :: h
@@ -41,7 +46,8 @@ program calc_chiqw
   integer(c_int32_t) :: iout(16), ng, ngc
   integer(c_int32_t), allocatable :: ns(:), nb(:)
   integer(c_int32_t), allocatable, target :: lg(:, :)
-  real(8) :: dout(8), q(3), b(3, 3), volume, timings(8)
+  real(8) :: dout(8), q(3), b(3, 3), volume, timings(8), tbatch
+  real(8), allocatable, target :: qall(:, :)
   complex(8), allocatable :: umat(:, :, :)
   complex(8), allocatable, target :: em(:)
   complex(8), allocatable, target :: eps(:, :, :, :)
@@ -136,6 +142,18 @@ program calc_chiqw
   call write_chi_cutoff(dout(2))
   call write_wgrid(em)
 
+  if (Batch_q_graph == 1 .and. nq > 0) then
+    ! every remaining q in one captured launch sequence, replayed once
+    allocate(qall(3, nq))
+    do iq = 1, nq
+      qall(:, iq) = fold_q(w%ski(:, qlist(iq)))
+    end do
+    call rp_check(rp_chiqw_autotune(h, 1_c_int32_t), 'rp_chiqw_autotune')
+    call rp_check(rp_chiqw_batch_plan(h, int(nq, c_int32_t), c_loc(qall), c_null_ptr, c_null_ptr, &
+                                      int(Contraction_precision, c_int32_t)), 'rp_chiqw_batch_plan')
+    call rp_check(rp_chiqw_batch_run(h, tbatch), 'rp_chiqw_batch_run')
+    write(6, '(a,i6,a,f10.4,a)') ' batch of', nq, ' q replayed from one graph:', tbatch, ' s'
+  end if
   do iq = 1, nq
     q = fold_q(w%ski(:, qlist(iq)))
     qnum = 0
@@ -143,18 +161,30 @@ program calc_chiqw
       if (all(abs(fold_q(w%ski(:, ik)) - q) < 1.0d-6)) qnum = ik
     end do
     write(dirname, "('dir-eps/q',i3.3)") qnum
-    call rp_check(rp_chiqw_prepare_q(h, q, ng, c_null_ptr, 0_c_int32_t), 'rp_chiqw_prepare_q')
-    if (allocated(lg)) deallocate(lg)
-    allocate(lg(3, ng))
-    call rp_check(rp_chiqw_prepare_q(h, q, ngc, c_loc(lg), ng), 'rp_chiqw_prepare_q')
-    ndir = 1
-    if (q(1) == 0.0d0 .and. q(2) == 0.0d0 .and. q(3) == 0.0d0) ndir = 3
-    if (allocated(eps)) deallocate(eps)
-    allocate(eps(ng, ng, nen, ndir))
-    call rp_check(rp_chiqw_q(h, q, int(Contraction_precision, c_int32_t), c_loc(eps), c_null_ptr, &
-                             timings), 'rp_chiqw_q')
-    write(6, '(a,i4,a,3f10.5,a,i6,a,f10.4,a)') ' q', qnum, ' =', q, '  NG=', ng, '  device ', &
-      timings(5), ' s'
+    if (Batch_q_graph == 1) then
+      call rp_check(rp_chiqw_batch_result(h, int(iq - 1, c_int32_t), ng, ndir, c_null_ptr, 0_c_int32_t, &
+                                          c_null_ptr), 'rp_chiqw_batch_result')
+      if (allocated(lg)) deallocate(lg)
+      allocate(lg(3, ng))
+      if (allocated(eps)) deallocate(eps)
+      allocate(eps(ng, ng, nen, ndir))
+      call rp_check(rp_chiqw_batch_result(h, int(iq - 1, c_int32_t), ngc, ndir, c_loc(lg), ng, &
+                                          c_loc(eps)), 'rp_chiqw_batch_result')
+      write(6, '(a,i4,a,3f10.5,a,i6)') ' q', qnum, ' =', q, '  NG=', ng
+    else
+      call rp_check(rp_chiqw_prepare_q(h, q, ng, c_null_ptr, 0_c_int32_t), 'rp_chiqw_prepare_q')
+      if (allocated(lg)) deallocate(lg)
+      allocate(lg(3, ng))
+      call rp_check(rp_chiqw_prepare_q(h, q, ngc, c_loc(lg), ng), 'rp_chiqw_prepare_q')
+      ndir = 1
+      if (q(1) == 0.0d0 .and. q(2) == 0.0d0 .and. q(3) == 0.0d0) ndir = 3
+      if (allocated(eps)) deallocate(eps)
+      allocate(eps(ng, ng, nen, ndir))
+      call rp_check(rp_chiqw_q(h, q, int(Contraction_precision, c_int32_t), c_loc(eps), c_null_ptr, &
+                               timings), 'rp_chiqw_q')
+      write(6, '(a,i4,a,3f10.5,a,i6,a,f10.4,a)') ' q', qnum, ' =', q, '  NG=', ng, '  device ', &
+        timings(5), ' s'
+    end if
     call write_sq(trim(dirname), q)
     do ix = 1, ndir
       call write_eps_diag(trim(dirname), file_num_eps_base_start + (ix - 1) * 1000, &

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "respackdev_amd", "bin", "calc_chiqw")
 
 
-def _case(tmp_path, calc_k, prec=0, name="tiny", crpa=0):
+def _case(tmp_path, calc_k, prec=0, name="tiny", crpa=0, batch=0):
     wl = gen.WORKLOADS[name]
     keys = ["a", "nk", "nocc", "nvir", "ecut_psi", "seed"] + (["n_cross"] if "n_cross" in wl else [])
     s = gen.make_system(**{k: wl[k] for k in keys})
@@ -31,6 +31,7 @@ def _case(tmp_path, calc_k, prec=0, name="tiny", crpa=0):
            f" flg_cRPA = {crpa}\n"
            f" n_calc_q = {len(calc_k)}\n"
            f" Contraction_precision = {prec}\n"
+           f" Batch_q_graph = {batch}\n"
            "/\n" + " ".join(str(k) for k in calc_k) + "\n")
     return wl, from_system(s), nml
 
@@ -193,3 +194,27 @@ def test_host_metal(tmp_path):
            for ie in range(wl["nen"])]
     print(f"metal q=0 eps^-1 per frequency {np.array2string(np.array(per), precision=1)}", flush=True)
     assert max(np.array(per)[low]) <= 1e-6 and max(per) <= 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 2])
+def test_host_batch_graph(tmp_path, prec):
+    """Batch_q_graph = 1: every q of the run (here the first 4 of the 2^3
+    mesh, q = 0 among them) in one captured graph, autotuned; the files match
+    the oracle as the q-by-q run does (FP64 1e-6; split-BF16 1e-5)."""
+    from oracle import pipeline
+    wl, w, nml = _case(tmp_path, [1, 2, 3, 4], prec=prec, batch=1)
+    r = _run(tmp_path, nml)
+    print(r.stdout[-2500:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    assert "replayed from one graph" in r.stdout
+    tol = 1e-6 if prec == 0 else 1e-5
+    for k in range(1, 5):
+        q = tuple(float(x) for x in w.ski[k - 1])
+        o = pipeline.run_q(w, q, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        q0 = np.all(np.abs(q) < 1e-12)
+        eps = _eps_files(tmp_path / "dir-eps", k, q0, o["lg"].shape[0], wl["nen"])
+        err = np.linalg.norm(eps - o["epsinv"]) / np.linalg.norm(o["epsinv"])
+        print(f"q{k} {q}: eps^-1 rel err {err:.3e}", flush=True)
+        assert err <= tol
+        assert (tmp_path / "dir-eps" / f"q{k:03d}" / "dat.log.400").exists()
