@@ -35,6 +35,7 @@ void swap_q(rp_ctx* h, QSlot& s) {
   h->pocc.swap(s.pocc);
   h->pvir.swap(s.pvir);
   h->intra.swap(s.intra);
+  h->pdplan.swap(s.pdplan);
   h->qn.swap(s.qn);
   std::swap(h->lda, s.lda);
   std::swap(h->kpad, s.kpad);
@@ -46,6 +47,7 @@ void swap_q(rp_ctx* h, QSlot& s) {
   h->d_pocc.swap(s.d_pocc);
   h->d_pvir.swap(s.d_pvir);
   h->d_intra.swap(s.d_intra);
+  h->d_pdplan.swap(s.d_pdplan);
   h->d_qn.swap(s.d_qn);
   h->d_eps.swap(s.d_eps);
 }

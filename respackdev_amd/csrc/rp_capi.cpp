@@ -478,6 +478,8 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       h->d_pocc.upload(h->pocc, s);
       h->d_pvir.upload(h->pvir, s);
       if (!h->intra.empty()) h->d_intra.upload(h->intra, s);
+      h->pdplan = pairdens_plan(h->lg, h->nw[0], h->nw[1], h->nw[2]);
+      h->d_pdplan.upload(h->pdplan, s);
       h->d_qn.upload(h->qn, s);
       RP_HIP(hipStreamSynchronize(s));
       h->qready = true;
@@ -540,9 +542,9 @@ void issue_pairdens(rp_ctx* h) {
   const size_t K = (size_t)h->npair * h->ntk;
   grow(h, h->d_A, h->kpad * h->lda, "pair-density buffer");
   if (h->kpad > K) launch_zero(h->d_A.p + K * h->lda, (h->kpad - K) * h->lda * sizeof(float2), s);
-  launch_pair_densities(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
-                        h->nw[0], h->nw[1], h->nw[2], h->d_lg.p, h->ng, h->lda, (int)K, h->d_tw.p,
-                        h->d_A.p, s);
+  launch_pair_densities_pruned(h->d_ur.p, h->ntk, h->d_pocc.p, h->d_pvir.p, h->d_ikq.p, h->d_shift.p,
+                               h->nw[0], h->nw[1], h->nw[2], h->d_pdplan.p, h->pdplan[0], h->ng,
+                               h->lda, (int)K, h->d_tw.p, h->d_A.p, s);
   if (h->q0) {
     grow(h, h->d_A3, 3 * h->kpad * h->lda, "q = 0 pair-density buffer");
     for (int ix = 0; ix < 3; ++ix)

@@ -76,12 +76,12 @@ struct QSlot {
   bool qready = false, q0 = false, have_A = false, have_w = false, have_chi = false;
   double q[3] = {0, 0, 0};
   int ng = 0, g0 = -1, npair = 0;
-  std::vector<int> lg, ikq, shift, pocc, pvir, intra;
+  std::vector<int> lg, ikq, shift, pocc, pvir, intra, pdplan;
   std::vector<double> qn;
   size_t lda = 0, kpad = 0;
   double ecut_eps_win = 0;
   int n_vir_win = 0;
-  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra, d_pdplan;
   DevBuf<double> d_qn;
   DevBuf<double2> d_eps;  // eps^-1 of this q (a batched run's output)
   ChiLaunch launch;
@@ -155,11 +155,12 @@ struct rp_ctx {
   double q[3] = {0, 0, 0};
   int ng = 0, g0 = -1, npair = 0;
   std::vector<int> lg, ikq, shift, pocc, pvir;
-  std::vector<int> intra;  // q = 0, fRPA: partially occupied bands paired with themselves
+  std::vector<int> intra;   // q = 0, fRPA: partially occupied bands paired with themselves
+  std::vector<int> pdplan;  // pruned pair-density transform of the chi sphere
   std::vector<double> qn;
   size_t lda = 0, kpad = 0;
   double prep_seconds = 0;
-  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra, d_status, d_ipiv, d_gjwork;
+  DevBuf<int> d_lg, d_ikq, d_shift, d_pocc, d_pvir, d_intra, d_pdplan, d_status, d_ipiv, d_gjwork;
   DevBuf<double2> d_drude;  // per intraband band and direction: sum_k Im x(k) p(k)^2
   DevBuf<float2> d_A, d_A3;
   DevBuf<unsigned char> d_Ao;  // split-bf16 octets of the left operand

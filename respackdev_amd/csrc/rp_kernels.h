@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <vector>
 
 #include "../../include/respack_gpu.h"
 
@@ -16,6 +17,15 @@ void launch_twiddles(double2* tab, hipStream_t s);
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                        int nband, int nx, int ny, int nz, const double2* twtab, float2* ur,
                        hipStream_t s);
+// Plan of the pruned forward transform for a chi sphere lg (see rp_pairdens.hip);
+// plan[0] is the number of kept z planes.
+std::vector<int> pairdens_plan(const std::vector<int>& lg, int nx, int ny, int nz);
+// Pair densities as launch_pair_densities, computing only the transform
+// outputs the chi sphere needs (bit-identical values).
+void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, const int* pvir,
+                                  const int* ikq, const int* shift, int nx, int ny, int nz,
+                                  const int* plan, int nzz, int ng, size_t lda, int nK,
+                                  const double2* twtab, float2* A, hipStream_t s);
 void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
                            const int* ikq, const int* shift, int nx, int ny, int nz,
                            const int* lg, int ng, size_t lda, int nK, const double2* twtab,

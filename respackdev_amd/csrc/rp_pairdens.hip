@@ -15,6 +15,8 @@
 // per-thread line DFTs (the axis lengths are the reference's 2,3,5-smooth
 // sizes, <= 32), so HBM sees one read of the inputs and one write of the
 // outputs per transform.
+#include <vector>
+
 #include "rp_common.h"
 #include "rp_kernels.h"
 
@@ -230,6 +232,221 @@ __global__ void __launch_bounds__(256) k_pair_densities(
   }
 }
 
+// ----------------------------------------------- pruned forward transform
+// Only the chi sphere's points of the forward transform are read, so each axis
+// computes only the outputs that a later axis or the final read needs (every
+// output that is computed takes exactly the operations of the full transform):
+//   z axis: straight from the band grids in registers (no LDS pass for the
+//           product), only the z planes that hold sphere points, stored
+//           compacted: g[x + nx (y + ny iz)], iz the plane's rank;
+//   y axis: the lines (x, iz) of those planes, only the y rows that hold
+//           sphere points in plane iz;
+//   x axis: only the lines (y, iz) that hold sphere points, only their x.
+// The plan (per q, built on the host from the sphere): [0] nzz, [1] nxl,
+// [2] zmask, [3] 0, ymask[nzz], xline[nxl] = y | iz << 8, xmask[nxl],
+// pos[ng] = position of sphere point i in the compacted grid.
+std::vector<int> pairdens_plan(const std::vector<int>& lg, int nx, int ny, int nz) {
+  const int ng = (int)(lg.size() / 3);
+  std::vector<int> px(ng), py(ng), pz(ng);
+  unsigned zmask = 0;
+  for (int i = 0; i < ng; ++i) {
+    px[i] = ((-lg[3 * i]) % nx + nx) % nx;
+    py[i] = ((-lg[3 * i + 1]) % ny + ny) % ny;
+    pz[i] = ((-lg[3 * i + 2]) % nz + nz) % nz;
+    zmask |= 1u << pz[i];
+  }
+  std::vector<int> izof(nz, -1);
+  int nzz = 0;
+  for (int z = 0; z < nz; ++z)
+    if ((zmask >> z) & 1u) izof[z] = nzz++;
+  std::vector<unsigned> ymask(nzz, 0u), xm((size_t)ny * nzz, 0u);
+  for (int i = 0; i < ng; ++i) {
+    const int iz = izof[pz[i]];
+    ymask[iz] |= 1u << py[i];
+    xm[(size_t)iz * ny + py[i]] |= 1u << px[i];
+  }
+  std::vector<int> xline, xmask;
+  for (int iz = 0; iz < nzz; ++iz)
+    for (int y = 0; y < ny; ++y)
+      if (xm[(size_t)iz * ny + y]) {
+        xline.push_back(y | (iz << 8));
+        xmask.push_back((int)xm[(size_t)iz * ny + y]);
+      }
+  const int nxl = (int)xline.size();
+  std::vector<int> plan;
+  plan.reserve(4 + nzz + 2 * nxl + ng);
+  plan.push_back(nzz);
+  plan.push_back(nxl);
+  plan.push_back((int)zmask);
+  plan.push_back(0);
+  for (int iz = 0; iz < nzz; ++iz) plan.push_back((int)ymask[iz]);
+  plan.insert(plan.end(), xline.begin(), xline.end());
+  plan.insert(plan.end(), xmask.begin(), xmask.end());
+  for (int i = 0; i < ng; ++i) plan.push_back(px[i] + nx * (py[i] + ny * izof[pz[i]]));
+  return plan;
+}
+
+// Outputs k of mask of one line v (twiddle rows of tw, wave-uniform), to
+// dst[k * stride] (z axis: the compacted plane of k).
+template <int N, bool ZPLANES>
+__device__ __forceinline__ void masked_line(const double2 (&v)[N], unsigned mask, const double2* tw,
+                                            double2* dst, int stride) {
+#pragma unroll 1
+  for (int k = 0; k < N; ++k) {
+    if (!((mask >> k) & 1u)) continue;
+    const double2* row = tw + k * N;
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const double2 w = row[j];
+      re = fma(v[j].x, w.x, re);
+      re = fma(-v[j].y, w.y, re);
+      im = fma(v[j].x, w.y, im);
+      im = fma(v[j].y, w.x, im);
+    }
+    const int slot = ZPLANES ? __builtin_popcount(mask & ((1u << k) - 1u)) : k;
+    dst[slot * stride] = make_double2(re, im);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void pd_zaxis(const float2* __restrict__ uk, const float2* __restrict__ ukq,
+                                         const double2* ph, int nx, int ny, unsigned zmask,
+                                         const double2* tw, double2* g) {
+  const int nl = nx * ny;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    double2 v[N];
+    if (ph) {
+      const int r1 = l % nx, r2 = l / nx;
+      const double2 px = ph[r1], py = ph[nx + r2];
+      const double qx = px.x * py.x - px.y * py.y, qy = px.x * py.y + px.y * py.x;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double2 pz = ph[nx + ny + j];
+        const double fx = qx * pz.x - qy * pz.y, fy = qx * pz.y + qy * pz.x;
+        const float2 a = uk[l + nl * j], b = ukq[l + nl * j];
+        const double bx = (double)b.x * fx - (double)b.y * fy;
+        const double by = (double)b.x * fy + (double)b.y * fx;
+        const double ax2 = a.x, ay2 = a.y;
+        v[j] = make_double2(bx * ax2 + by * ay2, bx * ay2 - by * ax2);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const float2 a = uk[l + nl * j], b = ukq[l + nl * j];
+        const double bx = b.x, by = b.y, ax = a.x, ay = a.y;
+        v[j] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
+      }
+    }
+    masked_line<N, true>(v, zmask, tw, g + l, nl);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void pd_yaxis(double2* g, int nx, int nzz, const int* __restrict__ ymask,
+                                         const double2* tw) {
+  const int nl = nx * nzz;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    const int x = l % nx, iz = l / nx;
+    double2* base = g + x + nx * N * iz;
+    double2 v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = base[j * nx];
+    masked_line<N, false>(v, (unsigned)ymask[iz], tw, base, nx);
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void pd_xaxis(double2* g, int ny, int nxl, const int* __restrict__ xline,
+                                         const int* __restrict__ xmask, const double2* tw) {
+  for (int m = threadIdx.x; m < nxl; m += blockDim.x) {
+    const int y = xline[m] & 255, iz = xline[m] >> 8;
+    double2* base = g + N * (y + ny * iz);
+    double2 v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = base[j];
+    masked_line<N, false>(v, (unsigned)xmask[m], tw, base, 1);
+  }
+}
+
+#define RP_PD_SWITCH(n, MAXN, CALL)                                                            \
+  switch (n) {                                                                                 \
+    RP_PD_CASE(1, MAXN, CALL) RP_PD_CASE(2, MAXN, CALL) RP_PD_CASE(3, MAXN, CALL)              \
+    RP_PD_CASE(4, MAXN, CALL) RP_PD_CASE(5, MAXN, CALL) RP_PD_CASE(6, MAXN, CALL)              \
+    RP_PD_CASE(8, MAXN, CALL) RP_PD_CASE(9, MAXN, CALL) RP_PD_CASE(10, MAXN, CALL)             \
+    RP_PD_CASE(12, MAXN, CALL) RP_PD_CASE(15, MAXN, CALL) RP_PD_CASE(16, MAXN, CALL)           \
+    RP_PD_CASE(18, MAXN, CALL) RP_PD_CASE(20, MAXN, CALL) RP_PD_CASE(24, MAXN, CALL)           \
+    RP_PD_CASE(25, MAXN, CALL) RP_PD_CASE(27, MAXN, CALL) RP_PD_CASE(30, MAXN, CALL)           \
+    RP_PD_CASE(32, MAXN, CALL)                                                                 \
+    default: break;                                                                            \
+  }
+#define RP_PD_CASE(NN, MAXN, CALL) \
+  case NN: {                       \
+    constexpr int N = NN;          \
+    if constexpr (N <= MAXN) CALL; \
+  } break;
+
+// grid: K blocks as k_pair_densities; the same inputs and outputs, plus the
+// plan of the chi sphere.
+template <int MAXN>
+__global__ void __launch_bounds__(256) k_pair_densities_pruned(
+    const float2* __restrict__ ur, int ntk, const int* __restrict__ pocc,
+    const int* __restrict__ pvir, const int* __restrict__ ikq, const int* __restrict__ shift,
+    int nx, int ny, int nz, const int* __restrict__ plan, int ng, size_t lda, int k_offset,
+    const double2* __restrict__ twtab, float2* __restrict__ A) {
+  extern __shared__ double2 lds[];
+  const int nzz = plan[0], nxl = plan[1];
+  const unsigned zmask = (unsigned)plan[2];
+  const int* ymask = plan + 4;
+  const int* xline = ymask + nzz;
+  const int* xmask = xline + nxl;
+  const int* pos = xmask + nxl;
+  double2* g = lds;                           // [nzz][ny][nx]
+  double2* ph = g + (size_t)nx * ny * nzz;    // umklapp phase factors per axis
+  const int K = blockIdx.x + k_offset;
+  const int p = K / ntk, k = K % ntk;
+  const int kq = ikq[k];
+  const int nr = nx * ny * nz;
+  const float2* uk = ur + ((size_t)pocc[p] * ntk + k) * nr;
+  const float2* ukq = ur + ((size_t)pvir[p] * ntk + kq) * nr;
+  const int d1 = shift[3 * k], d2 = shift[3 * k + 1], d3 = shift[3 * k + 2];
+  const bool umklapp = d1 != 0 || d2 != 0 || d3 != 0;  // uniform over the block
+  if (umklapp) {
+    // exp(-i 2 pi r.dG/N) = phx(r1) phy(r2) phz(r3) (chiqw_sub.F:165-185)
+    const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+    for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x) {
+      const int ax = i < nx ? 0 : (i < nx + ny ? 1 : 2);
+      const int ri = ax == 0 ? i : (ax == 1 ? i - nx : i - nx - ny);
+      const int n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+      const int dg = ax == 0 ? d1 : (ax == 1 ? d2 : d3);
+      double sn, cs;
+      sincos(tpi * ((double)ri * (double)dg / (double)n), &sn, &cs);
+      ph[i] = make_double2(cs, -sn);
+    }
+    __syncthreads();
+  }
+  const double2* phz = umklapp ? ph : nullptr;
+  // z, then y, then x (the order of fft3_fw, m_fft3d_20150826.f90:1598-1720)
+  RP_PD_SWITCH(nz, MAXN, (pd_zaxis<N>(uk, ukq, phz, nx, ny, zmask, twiddles_of(twtab, N, -1), g)))
+  __syncthreads();
+  RP_PD_SWITCH(ny, MAXN, (pd_yaxis<N>(g, nx, nzz, ymask, twiddles_of(twtab, N, -1))))
+  __syncthreads();
+  RP_PD_SWITCH(nx, MAXN, (pd_xaxis<N>(g, ny, nxl, xline, xmask, twiddles_of(twtab, N, -1))))
+  __syncthreads();
+  const double scl = 1.0 / (double)nr;
+  float2* out = A + (size_t)K * lda;
+  for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) {
+    if (i < ng) {
+      const double2 v = g[pos[i]];
+      out[i] = make_float2((float)(v.x * scl), (float)(v.y * scl));
+    } else {
+      out[i] = make_float2(0.f, 0.f);
+    }
+  }
+}
+#undef RP_PD_CASE
+#undef RP_PD_SWITCH
+
 // q = 0: the G = 0 entry per direction ix is -i<vir|p|occ>/de (calc_VMab,
 // chiqw_sub.F:207-232, and chiqw.F:1787-1805), zero when |de| <= delt.
 // Writes three copies A3[ix][K][lda] from A[K][lda] with that entry replaced.
@@ -303,6 +520,29 @@ void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int n
   else
     hipLaunchKernelGGL(k_band_grids<32>, grid, dim3(256), lds, s, coef, kg, ng0, ntk, ntg, nx, ny, nz, twtab, ur);
   RP_HIP(hipGetLastError());
+}
+
+void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, const int* pvir,
+                                  const int* ikq, const int* shift, int nx, int ny, int nz,
+                                  const int* plan, int nzz, int ng, size_t lda, int nK,
+                                  const double2* twtab, float2* A, hipStream_t s) {
+  const size_t lds = sizeof(double2) * ((size_t)nx * ny * nzz + nx + ny + nz);
+  const bool small = nx <= 16 && ny <= 16 && nz <= 16;
+  const void* fn =
+      small ? (const void*)k_pair_densities_pruned<16> : (const void*)k_pair_densities_pruned<32>;
+  if (lds > 65536)
+    RP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int chunk = 1 << 30;
+  for (int off = 0; off < nK; off += chunk) {
+    const int n = nK - off < chunk ? nK - off : chunk;
+    if (small)
+      hipLaunchKernelGGL(k_pair_densities_pruned<16>, dim3(n), dim3(256), lds, s, ur, ntk, pocc,
+                         pvir, ikq, shift, nx, ny, nz, plan, ng, lda, off, twtab, A);
+    else
+      hipLaunchKernelGGL(k_pair_densities_pruned<32>, dim3(n), dim3(256), lds, s, ur, ntk, pocc,
+                         pvir, ikq, shift, nx, ny, nz, plan, ng, lda, off, twtab, A);
+    RP_HIP(hipGetLastError());
+  }
 }
 
 void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
