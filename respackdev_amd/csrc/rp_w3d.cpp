@@ -51,7 +51,7 @@ struct W3d {
   DevBuf<float2> d_ur, d_eps, d_A;
   DevBuf<long long> d_eps_off, d_rho_off, d_map_off;
   DevBuf<int4> d_qinfo;
-  DevBuf<int> d_minv, d_lg, d_ikq, d_shift, d_pw;
+  DevBuf<int> d_minv, d_lg, d_ikq, d_shift, d_pw, d_plan;
   DevBuf<double> d_qg, d_sq;
   DevBuf<double2> d_rho, d_pf1, d_pf2, d_func, d_shell, d_funcw, d_cheadw, d_vmat, d_wmat;
 };
@@ -499,8 +499,13 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
       RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
       RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
       RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
-      launch_pair_densities(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_ikq.p, w.d_shift.p, w.nw[0], w.nw[1],
-                            w.nw[2], w.d_lg.p, np, lda, nwf * ntk, w.d_tw.p, w.d_A.p, s);
+      const std::vector<int> plan = pairdens_plan(
+          std::vector<int>(w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np), w.nw[0], w.nw[1], w.nw[2]);
+      w.d_plan.reserve(plan.size());
+      RP_HIP(hipMemcpyAsync(w.d_plan.p, plan.data(), sizeof(int) * plan.size(), hipMemcpyHostToDevice, s));
+      launch_pair_densities_pruned(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_ikq.p, w.d_shift.p, w.nw[0],
+                                   w.nw[1], w.nw[2], w.d_plan.p, plan[0], np, lda, nwf * ntk, w.d_tw.p,
+                                   w.d_A.p, s);
       launch_w_rho(w.d_A.p, lda, ntk, nwf, np, w.d_qg.p, g0, w.d_rho.p + w.rho_off[iq], s);
       // the host vectors are reused by the next q: wait for the copies
       RP_HIP(hipStreamSynchronize(s));
@@ -606,8 +611,13 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
       RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
       RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
       RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
-      launch_pair_densities(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_ikq.p, w.d_shift.p, w.nw[0], w.nw[1],
-                            w.nw[2], w.d_lg.p, np, lda, npair * ntk, w.d_tw.p, w.d_A.p, s);
+      const std::vector<int> plan = pairdens_plan(
+          std::vector<int>(w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np), w.nw[0], w.nw[1], w.nw[2]);
+      w.d_plan.reserve(plan.size());
+      RP_HIP(hipMemcpyAsync(w.d_plan.p, plan.data(), sizeof(int) * plan.size(), hipMemcpyHostToDevice, s));
+      launch_pair_densities_pruned(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_ikq.p, w.d_shift.p, w.nw[0],
+                                   w.nw[1], w.nw[2], w.d_plan.p, plan[0], np, lda, npair * ntk,
+                                   w.d_tw.p, w.d_A.p, s);
       for (int r = 0; r < nr; ++r)
         launch_j_rho(w.d_A.p, lda, ntk, npair, np, w.d_qg.p, geo.g0, geo.q0 ? 1 : 0,
                      d_pfk.p + (size_t)r * ntk, d_jrho.p + (size_t)r * tot + joff[iq], s);
