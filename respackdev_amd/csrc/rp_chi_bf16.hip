@@ -33,8 +33,9 @@
 // instead of HBM.
 // Every kFlushChunks chunks the FP32 accumulators are added into the FP64
 // partial tile of the item in HBM, which the workgroup owns exclusively (FP64
-// accumulation across K blocks of 4096, FP32 within; the first flush of an
-// item stores).
+// accumulation across K blocks of 8192, FP32 within; the first flush of an
+// item stores).  At C2 shapes a span of 4096 K costs 4% more time (the flush
+// read-modify-writes the tile) for chi errors of 4.6e-7 instead of 6.5e-7.
 #include <hip/hip_bfloat16.h>
 
 #include <algorithm>
@@ -53,7 +54,7 @@ constexpr int kTM = 128, kTN = 128;   // tile (G x G')
 constexpr int kKC = 32;               // complex K per chunk
 constexpr int kK8 = kKC / 8;          // octets per row per chunk
 constexpr int kRowBytes = kK8 * 64 + 16;  // padded LDS row (bytes)
-constexpr int kFlushChunks = 128;     // FP32 accumulation span: 4096 K
+constexpr int kFlushChunks = 256;     // FP32 accumulation span: 8192 K
 constexpr int kXcd = 8;               // XCDs of gfx950 (workgroup b -> XCD b % 8)
 constexpr int kSyncChunks = 32;       // chunks between rendezvous of an XCD's workgroups
 constexpr int kSyncSpin = 512;        // bound on the polls of one rendezvous
