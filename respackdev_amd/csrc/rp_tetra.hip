@@ -218,9 +218,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   }
   if (threadIdx.x < 2) count[threadIdx.x] = 0;
   __syncthreads();
-  // ---- pass 1: cut test (ttct), keep test, class
-  for (int it = threadIdx.x; it < nitem; it += blockDim.x) {
-    const int t = it % ntet;
+  // ---- pass 1: cut test (ttct), keep test, class, per tetrahedron: its two
+  // items (dsgn = +1 and -1) have vertex values g and -g, hence the same cut
+  // and keep tests and the same class (chk1 compares |g_p - g_q| and
+  // |g_p + g_q|); the list holds the dsgn = +1 item
+  for (int it = threadIdx.x; it < ntet; it += blockDim.x) {
+    const int t = it;
     const int c0 = imt[4 * t], c1 = imt[4 * t + 1], c2 = imt[4 * t + 2], c3 = imt[4 * t + 3];
     const double a0 = e1[c0], a1 = e1[c1], a2 = e1[c2], a3 = e1[c3];
     const double b0 = e2[c0], b1 = e2[c1], b2 = e2[c2], b3 = e2[c3];
@@ -273,43 +276,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const zd K31 = P2 * -s6, K32 = r12 * P1 * s6, K33 = r23 * P3 * s6, K34 = r02 * P0 * s6;
     // sub1normal(A0,A3,A1,A2)
     const zd J1 = P0 * 0.5, J2 = P1 * -0.5, J3 = P2 * -0.5;
-    const double ds = m.dsgn;
+    // Both items of the tetrahedron: dsgn = s takes the vertex values s*A, so
+    // every r_ij takes the factor s: the factors of three of them (K11, K21,
+    // K31, J1..J3) change sign, those of four do not (exactly, in IEEE
+    // arithmetic).  Their corner terms are summed before the LDS atomics.
 #pragma unroll 1
     for (int f = 0; f < nfb; ++f) {
       const double2 ev = em[f0 + f];
       const zd e = mkz(ev.x, ev.y);
-      const zd v0 = A0 + e, v1 = A1 + e, v2 = A2 + e, v3 = A3 + e;
-      const zd L0 = logz(v0), L1 = logz(v1), L2 = logz(v2), L3 = logz(v3);
-      const zd D10 = logdiff(L1, L0), D20 = logdiff(L2, L0), D30 = logdiff(L3, L0);
-      const zd D21 = logdiff(L2, L1), D31 = logdiff(L3, L1), D32 = logdiff(L3, L2);
-      const zd q0 = v0 * v0, q1 = v1 * v1, q2 = v2 * v2, q3 = v3 * v3;
-      const zd u0 = q0 * v0, u1 = q1 * v1, u2 = q2 * v2, u3 = q3 * v3;
-      zd c1 = -q0 * K11;
-      c1 = c1 - u2 * D20 * K12;
-      c1 = c1 + u3 * D30 * K13;
-      c1 = c1 + u1 * D10 * K14;
-      zd c2 = -q1 * K21;
-      c2 = c2 - u2 * D21 * K22;
-      c2 = c2 + u3 * D31 * K23;
-      c2 = c2 - u0 * D10 * K24;
-      zd c3 = -q2 * K31;
-      c3 = c3 + u1 * D21 * K32;
-      c3 = c3 + u3 * D32 * K33;
-      c3 = c3 - u0 * D20 * K34;
-      const zd l1 = q3 * D30;
-      zd c4 = -l1 * J1;
-      c4 = c4 - (l1 - q1 * D10) * J2;
-      c4 = c4 + (l1 - q2 * D20) * J3;
-      lds_add(x0 + f, c1 * ds);
-      lds_add(x1 + f, c2 * ds);
-      lds_add(x2 + f, c3 * ds);
-      lds_add(x3 + f, (c4 - c3 - c2 - c1) * ds);
+      zd T0 = mkz(0.0, 0.0), T1 = mkz(0.0, 0.0), T2 = mkz(0.0, 0.0), T3 = mkz(0.0, 0.0);
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        const double s = sg ? -1.0 : 1.0;
+        const zd v0 = A0 * s + e, v1 = A1 * s + e, v2 = A2 * s + e, v3 = A3 * s + e;
+        const zd L0 = logz(v0), L1 = logz(v1), L2 = logz(v2), L3 = logz(v3);
+        const zd D10 = logdiff(L1, L0), D20 = logdiff(L2, L0), D30 = logdiff(L3, L0);
+        const zd D21 = logdiff(L2, L1), D31 = logdiff(L3, L1), D32 = logdiff(L3, L2);
+        const zd q0 = v0 * v0, q1 = v1 * v1, q2 = v2 * v2, q3 = v3 * v3;
+        const zd u0 = q0 * v0, u1 = q1 * v1, u2 = q2 * v2, u3 = q3 * v3;
+        zd c1 = -q0 * (K11 * s);
+        c1 = c1 - u2 * D20 * K12;
+        c1 = c1 + u3 * D30 * K13;
+        c1 = c1 + u1 * D10 * K14;
+        zd c2 = -q1 * (K21 * s);
+        c2 = c2 - u2 * D21 * K22;
+        c2 = c2 + u3 * D31 * K23;
+        c2 = c2 - u0 * D10 * K24;
+        zd c3 = -q2 * (K31 * s);
+        c3 = c3 + u1 * D21 * K32;
+        c3 = c3 + u3 * D32 * K33;
+        c3 = c3 - u0 * D20 * K34;
+        const zd l1 = q3 * D30;
+        zd c4 = -l1 * (J1 * s);
+        c4 = c4 - (l1 - q1 * D10) * (J2 * s);
+        c4 = c4 + (l1 - q2 * D20) * (J3 * s);
+        T0 = T0 + c1 * s;
+        T1 = T1 + c2 * s;
+        T2 = T2 + c3 * s;
+        T3 = T3 + (c4 - c3 - c2 - c1) * s;
+      }
+      lds_add(x0 + f, T0);
+      lds_add(x1 + f, T1);
+      lds_add(x2 + f, T2);
+      lds_add(x3 + f, T3);
     }
   }
   // ---- pass 3: degenerate tetrahedra (calc, m_tetrahedron_20170325.F:231-300)
-  for (int j = threadIdx.x; j < nother; j += blockDim.x) {
+  for (int jj = threadIdx.x; jj < 2 * nother; jj += blockDim.x) {
+    const int j = jj >> 1;  // entry j, both of its items
     const int2 ent = list[nitem - 1 - j];
-    const Item m = load_item(ent.x, ntet, imt, e1, e2, delt);
+    const Item m = load_item(ent.x + (jj & 1) * ntet, ntet, imt, e1, e2, delt);
     const int code = ent.y, kind = code & 15;
     const int b0 = vtx(code, 0, 0), b1 = vtx(code, 0, 1), b2 = vtx(code, 0, 2), b3 = vtx(code, 0, 3);
     const int o0 = vtx(code, 1, 0), o1 = vtx(code, 1, 1);
