@@ -10,6 +10,8 @@
 // interchanges as column swaps.
 #include <algorithm>
 
+#include <utility>
+
 #include "rp_common.h"
 #include "rp_kernels.h"
 
@@ -178,6 +180,139 @@ __global__ void __launch_bounds__(1024) k_gj_panel(double2* __restrict__ a, int 
   if (tid == 0) mv[0] = singular ? -1 : nb + nmove;
 }
 
+// The same panel step for full panels (nb = 16) of n <= 768 rows, with the
+// panel in registers: thread i holds row i of the 16 panel columns.  The 16
+// column steps are expanded at compile time (gj_reg_step<L>), so every register
+// index is a constant.  The pivot search reads the threads' own rows, and the
+// pivot row and row c are exchanged through LDS slots: three barriers per
+// column.  The arithmetic is k_gj_panel's, operation for operation.
+struct GjRegShared {
+  double wv[16];
+  int wi[16];
+  double2 rowp[kPanelMax], rowc[kPanelMax], prow[kPanelMax];
+  int orig[768 + 1];  // + 1: a pivot search over NaNs can return row n, as in k_gj_panel
+  int nmove;
+  int singular;
+};
+
+// Returns true on a zero pivot (the same for every thread of the block: the
+// pivot is read from LDS after a barrier).
+template <int L>
+__device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared& sh, int i, int n,
+                                            int c0, bool mine, int b, int* __restrict__ ipiv,
+                                            int* __restrict__ status) {
+  constexpr int NB = kPanelMax;
+  const int lane = i & 63, wave = i >> 6, nwave = (int)(blockDim.x >> 6);
+  const int c = c0 + L;
+  // pivot: largest |Re|+|Im| in rows c..n-1, first index on ties (izamax)
+  double best = (mine && i >= c) ? fabs(r[L].x) + fabs(r[L].y) : -1.0;
+  int bi = (mine && i >= c) ? i : n;
+  for (int off = 32; off > 0; off >>= 1) {
+    const double v2 = __shfl_xor(best, off);
+    const int i2 = __shfl_xor(bi, off);
+    if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+  }
+  if (lane == 0) { sh.wv[wave] = best; sh.wi[wave] = bi; }
+  __syncthreads();
+  best = sh.wv[0];
+  bi = sh.wi[0];
+  for (int w = 1; w < nwave; ++w)
+    if (sh.wv[w] > best || (sh.wv[w] == best && sh.wi[w] < bi)) { best = sh.wv[w]; bi = sh.wi[w]; }
+  const int p = bi;
+  if (i == 0) {
+    ipiv[(size_t)b * n + c] = p;
+    const int t = sh.orig[c];
+    sh.orig[c] = sh.orig[p];
+    sh.orig[p] = t;
+  }
+  if (i == p) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) sh.rowp[q] = r[q];
+  }
+  if (i == c) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) sh.rowc[q] = r[q];
+  }
+  __syncthreads();
+  const double2 piv = sh.rowp[L];
+  if (piv.x == 0.0 && piv.y == 0.0) {  // uniform: every thread reads the same pivot
+    if (i == 0) { atomicOr(status, 4); sh.singular = 1; }
+    return true;
+  }
+  // rows c and p exchanged (the pivot row moves to c)
+  if (i == p && p != c) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) r[q] = sh.rowc[q];
+  }
+  // pivot row c scaled (its column L becomes 1/piv)
+  if (i < NB) {
+    const zd pinv = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
+    const zd u = i == L ? pinv : mkz(sh.rowp[i].x, sh.rowp[i].y) * pinv;
+    sh.prow[i] = make_double2(u.x, u.y);
+  }
+  __syncthreads();
+  if (i == c) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q) r[q] = sh.prow[q];
+  } else if (mine) {
+    const zd t = mkz(r[L].x, r[L].y);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const zd v = q == L ? mkz(0.0, 0.0) : mkz(r[q].x, r[q].y);
+      const zd u = v - t * mkz(sh.prow[q].x, sh.prow[q].y);
+      r[q] = make_double2(u.x, u.y);
+    }
+  }
+  return false;
+}
+
+template <int... L>
+__device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, L...>, double2 (&r)[kPanelMax],
+                                             GjRegShared& sh, int i, int n, int c0, bool mine, int b,
+                                             int* __restrict__ ipiv, int* __restrict__ status) {
+  // in order; a zero pivot ends the sequence for every thread alike (the flag
+  // is each thread's own, set from the same LDS value)
+  bool dead = false;
+  ((dead ? void() : void(dead = gj_reg_step<L>(r, sh, i, n, c0, mine, b, ipiv, status))), ...);
+}
+
+__global__ void __launch_bounds__(768) k_gj_panel_reg(double2* __restrict__ a, int n, int c0,
+                                                      int* __restrict__ ipiv,
+                                                      int* __restrict__ moves,
+                                                      int* __restrict__ status) {
+  constexpr int NB = kPanelMax;
+  __shared__ GjRegShared sh;
+  const int b = blockIdx.x, i = threadIdx.x;
+  double2* m = a + (size_t)b * n * n;
+  const bool mine = i < n;
+  double2 r[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) r[q] = mine ? m[i + (size_t)n * (c0 + q)] : make_double2(0.0, 0.0);
+  if (mine) sh.orig[i] = i;
+  if (i == 0) { sh.nmove = 0; sh.singular = 0; }
+  __syncthreads();
+  gj_reg_steps(std::make_integer_sequence<int, NB>{}, r, sh, i, n, c0, mine, b, ipiv, status);
+  __syncthreads();
+  const bool singular = sh.singular != 0;
+  if (!singular && mine)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) m[i + (size_t)n * (c0 + q)] = r[q];
+  // move list: the nb panel rows first (in order), then other rows whose
+  // content changed
+  int* mv = moves + (size_t)b * kMoveInts;
+  if (i < NB) {
+    mv[1 + i] = c0 + i;
+    mv[1 + kMoveMax + i] = sh.orig[c0 + i];
+  }
+  if (mine && (i < c0 || i >= c0 + NB) && sh.orig[i] != i) {
+    const int e = NB + atomicAdd(&sh.nmove, 1);
+    mv[1 + e] = i;
+    mv[1 + kMoveMax + e] = sh.orig[i];
+  }
+  __syncthreads();
+  if (i == 0) mv[0] = singular ? -1 : NB + sh.nmove;
+}
+
 // Rank-nb update of every column outside the panel.  grid: (ceil((n-nb)/TJ),
 // nbatch), 256 threads; workgroup = TJ whole columns of one matrix, so the
 // reads of moved rows (staged in LDS before the barrier) never race with
@@ -270,8 +405,12 @@ void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, in
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       maxlds = (int)lds;
     }
-    hipLaunchKernelGGL(k_gj_panel, dim3(nbatch), dim3(1024), lds, s, a, n, c0, w, ipiv, work,
-                       status);
+    if (n <= 768 && w == kPanelMax)
+      hipLaunchKernelGGL(k_gj_panel_reg, dim3(nbatch), dim3(768), 0, s, a, n, c0, ipiv, work,
+                         status);
+    else
+      hipLaunchKernelGGL(k_gj_panel, dim3(nbatch), dim3(1024), lds, s, a, n, c0, w, ipiv, work,
+                         status);
     if (n - w > 0)
       hipLaunchKernelGGL(k_gj_update, dim3((n - w + kTJ - 1) / kTJ, nbatch), dim3(256), 0, s, a, n,
                          c0, w, (const int*)work);
