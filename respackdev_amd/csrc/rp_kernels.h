@@ -20,16 +20,12 @@ void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int n
 // Plan of the pruned forward transform for a chi sphere lg (see rp_pairdens.hip);
 // plan[0] is the number of kept z planes.
 std::vector<int> pairdens_plan(const std::vector<int>& lg, int nx, int ny, int nz);
-// Pair densities as launch_pair_densities, computing only the transform
-// outputs the chi sphere needs (bit-identical values).
+// Pair densities of every (pair, k) for the points lg of a G sphere (the plan
+// of pairdens_plan): only the transform outputs the sphere needs are computed.
 void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, const int* pvir,
                                   const int* ikq, const int* shift, int nx, int ny, int nz,
                                   const int* plan, int nzz, int ng, size_t lda, int nK,
                                   const double2* twtab, float2* A, hipStream_t s);
-void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
-                           const int* ikq, const int* shift, int nx, int ny, int nz,
-                           const int* lg, int ng, size_t lda, int nK, const double2* twtab,
-                           float2* A, hipStream_t s);
 void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                      const int* pocc, const int* pvir, const double* eband, const double* bvec,
                      double delt, int g0, int ng, size_t lda, size_t rows, int nK, const float2* A,

@@ -10,11 +10,12 @@
 // (fft3_fw), read at -G for the chi sphere and rounded to single precision
 // (chiqw_sub.F:192-202): A[K][lda] complex64, K = pair*NTK + k.
 //
-// Both transforms are LDS-resident: the whole FP64 grid (<= 10000 points)
-// lives in one workgroup's LDS and each axis is done in place as independent
-// per-thread line DFTs (the axis lengths are the reference's 2,3,5-smooth
-// sizes, <= 32), so HBM sees one read of the inputs and one write of the
-// outputs per transform.
+// Both transforms are LDS-resident: the FP64 grid (<= 10000 points) lives in
+// one workgroup's LDS and each axis is done in place as independent per-thread
+// line DFTs (the axis lengths are the reference's 2,3,5-smooth sizes, <= 32),
+// so HBM sees one read of the inputs and one write of the outputs per
+// transform.  The forward transform of stage 2 is pruned to the outputs the
+// sphere needs (below).
 #include <vector>
 
 #include "rp_common.h"
@@ -160,76 +161,6 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
   float2* out = ur + ((size_t)b * ntk + k) * nr;
   for (int r = threadIdx.x; r < nr; r += blockDim.x)
     out[r] = make_float2((float)g[r].x, (float)g[r].y);  // CMPLX(): single precision
-}
-
-// grid: K = npair*ntk blocks (flattened).  u_occ = ur of band pocc[p] at k,
-// u_vir = ur of band pvir[p] at ikq[k]; bands are rows of ur (occ and vir
-// both index the same band array).
-template <int MAXN>
-__global__ void __launch_bounds__(256) k_pair_densities(
-    const float2* __restrict__ ur, int ntk, const int* __restrict__ pocc,
-    const int* __restrict__ pvir, const int* __restrict__ ikq, const int* __restrict__ shift,
-    int nx, int ny, int nz, const int* __restrict__ lg, int ng, size_t lda, int k_offset,
-    const double2* __restrict__ twtab, float2* __restrict__ A) {
-  extern __shared__ double2 lds[];
-  const int nr = nx * ny * nz;
-  double2* g = lds;
-  const int K = blockIdx.x + k_offset;
-  const int p = K / ntk, k = K % ntk;
-  const int kq = ikq[k];
-  const float2* uk = ur + ((size_t)pocc[p] * ntk + k) * nr;
-  const float2* ukq = ur + ((size_t)pvir[p] * ntk + kq) * nr;
-  const int d1 = shift[3 * k], d2 = shift[3 * k + 1], d3 = shift[3 * k + 2];
-  if (d1 == 0 && d2 == 0 && d3 == 0) {
-    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-      const float2 a = uk[r], b = ukq[r];
-      // conj(u_kq') * u_k
-      const double bx = b.x, by = b.y, ax = a.x, ay = a.y;
-      g[r] = make_double2(bx * ax + by * ay, bx * ay - by * ax);
-    }
-  } else {
-    // umklapp phase exp(-i 2 pi r.dG/N) = phx(r1) phy(r2) phz(r3), the 1D
-    // factors of calc_InterStateMatrix (chiqw_sub.F:165-185), staged in LDS
-    // behind the grid
-    double2* ph = g + nr;
-    const double tpi = 2.0 * 3.141592653589793238462643383279502884;
-    for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x) {
-      const int ax = i < nx ? 0 : (i < nx + ny ? 1 : 2);
-      const int ri = ax == 0 ? i : (ax == 1 ? i - nx : i - nx - ny);
-      const int n = ax == 0 ? nx : (ax == 1 ? ny : nz);
-      const int dg = ax == 0 ? d1 : (ax == 1 ? d2 : d3);
-      double sn, cs;
-      sincos(tpi * ((double)ri * (double)dg / (double)n), &sn, &cs);
-      ph[i] = make_double2(cs, -sn);
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < nr; r += blockDim.x) {
-      const int r1 = r % nx, r2 = (r / nx) % ny, r3 = r / (nx * ny);
-      const double2 px = ph[r1], py = ph[nx + r2], pz = ph[nx + ny + r3];
-      // pf = phx * phy * phz
-      const double qx = px.x * py.x - px.y * py.y, qy = px.x * py.y + px.y * py.x;
-      const double fx = qx * pz.x - qy * pz.y, fy = qx * pz.y + qy * pz.x;
-      const float2 a = uk[r], b = ukq[r];
-      // u_kq * pf, then conj(u_kq') * u_k
-      const double bx = (double)b.x * fx - (double)b.y * fy;
-      const double by = (double)b.x * fy + (double)b.y * fx;
-      const double ax2 = a.x, ay2 = a.y;
-      g[r] = make_double2(bx * ax2 + by * ay2, bx * ay2 - by * ax2);
-    }
-  }
-  __syncthreads();
-  dft3<MAXN>(g, nx, ny, nz, -1, twtab);
-  const double scl = 1.0 / (double)nr;
-  float2* out = A + (size_t)K * lda;
-  for (int i = threadIdx.x; i < (int)lda; i += blockDim.x) {
-    if (i < ng) {
-      int r = wrap(-lg[3 * i], nx) + nx * (wrap(-lg[3 * i + 1], ny) + ny * wrap(-lg[3 * i + 2], nz));
-      double2 v = g[r];
-      out[i] = make_float2((float)(v.x * scl), (float)(v.y * scl));
-    } else {
-      out[i] = make_float2(0.f, 0.f);
-    }
-  }
 }
 
 // ----------------------------------------------- pruned forward transform
@@ -386,8 +317,11 @@ __device__ __forceinline__ void pd_xaxis(double2* g, int ny, int nxl, const int*
     if constexpr (N <= MAXN) CALL; \
   } break;
 
-// grid: K blocks as k_pair_densities; the same inputs and outputs, plus the
-// plan of the chi sphere.
+// grid: K = npair*ntk blocks (flattened).  u_occ = ur of band pocc[p] at k,
+// u_vir = ur of band pvir[p] at ikq[k] (both index the same band array), the
+// product conj(u_vir(r) e^{-i 2pi r.dG/N}) u_occ(r) in FP64, transformed with
+// e^{-i} and 1/N and read at -G for the chi sphere, rounded to single precision
+// (chiqw_sub.F:192-202): A[K][lda] complex64.
 template <int MAXN>
 __global__ void __launch_bounds__(256) k_pair_densities_pruned(
     const float2* __restrict__ ur, int ntk, const int* __restrict__ pocc,
@@ -545,28 +479,6 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
   }
 }
 
-void launch_pair_densities(const float2* ur, int ntk, const int* pocc, const int* pvir,
-                           const int* ikq, const int* shift, int nx, int ny, int nz,
-                           const int* lg, int ng, size_t lda, int nK, const double2* twtab, float2* A,
-                           hipStream_t s) {
-  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz + nx + ny + nz);
-  const bool small = nx <= 16 && ny <= 16 && nz <= 16;
-  const void* fn = small ? (const void*)k_pair_densities<16> : (const void*)k_pair_densities<32>;
-  if (lds > 65536)
-    RP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  // grid.x limited to 2^31-1; chunk for safety
-  const int chunk = 1 << 30;
-  for (int off = 0; off < nK; off += chunk) {
-    int n = nK - off < chunk ? nK - off : chunk;
-    if (small)
-      hipLaunchKernelGGL(k_pair_densities<16>, dim3(n), dim3(256), lds, s, ur, ntk, pocc, pvir,
-                         ikq, shift, nx, ny, nz, lg, ng, lda, off, twtab, A);
-    else
-      hipLaunchKernelGGL(k_pair_densities<32>, dim3(n), dim3(256), lds, s, ur, ntk, pocc, pvir,
-                         ikq, shift, nx, ny, nz, lg, ng, lda, off, twtab, A);
-    RP_HIP(hipGetLastError());
-  }
-}
 
 void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                      const int* pocc, const int* pvir, const double* eband, const double* bvec,
