@@ -5,9 +5,10 @@
 // The inversion is Gauss-Jordan with partial pivoting by |Re|+|Im| (the pivot
 // rule of LAPACK's izamax in zgetrf, which the reference calls through
 // invZGE.F90:1-23), blocked by panels of up to 16 columns: one workgroup per
-// matrix factors a panel in LDS, one batched kernel applies the panel's
-// rank-nb update to all other columns, and a final kernel undoes the row
-// interchanges as column swaps.
+// matrix factors a panel (in registers for n <= 768, else in LDS), a batched
+// kernel applies the panel's rank-nb update to all other columns (for n <= 768
+// in the same launch as the factoring of the next panel), and a final kernel
+// undoes the row interchanges as column swaps.
 #include <algorithm>
 
 #include <utility>
@@ -117,6 +118,7 @@ __global__ void __launch_bounds__(1024) k_gj_panel(double2* __restrict__ a, int 
     if (tid == 0) {
       for (int w = 1; w < nt / 64; ++w)
         if (wv[w] > best || (wv[w] == best && wi[w] < bi)) { best = wv[w]; bi = wi[w]; }
+      if (bi >= n) bi = c;  // no candidate compared (NaN column): keep row c, as izamax does
       wi[0] = bi;
       ipiv[(size_t)b * n + c] = bi;
       const int t = orig[c];
@@ -190,7 +192,7 @@ struct GjRegShared {
   double wv[16];
   int wi[16];
   double2 rowp[kPanelMax], rowc[kPanelMax], prow[kPanelMax];
-  int orig[768 + 1];  // + 1: a pivot search over NaNs can return row n, as in k_gj_panel
+  int orig[768];
   int nmove;
   int singular;
 };
@@ -218,7 +220,7 @@ __device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared
   bi = sh.wi[0];
   for (int w = 1; w < nwave; ++w)
     if (sh.wv[w] > best || (sh.wv[w] == best && sh.wi[w] < bi)) { best = sh.wv[w]; bi = sh.wi[w]; }
-  const int p = bi;
+  const int p = bi < n ? bi : c;  // no candidate compared (NaN column): keep row c
   if (i == 0) {
     ipiv[(size_t)b * n + c] = p;
     const int t = sh.orig[c];
@@ -276,6 +278,37 @@ __device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, L...>, d
   ((dead ? void() : void(dead = gj_reg_step<L>(r, sh, i, n, c0, mine, b, ipiv, status))), ...);
 }
 
+// Factors the panel [c0, c0+16) of matrix m held in r (row i of the panel in
+// thread i), writes it back and leaves the move list in mv.
+__device__ __forceinline__ void gj_reg_factor(double2 (&r)[kPanelMax], GjRegShared& sh, int i,
+                                              int n, int c0, bool mine, int b,
+                                              double2* __restrict__ m, int* __restrict__ ipiv,
+                                              int* __restrict__ mv, int* __restrict__ status) {
+  constexpr int NB = kPanelMax;
+  if (mine) sh.orig[i] = i;
+  if (i == 0) { sh.nmove = 0; sh.singular = 0; }
+  __syncthreads();
+  gj_reg_steps(std::make_integer_sequence<int, NB>{}, r, sh, i, n, c0, mine, b, ipiv, status);
+  __syncthreads();
+  const bool singular = sh.singular != 0;
+  if (!singular && mine)
+#pragma unroll
+    for (int q = 0; q < NB; ++q) m[i + (size_t)n * (c0 + q)] = r[q];
+  // move list: the nb panel rows first (in order), then other rows whose
+  // content changed
+  if (i < NB) {
+    mv[1 + i] = c0 + i;
+    mv[1 + kMoveMax + i] = sh.orig[c0 + i];
+  }
+  if (mine && (i < c0 || i >= c0 + NB) && sh.orig[i] != i) {
+    const int e = NB + atomicAdd(&sh.nmove, 1);
+    mv[1 + e] = i;
+    mv[1 + kMoveMax + e] = sh.orig[i];
+  }
+  __syncthreads();
+  if (i == 0) mv[0] = singular ? -1 : NB + sh.nmove;
+}
+
 __global__ void __launch_bounds__(768) k_gj_panel_reg(double2* __restrict__ a, int n, int c0,
                                                       int* __restrict__ ipiv,
                                                       int* __restrict__ moves,
@@ -288,29 +321,106 @@ __global__ void __launch_bounds__(768) k_gj_panel_reg(double2* __restrict__ a, i
   double2 r[NB];
 #pragma unroll
   for (int q = 0; q < NB; ++q) r[q] = mine ? m[i + (size_t)n * (c0 + q)] : make_double2(0.0, 0.0);
-  if (mine) sh.orig[i] = i;
-  if (i == 0) { sh.nmove = 0; sh.singular = 0; }
+  gj_reg_factor(r, sh, i, n, c0, mine, b, m, ipiv, moves + (size_t)b * kMoveInts, status);
+}
+
+// One step of the register path (n <= 768, full panels) with look-ahead: the
+// rank-16 update of panel k (columns [ck, ck+16), factored by the previous
+// launch, its move list in moves_k) and, when next is set, the factoring of
+// panel k+1 in the same launch.  Blocks [0, nbatch) (next set) are panel
+// blocks: they apply panel k's update to the 16 columns of panel k+1 in
+// registers and factor them at once (gj_reg_factor), writing the move list to
+// moves_next.  Every other block updates one group of up to 16 other columns
+// of one matrix: the columns outside [ck, ck + 16 (1 + next)), in groups of
+// 16, ngroups per matrix.  The two kinds touch disjoint columns and read
+// panel k's columns only, so nothing in the launch waits for anything else:
+// the panel factoring (one block per matrix, latency-bound) runs beside the
+// bandwidth-bound update instead of after it.  Thread i handles row i; the
+// arithmetic is k_gj_update's, operation for operation.  With nbatch a
+// multiple of 8 the update blocks of matrix b run on XCD b mod 8 (blocks are
+// dealt to the XCDs round-robin), so each XCD's L2 holds its matrices' panels.
+struct GjStepShared {
+  GjRegShared p;
+  double2 R[kPanelMax][kPanelMax];  // R[q][t]: panel row q's source row in column t (old)
+  int dst[kMoveMax], src[kMoveMax];
+};
+
+__global__ void __launch_bounds__(768) k_gj_step(double2* __restrict__ a, int n, int ck, int next,
+                                                 int nbatch, int ngroups, int* __restrict__ ipiv,
+                                                 const int* __restrict__ moves_k,
+                                                 int* __restrict__ moves_next,
+                                                 int* __restrict__ status) {
+  constexpr int NB = kPanelMax;
+  __shared__ GjStepShared sh;
+  const int i = threadIdx.x;
+  const bool mine = i < n;
+  const bool panel = next && (int)blockIdx.x < nbatch;
+  int b, cj, ntj;
+  if (panel) {
+    b = blockIdx.x;
+    cj = ck + NB;
+    ntj = NB;
+  } else {
+    const int u = (int)blockIdx.x - (next ? nbatch : 0);
+    int t;
+    if (nbatch % 8 == 0) {
+      const int x = u % 8, s = u / 8;
+      b = x + 8 * (s / ngroups);
+      t = s % ngroups;
+    } else {
+      b = u / ngroups;
+      t = u % ngroups;
+    }
+    const int skip = NB * (1 + next);  // columns [ck, ck + skip) are not in the groups
+    const int t0 = t * NB;             // groups never straddle ck (both multiples of 16)
+    cj = t0 < ck ? t0 : t0 + skip;
+    ntj = min(NB, n - skip - t0);
+  }
+  double2* m = a + (size_t)b * n * n;
+  const int* mv = moves_k + (size_t)b * kMoveInts;
+  const int nm = mv[0];
+  if (nm < 0) {  // a zero pivot earlier (the host reports it): nothing more for this matrix
+    if (panel && i == 0) moves_next[(size_t)b * kMoveInts] = -1;
+    return;  // uniform: every thread read the same word
+  }
+  if (i < nm) {
+    sh.dst[i] = mv[1 + i];
+    sh.src[i] = mv[1 + kMoveMax + i];
+  }
   __syncthreads();
-  gj_reg_steps(std::make_integer_sequence<int, NB>{}, r, sh, i, n, c0, mine, b, ipiv, status);
-  __syncthreads();
-  const bool singular = sh.singular != 0;
-  if (!singular && mine)
+  // old values: the pivot rows of the block's columns, and row i's source
+  if (i < NB * NB) {
+    const int q = i / NB, t = i % NB;
+    sh.R[q][t] = t < ntj ? m[sh.src[q] + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
+  }
+  const bool inpanel = i >= ck && i < ck + NB;
+  int srow = i;
+  for (int e = NB; e < nm; ++e)
+    if (sh.dst[e] == i) srow = sh.src[e];
+  double2 r[NB];
 #pragma unroll
-    for (int q = 0; q < NB; ++q) m[i + (size_t)n * (c0 + q)] = r[q];
-  // move list: the nb panel rows first (in order), then other rows whose
-  // content changed
-  int* mv = moves + (size_t)b * kMoveInts;
-  if (i < NB) {
-    mv[1 + i] = c0 + i;
-    mv[1 + kMoveMax + i] = sh.orig[c0 + i];
+  for (int t = 0; t < NB; ++t)
+    r[t] = (mine && !inpanel && t < ntj) ? m[srow + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
+  __syncthreads();  // every read of the block's columns precedes every write
+  if (mine) {
+    for (int q = 0; q < NB; ++q) {
+      const double2 wq = m[i + (size_t)n * (ck + q)];
+      const zd w = mkz(wq.x, wq.y);
+#pragma unroll
+      for (int t = 0; t < NB; ++t) {
+        const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.R[q][t].x, sh.R[q][t].y);
+        r[t] = make_double2(u.x, u.y);
+      }
+    }
   }
-  if (mine && (i < c0 || i >= c0 + NB) && sh.orig[i] != i) {
-    const int e = NB + atomicAdd(&sh.nmove, 1);
-    mv[1 + e] = i;
-    mv[1 + kMoveMax + e] = sh.orig[i];
+  if (!panel) {
+    if (mine)
+#pragma unroll
+      for (int t = 0; t < NB; ++t)
+        if (t < ntj) m[i + (size_t)n * (cj + t)] = r[t];
+    return;  // uniform per block
   }
-  __syncthreads();
-  if (i == 0) mv[0] = singular ? -1 : NB + sh.nmove;
+  gj_reg_factor(r, sh.p, i, n, cj, mine, b, m, ipiv, moves_next + (size_t)b * kMoveInts, status);
 }
 
 // Rank-nb update of every column outside the panel.  grid: (ceil((n-nb)/TJ),
@@ -388,16 +498,40 @@ void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, i
   RP_HIP(hipGetLastError());
 }
 
-size_t gauss_jordan_work_ints(int nbatch) { return (size_t)nbatch * kMoveInts; }
+// Two move lists per matrix: the fused step reads panel k's while its panel
+// blocks write panel k+1's.
+size_t gauss_jordan_work_ints(int nbatch) { return 2 * (size_t)nbatch * kMoveInts; }
 
 void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
                          hipStream_t s) {
-  // panel width: as many columns as fit in LDS next to the row map, at most 16
+  // move list of panel number k: work + (k & 1) * nbatch * kMoveInts
+  auto moves = [&](int k) { return work + (size_t)(k & 1) * nbatch * kMoveInts; };
+  int c0 = 0, panel = 0;
+  if (n <= 768 && n >= kPanelMax) {
+    // a thread per row (at least 256: the step stages 16 x 16 pivot-row values)
+    const int nt = std::max(256, (n + 63) / 64 * 64);
+    // full panels in registers, each launch updating panel k and factoring k+1
+    const int nfull = n / kPanelMax;
+    hipLaunchKernelGGL(k_gj_panel_reg, dim3(nbatch), dim3(nt), 0, s, a, n, 0, ipiv, moves(0), status);
+    for (int k = 0; k < nfull; ++k) {
+      const int next = k + 1 < nfull ? 1 : 0;
+      const int nrest = n - kPanelMax * (1 + next);
+      const int ngroups = (nrest + kPanelMax - 1) / kPanelMax;
+      const int nblk = (next ? nbatch : 0) + nbatch * ngroups;
+      if (nblk > 0)
+        hipLaunchKernelGGL(k_gj_step, dim3(nblk), dim3(nt), 0, s, a, n, k * kPanelMax, next,
+                           nbatch, ngroups, ipiv, (const int*)moves(k), moves(k + 1), status);
+    }
+    c0 = panel = nfull;
+    c0 *= kPanelMax;
+  }
+  // the remaining columns (every column when n > 768): panels in LDS, as many
+  // columns as fit next to the row map, at most 16, then a rank-nb update
   const size_t budget = 160 * 1024 - 1024 - sizeof(int) * (size_t)n;
-  int nb = (int)std::min<size_t>(kPanelMax, budget / (sizeof(double2) * (size_t)n));
+  const int nb = (int)std::min<size_t>(kPanelMax, budget / (sizeof(double2) * (size_t)n));
   RP_CHECK(nb >= 1, "Dyson: matrix too large for the LDS panel");
   int maxlds = 0;
-  for (int c0 = 0; c0 < n; c0 += nb) {
+  for (; c0 < n; ++panel) {
     const int w = std::min(nb, n - c0);
     const size_t lds = sizeof(double2) * (size_t)w * n + sizeof(int) * (size_t)n;
     if ((int)lds > maxlds && lds > 65536) {
@@ -405,15 +539,12 @@ void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, in
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       maxlds = (int)lds;
     }
-    if (n <= 768 && w == kPanelMax)
-      hipLaunchKernelGGL(k_gj_panel_reg, dim3(nbatch), dim3(768), 0, s, a, n, c0, ipiv, work,
-                         status);
-    else
-      hipLaunchKernelGGL(k_gj_panel, dim3(nbatch), dim3(1024), lds, s, a, n, c0, w, ipiv, work,
-                         status);
+    hipLaunchKernelGGL(k_gj_panel, dim3(nbatch), dim3(1024), lds, s, a, n, c0, w, ipiv,
+                       moves(panel), status);
     if (n - w > 0)
       hipLaunchKernelGGL(k_gj_update, dim3((n - w + kTJ - 1) / kTJ, nbatch), dim3(256), 0, s, a, n,
-                         c0, w, (const int*)work);
+                         c0, w, (const int*)moves(panel));
+    c0 += w;
   }
   hipLaunchKernelGGL(k_gj_colswap, dim3(nbatch), dim3(256), 0, s, a, n, (const int*)ipiv);
   RP_HIP(hipGetLastError());
