@@ -201,11 +201,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     int* __restrict__ status, double2* __restrict__ dout) {
   extern __shared__ double2 lds[];
   const int nk3 = n1 * n2 * n3;
-  const int ntet = 6 * nk3, nitem = 2 * ntet;
+  const int ntet = 6 * nk3;
   double2* x = lds;                               // [nk3][nf]
-  double* e1 = (double*)(x + (size_t)nk3 * nf);   // [nk3]
+  double2* lv = x + (size_t)nk3 * nf;             // [nk3][nf][2] log(s A_k + e_f), s = +1, -1
+  double* e1 = (double*)(lv + (size_t)nk3 * nf * 2);  // [nk3]
   double* e2 = e1 + nk3;                          // [nk3]
-  int2* list = (int2*)(e2 + nk3);                 // [nitem] {item, class}
+  int2* list = (int2*)(e2 + nk3);                 // [ntet] {tetrahedron, class}
   __shared__ int count[2];
   const int p = blockIdx.x;
   const int f0 = blockIdx.y * nf, nfb = min(nf, nen - f0);
@@ -242,8 +243,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       atomicOr(status, 2);
       continue;
     }
-    const int slot = kind == kNormal ? atomicAdd(&count[0], 1) : nitem - 1 - atomicAdd(&count[1], 1);
+    const int slot = kind == kNormal ? atomicAdd(&count[0], 1) : ntet - 1 - atomicAdd(&count[1], 1);
     list[slot] = make_int2(it, code);
+  }
+  // the vertex logarithms of the non-degenerate closed forms: a vertex value
+  // depends on the k point alone, and each k is a corner of 24 tetrahedra
+  for (int i = threadIdx.x; i < nk3 * nfb * 2; i += blockDim.x) {
+    const int k = i / (2 * nfb), f = (i >> 1) % nfb;
+    const double sg = (i & 1) ? -1.0 : 1.0;
+    const double2 ev = em[f0 + f];
+    const zd v = mkz(-(e2[k] - e1[k]), delt) * sg + mkz(ev.x, ev.y);
+    const zd L = logz(v);
+    lv[((size_t)k * nf + f) * 2 + (i & 1)] = make_double2(L.x, L.y);
   }
   __syncthreads();
   const int nnormal = count[0], nother = count[1];
@@ -258,10 +269,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const zd A1 = sel4(i1, m.g[0], m.g[1], m.g[2], m.g[3]);
     const zd A2 = sel4(i2, m.g[0], m.g[1], m.g[2], m.g[3]);
     const zd A3 = sel4(i3, m.g[0], m.g[1], m.g[2], m.g[3]);
-    double2* x0 = x + (size_t)sel4(i0, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
-    double2* x1 = x + (size_t)sel4(i1, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
-    double2* x2 = x + (size_t)sel4(i2, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
-    double2* x3 = x + (size_t)sel4(i3, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
+    const int k0 = sel4(i0, m.c[0], m.c[1], m.c[2], m.c[3]);
+    const int k1 = sel4(i1, m.c[0], m.c[1], m.c[2], m.c[3]);
+    const int k2 = sel4(i2, m.c[0], m.c[1], m.c[2], m.c[3]);
+    const int k3 = sel4(i3, m.c[0], m.c[1], m.c[2], m.c[3]);
+    double2* x0 = x + (size_t)k0 * nf;
+    double2* x1 = x + (size_t)k1 * nf;
+    double2* x2 = x + (size_t)k2 * nf;
+    double2* x3 = x + (size_t)k3 * nf;
+    const double2* l0 = lv + (size_t)k0 * nf * 2;
+    const double2* l1 = lv + (size_t)k1 * nf * 2;
+    const double2* l2 = lv + (size_t)k2 * nf * 2;
+    const double2* l3 = lv + (size_t)k3 * nf * 2;
     // r_ij = 1/(A_i - A_j); P0..P3 the triple products around each vertex
     const zd r01 = 1.0 / (A0 - A1), r02 = 1.0 / (A0 - A2), r03 = 1.0 / (A0 - A3);
     const zd r12 = 1.0 / (A1 - A2), r13 = 1.0 / (A1 - A3), r23 = 1.0 / (A2 - A3);
@@ -289,7 +308,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       for (int sg = 0; sg < 2; ++sg) {
         const double s = sg ? -1.0 : 1.0;
         const zd v0 = A0 * s + e, v1 = A1 * s + e, v2 = A2 * s + e, v3 = A3 * s + e;
-        const zd L0 = logz(v0), L1 = logz(v1), L2 = logz(v2), L3 = logz(v3);
+        // log(v_i), v_i = s A_i + e, from the vertex table (the same operations)
+        const double2 w0 = l0[2 * f + sg], w1 = l1[2 * f + sg], w2 = l2[2 * f + sg],
+                      w3 = l3[2 * f + sg];
+        const zd L0 = mkz(w0.x, w0.y), L1 = mkz(w1.x, w1.y), L2 = mkz(w2.x, w2.y),
+                 L3 = mkz(w3.x, w3.y);
         const zd D10 = logdiff(L1, L0), D20 = logdiff(L2, L0), D30 = logdiff(L3, L0);
         const zd D21 = logdiff(L2, L1), D31 = logdiff(L3, L1), D32 = logdiff(L3, L2);
         const zd q0 = v0 * v0, q1 = v1 * v1, q2 = v2 * v2, q3 = v3 * v3;
@@ -324,7 +347,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   // ---- pass 3: degenerate tetrahedra (calc, m_tetrahedron_20170325.F:231-300)
   for (int jj = threadIdx.x; jj < 2 * nother; jj += blockDim.x) {
     const int j = jj >> 1;  // entry j, both of its items
-    const int2 ent = list[nitem - 1 - j];
+    const int2 ent = list[ntet - 1 - j];
     const Item m = load_item(ent.x + (jj & 1) * ntet, ntet, imt, e1, e2, delt);
     const int code = ent.y, kind = code & 15;
     const int b0 = vtx(code, 0, 0), b1 = vtx(code, 0, 1), b2 = vtx(code, 0, 2), b3 = vtx(code, 0, 3);
@@ -820,14 +843,14 @@ __global__ void k_drude_finish(const double2* __restrict__ part, int nb, double 
 }
 
 size_t tetra_lds_bytes(int nk3, int nf) {
-  return sizeof(double2) * (size_t)nk3 * nf + 2 * sizeof(double) * (size_t)nk3 +
-         sizeof(int2) * 12 * (size_t)nk3;
+  return 3 * sizeof(double2) * (size_t)nk3 * nf + 2 * sizeof(double) * (size_t)nk3 +
+         sizeof(int2) * 6 * (size_t)nk3;
 }
 
 // Frequencies per workgroup: as many as keep two workgroups per CU (the
 // register budget allows two waves per SIMD), evened out over the slices.
 int tetra_freqs_per_block(int nk3, int nen) {
-  const size_t fixed = tetra_lds_bytes(nk3, 0), per = sizeof(double2) * (size_t)nk3;
+  const size_t fixed = tetra_lds_bytes(nk3, 0), per = 3 * sizeof(double2) * (size_t)nk3;
   const size_t target = 80 * 1024;
   int nf = target > fixed ? (int)((target - fixed) / per) : 1;
   nf = std::max(1, std::min(nf, nen));
