@@ -197,6 +197,23 @@ struct GjRegShared {
   int singular;
 };
 
+// A double moved across lanes by DPP (two 32-bit moves; every lane has a source).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = (int)(unsigned)u, hi = (int)(unsigned)(u >> 32);
+  const unsigned lo2 = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+  const unsigned hi2 = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+  return __builtin_bit_cast(double, ((unsigned long long)hi2 << 32) | lo2);
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // Returns true on a zero pivot (the same for every thread of the block: the
 // pivot is read from LDS after a barrier).
 template <int L>
@@ -207,13 +224,18 @@ __device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared
   const int lane = i & 63, wave = i >> 6, nwave = (int)(blockDim.x >> 6);
   const int c = c0 + L;
   // pivot: largest |Re|+|Im| in rows c..n-1, first index on ties (izamax)
-  double best = (mine && i >= c) ? fabs(r[L].x) + fabs(r[L].y) : -1.0;
-  int bi = (mine && i >= c) ? i : n;
-  for (int off = 32; off > 0; off >>= 1) {
-    const double v2 = __shfl_xor(best, off);
-    const int i2 = __shfl_xor(bi, off);
-    if (v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
-  }
+  const double own = (mine && i >= c) ? fabs(r[L].x) + fabs(r[L].y) : -1.0;
+  // wave maximum: butterfly within rows of 16 lanes (DPP), then the four rows
+  double best = own;
+  best = fmax(best, dpp_d<0xB1>(best));   // quad_perm [1,0,3,2]
+  best = fmax(best, dpp_d<0x4E>(best));   // quad_perm [2,3,0,1]
+  best = fmax(best, dpp_d<0x124>(best));  // row_ror:4
+  best = fmax(best, dpp_d<0x128>(best));  // row_ror:8
+  best = fmax(fmax(readlane_d(best, 0), readlane_d(best, 16)),
+              fmax(readlane_d(best, 32), readlane_d(best, 48)));
+  // first row holding it (rows increase with the lane); none: no candidate here
+  const unsigned long long hit = __ballot(own == best);
+  int bi = (best >= 0.0 && hit) ? wave * 64 + (int)__builtin_ctzll(hit) : n;
   if (lane == 0) { sh.wv[wave] = best; sh.wi[wave] = bi; }
   __syncthreads();
   best = sh.wv[0];
@@ -336,12 +358,17 @@ __global__ void __launch_bounds__(768) k_gj_panel_reg(double2* __restrict__ a, i
 // panel k's columns only, so nothing in the launch waits for anything else:
 // the panel factoring (one block per matrix, latency-bound) runs beside the
 // bandwidth-bound update instead of after it.  Thread i handles row i; the
-// arithmetic is k_gj_update's, operation for operation.  With nbatch a
+// arithmetic is k_gj_update's, operation for operation.  The rows the move
+// list names (at most 32) are staged in LDS first; after that each thread
+// reads and writes its own row only, so the waves run without further
+// barriers and one wave's loads overlap another's arithmetic.  With nbatch a
 // multiple of 8 the update blocks of matrix b run on XCD b mod 8 (blocks are
 // dealt to the XCDs round-robin), so each XCD's L2 holds its matrices' panels.
 struct GjStepShared {
   GjRegShared p;
-  double2 R[kPanelMax][kPanelMax];  // R[q][t]: panel row q's source row in column t (old)
+  // old values of the moved rows in the block's columns: vals[e][t] = a(src_e, cj + t);
+  // the first 16 entries are panel k's pivot rows
+  double2 vals[kMoveMax][kPanelMax];
   int dst[kMoveMax], src[kMoveMax];
 };
 
@@ -388,27 +415,30 @@ __global__ void __launch_bounds__(768) k_gj_step(double2* __restrict__ a, int n,
     sh.src[i] = mv[1 + kMoveMax + i];
   }
   __syncthreads();
-  // old values: the pivot rows of the block's columns, and row i's source
-  if (i < NB * NB) {
-    const int q = i / NB, t = i % NB;
-    sh.R[q][t] = t < ntj ? m[sh.src[q] + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
+  // the only rows a thread reads besides its own: those the move list names
+  for (int x = i; x < nm * NB; x += blockDim.x) {
+    const int e = x / NB, t = x % NB;
+    sh.vals[e][t] = t < ntj ? m[sh.src[e] + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
   }
+  __syncthreads();  // after this each thread reads and writes its own row only
   const bool inpanel = i >= ck && i < ck + NB;
-  int srow = i;
-  for (int e = NB; e < nm; ++e)
-    if (sh.dst[e] == i) srow = sh.src[e];
+  int slot = -1;
+  if (!inpanel)
+    for (int e = NB; e < nm; ++e)
+      if (sh.dst[e] == i) slot = e;
   double2 r[NB];
 #pragma unroll
   for (int t = 0; t < NB; ++t)
-    r[t] = (mine && !inpanel && t < ntj) ? m[srow + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
-  __syncthreads();  // every read of the block's columns precedes every write
+    r[t] = (!mine || inpanel || t >= ntj) ? make_double2(0.0, 0.0)
+           : slot >= 0                    ? sh.vals[slot][t]
+                                          : m[i + (size_t)n * (cj + t)];
   if (mine) {
     for (int q = 0; q < NB; ++q) {
       const double2 wq = m[i + (size_t)n * (ck + q)];
       const zd w = mkz(wq.x, wq.y);
 #pragma unroll
       for (int t = 0; t < NB; ++t) {
-        const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.R[q][t].x, sh.R[q][t].y);
+        const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.vals[q][t].x, sh.vals[q][t].y);
         r[t] = make_double2(u.x, u.y);
       }
     }
