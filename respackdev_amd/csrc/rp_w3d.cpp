@@ -51,8 +51,15 @@ struct W3d {
   DevBuf<float2> d_ur, d_eps, d_A;
   DevBuf<long long> d_eps_off, d_rho_off, d_map_off;
   DevBuf<int4> d_qinfo;
-  DevBuf<int> d_minv, d_lg, d_ikq, d_shift, d_pw, d_plan;
-  DevBuf<double> d_qg, d_sq;
+  DevBuf<int> d_minv, d_pw;
+  DevBuf<double> d_sq;
+  // per-q geometry of the pair densities, every q at once (built on first use
+  // after rp_w3d_set_q): |q+G|, k+q index and umklapp per k, pruned-transform plan
+  bool geo_ready = false;
+  std::vector<long long> geo_qg_off, geo_plan_off;
+  std::vector<int> geo_g0, geo_q0, geo_nzz;
+  DevBuf<double> d_geo_qg;
+  DevBuf<int> d_geo_ikq, d_geo_shift, d_geo_plan;
   DevBuf<double2> d_rho, d_pf1, d_pf2, d_func, d_shell, d_funcw, d_cheadw, d_vmat, d_wmat;
 };
 
@@ -153,6 +160,40 @@ QGeom q_geometry(const W3d& w, int iq) {
       r.g0 = g;
   }
   return r;
+}
+
+// Every q's pair-density inputs in HBM, so that the loops over q only launch.
+void ensure_geometry(W3d& w, hipStream_t s) {
+  if (w.geo_ready) return;
+  std::vector<double> qg;
+  std::vector<int> ikq, shift, plan;
+  w.geo_qg_off.assign(w.ntq, 0);
+  w.geo_plan_off.assign(w.ntq, 0);
+  w.geo_g0.assign(w.ntq, -1);
+  w.geo_q0.assign(w.ntq, 0);
+  w.geo_nzz.assign(w.ntq, 0);
+  for (int iq = 0; iq < w.ntq; ++iq) {
+    const QGeom geo = q_geometry(w, iq);
+    const int np = w.ngpsi[iq];
+    w.geo_qg_off[iq] = (long long)qg.size();
+    qg.insert(qg.end(), geo.qg.begin(), geo.qg.end());
+    ikq.insert(ikq.end(), geo.ikq.begin(), geo.ikq.end());
+    shift.insert(shift.end(), geo.shift.begin(), geo.shift.end());
+    const std::vector<int> pl = pairdens_plan(
+        std::vector<int>(w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np), w.nw[0], w.nw[1],
+        w.nw[2]);
+    w.geo_plan_off[iq] = (long long)plan.size();
+    w.geo_nzz[iq] = pl[0];
+    plan.insert(plan.end(), pl.begin(), pl.end());
+    w.geo_g0[iq] = geo.g0;
+    w.geo_q0[iq] = geo.q0 ? 1 : 0;
+  }
+  w.d_geo_qg.upload(qg, s);
+  w.d_geo_ikq.upload(ikq, s);
+  w.d_geo_shift.upload(shift, s);
+  w.d_geo_plan.upload(plan, s);
+  RP_HIP(hipStreamSynchronize(s));  // the host vectors end here
+  w.geo_ready = true;
 }
 
 }  // namespace
@@ -361,6 +402,7 @@ int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps
     RP_CHECK(w.ntq == w.ntk, "NTQ should be NTK (calc_intW.F:484-487)");
     // G lists per full q (calc_intW.F:500-545)
     w.lg0.assign(w.ntq, {});
+    w.geo_ready = false;
     w.nge.assign(w.ntq, 0);
     w.ngpsi.assign(w.ntq, 0);
     for (int iq = 0; iq < w.ntq; ++iq) {
@@ -487,28 +529,15 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
     const size_t lda = roundup((size_t)maxpsi, 64);
     w.d_A.reserve((size_t)nwf * ntk * lda);
-    w.d_lg.reserve(3 * (size_t)maxpsi);
-    w.d_qg.reserve((size_t)maxpsi);
-    w.d_ikq.reserve(ntk);
-    w.d_shift.reserve(3 * (size_t)ntk);
+    ensure_geometry(w, s);
     for (int iq = 0; iq < w.ntq; ++iq) {
-      const QGeom geo = q_geometry(w, iq);
       const int np = w.ngpsi[iq];
-      const int g0 = geo.g0;
-      RP_HIP(hipMemcpyAsync(w.d_lg.p, w.lg0[iq].data(), sizeof(int) * 3 * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
-      const std::vector<int> plan = pairdens_plan(
-          std::vector<int>(w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np), w.nw[0], w.nw[1], w.nw[2]);
-      w.d_plan.reserve(plan.size());
-      RP_HIP(hipMemcpyAsync(w.d_plan.p, plan.data(), sizeof(int) * plan.size(), hipMemcpyHostToDevice, s));
-      launch_pair_densities_pruned(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_ikq.p, w.d_shift.p, w.nw[0],
-                                   w.nw[1], w.nw[2], w.d_plan.p, plan[0], np, lda, nwf * ntk, w.d_tw.p,
-                                   w.d_A.p, s);
-      launch_w_rho(w.d_A.p, lda, ntk, nwf, np, w.d_qg.p, g0, w.d_rho.p + w.rho_off[iq], s);
-      // the host vectors are reused by the next q: wait for the copies
-      RP_HIP(hipStreamSynchronize(s));
+      launch_pair_densities_pruned(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_geo_ikq.p + (size_t)iq * ntk,
+                                   w.d_geo_shift.p + (size_t)3 * iq * ntk, w.nw[0], w.nw[1], w.nw[2],
+                                   w.d_geo_plan.p + w.geo_plan_off[iq], w.geo_nzz[iq], np, lda,
+                                   nwf * ntk, w.d_tw.p, w.d_A.p, s);
+      launch_w_rho(w.d_A.p, lda, ntk, nwf, np, w.d_geo_qg.p + w.geo_qg_off[iq], w.geo_g0[iq],
+                   w.d_rho.p + w.rho_off[iq], s);
     }
     w.d_func.reserve((size_t)w.ntq * nwf * nwf);
     w.d_shell.reserve((size_t)w.ntq * nwf * nwf);
@@ -600,28 +629,16 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     const int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
     const size_t lda = roundup((size_t)maxpsi, 64);
     w.d_A.reserve((size_t)npair * ntk * lda);
-    w.d_lg.reserve(3 * (size_t)maxpsi);
-    w.d_qg.reserve((size_t)maxpsi);
-    w.d_ikq.reserve(ntk);
-    w.d_shift.reserve(3 * (size_t)ntk);
+    ensure_geometry(w, s);
     for (int iq = 0; iq < w.ntq; ++iq) {
-      const QGeom geo = q_geometry(w, iq);
       const int np = w.ngpsi[iq];
-      RP_HIP(hipMemcpyAsync(w.d_lg.p, w.lg0[iq].data(), sizeof(int) * 3 * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_qg.p, geo.qg.data(), sizeof(double) * np, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_ikq.p, geo.ikq.data(), sizeof(int) * ntk, hipMemcpyHostToDevice, s));
-      RP_HIP(hipMemcpyAsync(w.d_shift.p, geo.shift.data(), sizeof(int) * 3 * ntk, hipMemcpyHostToDevice, s));
-      const std::vector<int> plan = pairdens_plan(
-          std::vector<int>(w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np), w.nw[0], w.nw[1], w.nw[2]);
-      w.d_plan.reserve(plan.size());
-      RP_HIP(hipMemcpyAsync(w.d_plan.p, plan.data(), sizeof(int) * plan.size(), hipMemcpyHostToDevice, s));
-      launch_pair_densities_pruned(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_ikq.p, w.d_shift.p, w.nw[0],
-                                   w.nw[1], w.nw[2], w.d_plan.p, plan[0], np, lda, npair * ntk,
-                                   w.d_tw.p, w.d_A.p, s);
+      launch_pair_densities_pruned(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_geo_ikq.p + (size_t)iq * ntk,
+                                   w.d_geo_shift.p + (size_t)3 * iq * ntk, w.nw[0], w.nw[1], w.nw[2],
+                                   w.d_geo_plan.p + w.geo_plan_off[iq], w.geo_nzz[iq], np, lda,
+                                   npair * ntk, w.d_tw.p, w.d_A.p, s);
       for (int r = 0; r < nr; ++r)
-        launch_j_rho(w.d_A.p, lda, ntk, npair, np, w.d_qg.p, geo.g0, geo.q0 ? 1 : 0,
-                     d_pfk.p + (size_t)r * ntk, d_jrho.p + (size_t)r * tot + joff[iq], s);
-      RP_HIP(hipStreamSynchronize(s));  // the host vectors of this q are reused
+        launch_j_rho(w.d_A.p, lda, ntk, npair, np, w.d_geo_qg.p + w.geo_qg_off[iq], w.geo_g0[iq],
+                     w.geo_q0[iq], d_pfk.p + (size_t)r * ntk, d_jrho.p + (size_t)r * tot + joff[iq], s);
     }
     d_jfunc.reserve((size_t)w.ntq * npair);
     d_jshell.reserve((size_t)w.ntq * npair);

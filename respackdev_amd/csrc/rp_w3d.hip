@@ -18,20 +18,36 @@
 namespace rp {
 
 // rho[q][w][G] (complex FP64) from the pair densities A[(w*ntk + k)][lda]
-// (complex64): sum over k in order, / |q+G|, / NTK; G = 0 at q = 0 is zero
-// (calc_intW.F:681-767).  grid: (ceil(ngpsi/256), nwf).
-__global__ void __launch_bounds__(256) k_w_rho(const float2* __restrict__ A, size_t lda, int ntk,
+// (complex64): sum over k, / |q+G|, / NTK; G = 0 at q = 0 is zero
+// (calc_intW.F:681-767).  The k sum is split over kRhoSplit chunks (threadIdx.y),
+// whose partial sums are added in chunk order.  grid: (ceil(ngpsi/64), nwf),
+// block (64, kRhoSplit).
+constexpr int kRhoSplit = 8;
+__global__ void __launch_bounds__(512) k_w_rho(const float2* __restrict__ A, size_t lda, int ntk,
                                                int ngpsi, const double* __restrict__ qg, int g0,
                                                double2* __restrict__ rho) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int w = blockIdx.y;
-  if (g >= ngpsi) return;
+  __shared__ double2 part[kRhoSplit][64];
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  const int w = blockIdx.y, c = threadIdx.y;
+  const int per = (ntk + kRhoSplit - 1) / kRhoSplit;
+  const int k0 = c * per, k1 = min(ntk, k0 + per);
   double sr = 0.0, si = 0.0;
-  const float2* a = A + (size_t)w * ntk * lda + g;
-  for (int k = 0; k < ntk; ++k) {
-    const float2 v = a[(size_t)k * lda];
-    sr += (double)v.x;
-    si += (double)v.y;
+  if (g < ngpsi) {
+    const float2* a = A + (size_t)w * ntk * lda + g;
+    for (int k = k0; k < k1; ++k) {
+      const float2 v = a[(size_t)k * lda];
+      sr += (double)v.x;
+      si += (double)v.y;
+    }
+  }
+  part[c][threadIdx.x] = make_double2(sr, si);
+  __syncthreads();
+  if (c != 0 || g >= ngpsi) return;
+  sr = si = 0.0;
+#pragma unroll
+  for (int j = 0; j < kRhoSplit; ++j) {
+    sr += part[j][threadIdx.x].x;
+    si += part[j][threadIdx.x].y;
   }
   double2 r;
   if (g == g0) {
@@ -90,42 +106,49 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
 // kDiag = false (calc_w3d): the vectors are the nv <= 8 Wannier functions and
 // every (i,j) is formed, out [q][w][j][i].  kDiag = true (calc_j3d): the
-// vectors are the (iw,jw) pair densities, 8 per workgroup (blockIdx.z), and
+// vectors are the (iw,jw) pair densities, VB per workgroup (blockIdx.z), and
 // only (p,p) is formed, out [q][w][p].
-// grid: (nen, ntq, groups), 256 threads.
-constexpr int kMaxWf = 8;
-constexpr int kTB = 128;  // columns of E staged per pass
-template <bool kDiag>
-__global__ void __launch_bounds__(256) k_w_contract(
+// Every element of E is read once per workgroup, and a workgroup takes all
+// the vectors of its (q, w) (up to VB), so E is read once per (q, w): 8 B of
+// HBM per 8 VB flops.  512 threads, rows a and a + 512 of E per thread (NG_eps
+// <= 1024); the column loop keeps several loads of E in flight per thread.
+// grid: (nen, ntq, groups).
+constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
+constexpr int kJVec = 25;      // pair densities per calc_j3d workgroup
+constexpr int kCT = 512;       // threads of the contraction
+constexpr int kTB = 32;        // columns of E staged per pass
+template <bool kDiag, int VB>
+__global__ void __launch_bounds__(512) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
     const int4* __restrict__ qinfo, const int* __restrict__ minv,
     const double2* __restrict__ pf1, const double2* __restrict__ pf2,
     const long long* __restrict__ map_off, const double2* __restrict__ shell, int nv, int nen,
     double2* __restrict__ funcw) {
+  constexpr int kPairs = kDiag ? VB : VB * VB;
+  constexpr int kWaves = kCT / 64;
   const int w = blockIdx.x, q = blockIdx.y;
   const int4 qi = qinfo[q];
   const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
-  const int c0 = kDiag ? (int)blockIdx.z * kMaxWf : 0;
-  const int nwf = kDiag ? min(kMaxWf, nv - c0) : nv;  // vectors of this workgroup
+  const int c0 = kDiag ? (int)blockIdx.z * VB : 0;
+  const int nvb = kDiag ? min(VB, nv - c0) : nv;  // vectors of this workgroup
   const double2* r = rho + rho_off[q] + (size_t)c0 * ngpsi;
   const int* mi = minv + map_off[q];
   const double2* p1 = pf1 + map_off[q];
   const double2* p2 = pf2 + map_off[q];
-  __shared__ double2 X[kMaxWf][kTB];
-  __shared__ double red[256];
-  // rows a = threadIdx.x + 256 t
-  constexpr int kRows = 4;  // nge <= 1024
-  double yr[kRows][kMaxWf], yi[kRows][kMaxWf];
+  __shared__ double2 X[VB][kTB];
+  __shared__ double red[kPairs][kWaves][2];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int a0 = t, a1 = t + kCT;
+  const bool v0 = a0 < nge, v1 = a1 < nge;
+  double y0r[VB], y0i[VB], y1r[VB], y1i[VB];
 #pragma unroll
-  for (int t = 0; t < kRows; ++t)
-#pragma unroll
-    for (int c = 0; c < kMaxWf; ++c) yr[t][c] = yi[t][c] = 0.0;
+  for (int c = 0; c < VB; ++c) y0r[c] = y0i[c] = y1r[c] = y1i[c] = 0.0;
   for (int b0 = 0; b0 < nge; b0 += kTB) {
     const int nb = min(kTB, nge - b0);
     __syncthreads();
-    for (int e = threadIdx.x; e < nwf * kTB; e += blockDim.x) {
+    for (int e = t; e < nvb * kTB; e += kCT) {
       const int c = e / kTB, bb = e % kTB;
       double2 v = make_double2(0.0, 0.0);
       if (bb < nb) {
@@ -139,100 +162,119 @@ __global__ void __launch_bounds__(256) k_w_contract(
       X[c][bb] = v;
     }
     __syncthreads();
+    const float2* col = E + (size_t)b0 * nge;
+#pragma unroll 4
+    for (int bb = 0; bb < nb; ++bb) {
+      const float2 e0 = v0 ? col[(size_t)bb * nge + a0] : make_float2(0.f, 0.f);
+      const float2 e1 = v1 ? col[(size_t)bb * nge + a1] : make_float2(0.f, 0.f);
+      const double e0r = e0.x, e0i = e0.y, e1r = e1.x, e1i = e1.y;
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int a = threadIdx.x + 256 * t;
-      if (a >= nge) continue;
-      for (int bb = 0; bb < nb; ++bb) {
-        const float2 ef = E[(size_t)(b0 + bb) * nge + a];
-        const double er = (double)ef.x, ei = (double)ef.y;
-#pragma unroll
-        for (int c = 0; c < kMaxWf; ++c) {
-          if (c < nwf) {
-            const double2 x = X[c][bb];
-            yr[t][c] += er * x.x - ei * x.y;
-            yi[t][c] += er * x.y + ei * x.x;
-          }
+      for (int c = 0; c < VB; ++c) {
+        if (c < nvb) {
+          const double2 x = X[c][bb];
+          y0r[c] += e0r * x.x - e0i * x.y;
+          y0i[c] += e0r * x.y + e0i * x.x;
+          y1r[c] += e1r * x.x - e1i * x.y;
+          y1i[c] += e1r * x.y + e1i * x.x;
         }
       }
     }
   }
-  // funcw_ij = sum_a Z(a) Y(a, .)
-  const int npairs = kDiag ? nwf : nwf * nwf;
-  for (int ij = 0; ij < npairs; ++ij) {
-    const int i = kDiag ? ij : ij % nwf, j = kDiag ? ij : ij / nwf;
-    double sr = 0.0, si = 0.0;
+  // funcw_ij = sum_a Z(a) Y(a, .): per thread over its two rows, then over
+  // the wave (shuffles) and the waves (LDS)
+  const int g0 = v0 ? mi[a0] : 0, g1 = v1 ? mi[a1] : 0;
 #pragma unroll
-    for (int t = 0; t < kRows; ++t) {
-      const int a = threadIdx.x + 256 * t;
-      if (a >= nge) continue;
-      const int g = mi[a];
-      double zr, zi, ycr = 0.0, yci = 0.0;
-      if (trs == 1) {
-        const double2 rv = r[(size_t)i * ngpsi + g], p = p1[g];
-        zr = rv.x * p.x - rv.y * p.y;
-        zi = rv.x * p.y + rv.y * p.x;
-#pragma unroll
-        for (int c = 0; c < kMaxWf; ++c)
-          if (c == j) { ycr = yr[t][c]; yci = yi[t][c]; }
-      } else {
-        const double2 rv = r[(size_t)j * ngpsi + g], p = p2[g];
-        zr = rv.x * p.x + rv.y * p.y;   // conj(rho) pf2
-        zi = rv.x * p.y - rv.y * p.x;
-#pragma unroll
-        for (int c = 0; c < kMaxWf; ++c)
-          if (c == i) { ycr = yr[t][c]; yci = yi[t][c]; }
-      }
-      sr += zr * ycr - zi * yci;
-      si += zr * yci + zi * ycr;
+  for (int zi = 0; zi < VB; ++zi) {
+    if (zi >= nvb) continue;
+    // Z of vector zi at both rows
+    double z0r = 0.0, z0i = 0.0, z1r = 0.0, z1i = 0.0;
+    if (v0) {
+      const double2 rv = r[(size_t)zi * ngpsi + g0];
+      if (trs == 1) { const double2 p = p1[g0]; z0r = rv.x * p.x - rv.y * p.y; z0i = rv.x * p.y + rv.y * p.x; }
+      else { const double2 p = p2[g0]; z0r = rv.x * p.x + rv.y * p.y; z0i = rv.x * p.y - rv.y * p.x; }
     }
-    for (int part = 0; part < 2; ++part) {
-      red[threadIdx.x] = part == 0 ? sr : si;
-      __syncthreads();
-      for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-      }
-      if (threadIdx.x == 0) {
-        double2* o;
-        double2 sh;
-        if (kDiag) {
-          sh = shell[(size_t)q * nv + c0 + i];
-          o = funcw + ((size_t)q * nen + w) * nv + c0 + i;
-        } else {
-          sh = shell[((size_t)q * nwf + j) * nwf + i];
-          o = funcw + (((size_t)q * nen + w) * nwf + j) * nwf + i;
-        }
-        if (part == 0) o->x = red[0] + sh.x;
-        else o->y = red[0] + sh.y;
-      }
-      __syncthreads();
+    if (v1) {
+      const double2 rv = r[(size_t)zi * ngpsi + g1];
+      if (trs == 1) { const double2 p = p1[g1]; z1r = rv.x * p.x - rv.y * p.y; z1i = rv.x * p.y + rv.y * p.x; }
+      else { const double2 p = p2[g1]; z1r = rv.x * p.x + rv.y * p.y; z1i = rv.x * p.y - rv.y * p.x; }
     }
+#pragma unroll
+    for (int yc = 0; yc < VB; ++yc) {
+      if (kDiag ? yc != zi : yc >= nvb) continue;
+      double sr = (z0r * y0r[yc] - z0i * y0i[yc]) + (z1r * y1r[yc] - z1i * y1i[yc]);
+      double si = (z0r * y0i[yc] + z0i * y0r[yc]) + (z1r * y1i[yc] + z1i * y1r[yc]);
+      for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off);
+        si += __shfl_xor(si, off);
+      }
+      // pair slot: J (p,p) at p; W: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
+      const int slot = kDiag ? zi : (trs == 1 ? zi + VB * yc : yc + VB * zi);
+      if (lane == 0) {
+        red[slot][wave][0] = sr;
+        red[slot][wave][1] = si;
+      }
+    }
+  }
+  __syncthreads();
+  const int npairs = kDiag ? nvb : nvb * nvb;
+  for (int e = t; e < 2 * npairs; e += kCT) {
+    const int pr = e >> 1, part = e & 1;
+    const int i = kDiag ? pr : pr % nvb, j = kDiag ? pr : pr / nvb;
+    const int slot = kDiag ? i : i + VB * j;
+    double sum = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < kWaves; ++wv) sum += red[slot][wv][part];
+    double2* o;
+    double2 sh;
+    if (kDiag) {
+      sh = shell[(size_t)q * nv + c0 + i];
+      o = funcw + ((size_t)q * nen + w) * nv + c0 + i;
+    } else {
+      sh = shell[((size_t)q * nv + j) * nv + i];
+      o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
+    }
+    if (part == 0) o->x = sum + sh.x;
+    else o->y = sum + sh.y;
   }
 }
 
 // calc_j3d rho (calc_intJ.F:634-790): rho_p(G) for the pair p = iw + nwf*jw,
 // sum_k M_(jw at k, iw at k+q)(G) e^{-i 2pi k.R} from A[(p*ntk + k)][lda]: at
 // q != 0 each term is divided by |q+G|, at q = 0 the sum is (G = 0 dropped);
-// then / NTK.  grid: (ceil(ngpsi/256), npair).
-__global__ void __launch_bounds__(256) k_j_rho(const float2* __restrict__ A, size_t lda, int ntk,
+// then / NTK.  The k sum is split as in k_w_rho.  grid: (ceil(ngpsi/64),
+// npair), block (64, kRhoSplit).
+__global__ void __launch_bounds__(512) k_j_rho(const float2* __restrict__ A, size_t lda, int ntk,
                                                int ngpsi, const double* __restrict__ qg, int g0,
                                                int q0, const double2* __restrict__ pfk,
                                                double2* __restrict__ rho) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = blockIdx.y;
-  if (g >= ngpsi) return;
+  __shared__ double2 part[kRhoSplit][64];
+  const int g = blockIdx.x * 64 + threadIdx.x;
+  const int p = blockIdx.y, c = threadIdx.y;
+  const int per = (ntk + kRhoSplit - 1) / kRhoSplit;
+  const int k0 = c * per, k1 = min(ntk, k0 + per);
   double sr = 0.0, si = 0.0;
-  const float2* a = A + (size_t)p * ntk * lda + g;
-  const double len = qg[g];
-  for (int k = 0; k < ntk; ++k) {
-    const float2 v = a[(size_t)k * lda];
-    const double2 f = pfk[k];
-    double tr = (double)v.x * f.x - (double)v.y * f.y;
-    double ti = (double)v.x * f.y + (double)v.y * f.x;
-    if (!q0) { tr = tr / len; ti = ti / len; }
-    sr += tr;
-    si += ti;
+  double len = 1.0;
+  if (g < ngpsi) {
+    const float2* a = A + (size_t)p * ntk * lda + g;
+    len = qg[g];
+    for (int k = k0; k < k1; ++k) {
+      const float2 v = a[(size_t)k * lda];
+      const double2 f = pfk[k];
+      double tr = (double)v.x * f.x - (double)v.y * f.y;
+      double ti = (double)v.x * f.y + (double)v.y * f.x;
+      if (!q0) { tr = tr / len; ti = ti / len; }
+      sr += tr;
+      si += ti;
+    }
+  }
+  part[c][threadIdx.x] = make_double2(sr, si);
+  __syncthreads();
+  if (c != 0 || g >= ngpsi) return;
+  sr = si = 0.0;
+#pragma unroll
+  for (int j = 0; j < kRhoSplit; ++j) {
+    sr += part[j][threadIdx.x].x;
+    si += part[j][threadIdx.x].y;
   }
   double2 r;
   if (q0 && g == g0) r = make_double2(0.0, 0.0);
@@ -353,12 +395,12 @@ __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ fu
 }
 
 int w3d_max_wannier() { return kMaxWf; }
-int w3d_max_eps_g() { return 4 * 256; }
+int w3d_max_eps_g() { return 2 * kCT; }
 
 void launch_w_rho(const float2* A, size_t lda, int ntk, int nwf, int ngpsi, const double* qg,
                   int g0, double2* rho, hipStream_t s) {
-  hipLaunchKernelGGL(k_w_rho, dim3((ngpsi + 255) / 256, nwf), dim3(256), 0, s, A, lda, ntk, ngpsi,
-                     qg, g0, rho);
+  hipLaunchKernelGGL(k_w_rho, dim3((ngpsi + 63) / 64, nwf), dim3(64, kRhoSplit), 0, s, A, lda, ntk,
+                     ngpsi, qg, g0, rho);
   RP_HIP(hipGetLastError());
 }
 
@@ -374,8 +416,8 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* shell, int ntq, int nwf, int nen, double2* funcw,
                        hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
-  hipLaunchKernelGGL(k_w_contract<false>, dim3(nen, ntq), dim3(256), 0, s, eps, eps_off, rho,
-                     rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
+  hipLaunchKernelGGL((k_w_contract<false, kMaxWf>), dim3(nen, ntq), dim3(kCT), 0, s, eps, eps_off,
+                     rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
   RP_HIP(hipGetLastError());
 }
 
@@ -384,16 +426,17 @@ void launch_j_contract(const float2* eps, const long long* eps_off, const double
                        const double2* pf1, const double2* pf2, const long long* map_off,
                        const double2* shell, int ntq, int npair, int nen, double2* funcw,
                        hipStream_t s) {
-  const int groups = (npair + kMaxWf - 1) / kMaxWf;
-  hipLaunchKernelGGL(k_w_contract<true>, dim3(nen, ntq, groups), dim3(256), 0, s, eps, eps_off, rho,
-                     rho_off, qinfo, minv, pf1, pf2, map_off, shell, npair, nen, funcw);
+  const int groups = (npair + kJVec - 1) / kJVec;
+  hipLaunchKernelGGL((k_w_contract<true, kJVec>), dim3(nen, ntq, groups), dim3(kCT), 0, s, eps,
+                     eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, npair, nen,
+                     funcw);
   RP_HIP(hipGetLastError());
 }
 
 void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
                   int g0, int q0, const double2* pfk, double2* rho, hipStream_t s) {
-  hipLaunchKernelGGL(k_j_rho, dim3((ngpsi + 255) / 256, npair), dim3(256), 0, s, A, lda, ntk,
-                     ngpsi, qg, g0, q0, pfk, rho);
+  hipLaunchKernelGGL(k_j_rho, dim3((ngpsi + 63) / 64, npair), dim3(64, kRhoSplit), 0, s, A, lda,
+                     ntk, ngpsi, qg, g0, q0, pfk, rho);
   RP_HIP(hipGetLastError());
 }
 
