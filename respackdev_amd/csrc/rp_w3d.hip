@@ -111,13 +111,14 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // Every element of E is read once per workgroup, and a workgroup takes all
 // the vectors of its (q, w) (up to VB), so E is read once per (q, w): 8 B of
 // HBM per 8 VB flops.  512 threads, rows a and a + 512 of E per thread (NG_eps
-// <= 1024); the column loop keeps several loads of E in flight per thread.
+// <= 1024); the loads of the next PD columns of E are in flight during a
+// column's arithmetic.
 // grid: (nen, ntq, groups).
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
 constexpr int kJVec = 25;      // pair densities per calc_j3d workgroup
 constexpr int kCT = 512;       // threads of the contraction
 constexpr int kTB = 32;        // columns of E staged per pass
-template <bool kDiag, int VB>
+template <bool kDiag, int VB, int PD = (VB > 12 ? 3 : 8)>
 __global__ void __launch_bounds__(512) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
@@ -148,10 +149,10 @@ __global__ void __launch_bounds__(512) k_w_contract(
   for (int b0 = 0; b0 < nge; b0 += kTB) {
     const int nb = min(kTB, nge - b0);
     __syncthreads();
-    for (int e = t; e < nvb * kTB; e += kCT) {
+    for (int e = t; e < VB * kTB; e += kCT) {
       const int c = e / kTB, bb = e % kTB;
-      double2 v = make_double2(0.0, 0.0);
-      if (bb < nb) {
+      double2 v = make_double2(0.0, 0.0);  // vectors beyond nvb stay zero
+      if (bb < nb && c < nvb) {
         const int b = b0 + bb, g = mi[b];
         const double2 rv = r[(size_t)c * ngpsi + g];
         const double2 p = trs == 1 ? p2[g] : p1[g];
@@ -162,15 +163,31 @@ __global__ void __launch_bounds__(512) k_w_contract(
       X[c][bb] = v;
     }
     __syncthreads();
-    const float2* col = E + (size_t)b0 * nge;
-#pragma unroll 4
-    for (int bb = 0; bb < nb; ++bb) {
-      const float2 e0 = v0 ? col[(size_t)bb * nge + a0] : make_float2(0.f, 0.f);
-      const float2 e1 = v1 ? col[(size_t)bb * nge + a1] : make_float2(0.f, 0.f);
-      const double e0r = e0.x, e0i = e0.y, e1r = e1.x, e1i = e1.y;
+    // rows past nge read row 0 (unconditional loads, several in flight); their
+    // sums are never used (Z is zero there)
+    const float2* c0p = E + (size_t)b0 * nge + (v0 ? a0 : 0);
+    const float2* c1p = E + (size_t)b0 * nge + (v1 ? a1 : 0);
+    // the loads of the next PD columns are in flight during a column's arithmetic
+    float2 q0[PD], q1[PD];
 #pragma unroll
-      for (int c = 0; c < VB; ++c) {
-        if (c < nvb) {
+    for (int j = 0; j < PD; ++j) {
+      const int cj = j < nb ? j : nb - 1;
+      q0[j] = c0p[(size_t)cj * nge];
+      q1[j] = c1p[(size_t)cj * nge];
+    }
+#pragma unroll 1
+    for (int bq = 0; bq < nb; bq += PD) {
+#pragma unroll
+      for (int j = 0; j < PD; ++j) {
+        const int bb = bq + j;
+        if (bb >= nb) break;  // uniform
+        const float2 e0 = q0[j], e1 = q1[j];
+        const int nx = bb + PD < nb ? bb + PD : nb - 1;
+        q0[j] = c0p[(size_t)nx * nge];
+        q1[j] = c1p[(size_t)nx * nge];
+        const double e0r = e0.x, e0i = e0.y, e1r = e1.x, e1i = e1.y;
+#pragma unroll
+        for (int c = 0; c < VB; ++c) {
           const double2 x = X[c][bb];
           y0r[c] += e0r * x.x - e0i * x.y;
           y0i[c] += e0r * x.y + e0i * x.x;
@@ -416,8 +433,17 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* shell, int ntq, int nwf, int nen, double2* funcw,
                        hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
-  hipLaunchKernelGGL((k_w_contract<false, kMaxWf>), dim3(nen, ntq), dim3(kCT), 0, s, eps, eps_off,
-                     rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, funcw);
+  // as many accumulators as Wannier functions
+  switch (nwf) {
+#define RP_WC(NV)                                                                              \
+  case NV:                                                                                     \
+    hipLaunchKernelGGL((k_w_contract<false, NV>), dim3(nen, ntq), dim3(kCT), 0, s, eps,        \
+                       eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, \
+                       funcw);                                                                 \
+    break;
+    RP_WC(1) RP_WC(2) RP_WC(3) RP_WC(4) RP_WC(5) RP_WC(6) RP_WC(7) RP_WC(8)
+#undef RP_WC
+  }
   RP_HIP(hipGetLastError());
 }
 
