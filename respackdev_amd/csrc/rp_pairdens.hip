@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "rp_common.h"
+#include "rp_fft.h"
 #include "rp_kernels.h"
 
 namespace rp {
@@ -165,8 +166,7 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
 
 // ----------------------------------------------- pruned forward transform
 // Only the chi sphere's points of the forward transform are read, so each axis
-// computes only the outputs that a later axis or the final read needs (every
-// output that is computed takes exactly the operations of the full transform):
+// stores only the outputs that a later axis or the final read needs:
 //   z axis: straight from the band grids in registers (no LDS pass for the
 //           product), only the z planes that hold sphere points, stored
 //           compacted: g[x + nx (y + ny iz)], iz the plane's rank;
@@ -218,10 +218,24 @@ std::vector<int> pairdens_plan(const std::vector<int>& lg, int nx, int ny, int n
 }
 
 // Outputs k of mask of one line v (twiddle rows of tw, wave-uniform), to
-// dst[k * stride] (z axis: the compacted plane of k).
+// dst[k * stride] (z axis: the compacted plane of k).  Lines of up to 16
+// points take the register FFT (all N outputs at about a quarter of the
+// multiply-adds of the masked direct sum), longer ones the direct sum of the
+// masked outputs (the FFT's registers would spill there).
 template <int N, bool ZPLANES>
 __device__ __forceinline__ void masked_line(const double2 (&v)[N], unsigned mask, const double2* tw,
                                             double2* dst, int stride) {
+  if constexpr (N <= 16) {
+    double2 y[N];
+    fft_reg<N, 1, N>(v, y, tw + N);  // row 1 of the table: w^j
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      if (!((mask >> k) & 1u)) continue;
+      const int slot = ZPLANES ? __builtin_popcount(mask & ((1u << k) - 1u)) : k;
+      dst[slot * stride] = y[k];
+    }
+    return;
+  }
 #pragma unroll 1
   for (int k = 0; k < N; ++k) {
     if (!((mask >> k) & 1u)) continue;
