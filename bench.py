@@ -117,6 +117,8 @@ def main() -> int:
     ap.add_argument("--cpu-pairs", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="run a single precision (profiling)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the contraction's launch candidates in an untimed pass first")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run an elementwise copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -155,7 +157,13 @@ def main() -> int:
 
     precs = [args.only] if args.only else ["fp64", args.mixed]
     results = {}
+    if args.autotune:
+        c.autotune(True)
     for prec in precs:
+        if args.autotune:
+            # the tuning pass (every candidate launch timed), outside the timed steps
+            c.run_q(wl["q"], prec, want_chi=False)
+            log(f"{prec} tuned: {c.tune_table()}")
         for i in range(args.warmup):
             r = c.run_q(wl["q"], prec, want_chi=False)
             log(f"{prec} warmup {i}: {r['timings']['device_total']:.3f} s")
