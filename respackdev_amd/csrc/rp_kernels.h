@@ -87,11 +87,13 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* pf1, const double2* pf2, const long long* map_off,
                        const double2* shell, int ntq, int nwf, int nen, double2* funcw,
                        hipStream_t s);
+// xg: scratch of j_contract_scratch(sum over q of NG_eps, npair) complex values
 void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int npair, int nen, double2* funcw,
-                       hipStream_t s);
+                       double2* xg, const double2* shell, int ntq, int npair, int nen,
+                       double2* funcw, hipStream_t s);
+size_t j_contract_scratch(size_t sum_nge, int npair);
 void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
                   int g0, int q0, const double2* pfk, double2* rho, hipStream_t s);
 void launch_j_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,

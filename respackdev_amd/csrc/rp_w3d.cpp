@@ -624,7 +624,7 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     for (int iq = 0; iq < w.ntq; ++iq) { joff[iq] = tot; tot += (long long)npair * w.ngpsi[iq]; }
     DevBuf<long long> d_joff;
     d_joff.upload(joff, s);
-    DevBuf<double2> d_jrho, d_jfunc, d_jshell, d_jfuncw, d_x, d_j, d_cw;
+    DevBuf<double2> d_jrho, d_jfunc, d_jshell, d_jfuncw, d_x, d_j, d_cw, d_xg;
     d_jrho.reserve((size_t)tot * nr);
     const int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
     const size_t lda = roundup((size_t)maxpsi, 64);
@@ -646,11 +646,13 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     d_x.reserve((size_t)nr * npair);
     d_j.reserve((size_t)nr * nen * npair);
     d_cw.upload(w.cheadw, s);
+    d_xg.reserve(j_contract_scratch(w.minv.size(), npair));
     for (int r = 0; r < nr; ++r) {
       const double2* rho = d_jrho.p + (size_t)r * tot;
       launch_j_func(rho, d_joff.p, w.d_qinfo.p, w.ntq, npair, d_jfunc.p, d_jshell.p, s);
       launch_j_contract(w.d_eps.p, w.d_eps_off.p, rho, d_joff.p, w.d_qinfo.p, w.d_minv.p, w.d_pf1.p,
-                        w.d_pf2.p, w.d_map_off.p, d_jshell.p, w.ntq, npair, nen, d_jfuncw.p, s);
+                        w.d_pf2.p, w.d_map_off.p, d_xg.p, d_jshell.p, w.ntq, npair, nen, d_jfuncw.p,
+                        s);
       const int r0 = rvec[3 * r] == 0 && rvec[3 * r + 1] == 0 && rvec[3 * r + 2] == 0;
       launch_j_sum(d_jfunc.p, d_jfuncw.p, w.ntq, nwf, nen, r0, w.volume, w.chead, d_cw.p,
                    d_x.p + (size_t)r * npair, d_j.p + (size_t)r * npair, (size_t)nr * npair, s);

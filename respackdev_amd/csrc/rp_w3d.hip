@@ -115,7 +115,6 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // column's arithmetic.
 // grid: (nen, ntq, groups).
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
-constexpr int kJVec = 25;      // pair densities per calc_j3d workgroup
 constexpr int kCT = 512;       // threads of the contraction
 constexpr int kTB = 32;        // columns of E staged per pass
 template <bool kDiag, int VB, int PD = (VB > 12 ? 3 : 8)>
@@ -250,6 +249,164 @@ __global__ void __launch_bounds__(512) k_w_contract(
       sh = shell[((size_t)q * nv + j) * nv + i];
       o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
     }
+    if (part == 0) o->x = sum + sh.x;
+    else o->y = sum + sh.y;
+  }
+}
+
+// calc_j3d's contraction on the FP64 matrix cores: the same funcw_pp as
+// k_w_contract<true>, Y = E X as real products v_mfma_f64_16x16x4 (Re Y =
+// Er Xr - Ei Xi, Im Y = Er Xi + Ei Xr) over 16-row blocks of E and 16-column
+// blocks of up to 32 pair vectors (blockIdx.z picks the 32).  X comes from
+// k_j_xvec, [b][32 groups] in HBM (zero past the last pair).  A wave takes up
+// to kJA row blocks per pass (wave + 8 j), so E is read once per (q, w).  Lane
+// l supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds
+// D(l / 16 + 4 v, l % 16), v < 4 (checked on the GPU with row- and column-numbered
+// operands).  grid: (nen, ntq, groups), 512 threads.
+typedef double jd4 __attribute__((ext_vector_type(4)));
+constexpr int kJA = 4;
+__global__ void __launch_bounds__(256) k_j_xvec(const double2* __restrict__ rho,
+                                                const long long* __restrict__ rho_off,
+                                                const int4* __restrict__ qinfo,
+                                                const int* __restrict__ minv,
+                                                const double2* __restrict__ pf1,
+                                                const double2* __restrict__ pf2,
+                                                const long long* __restrict__ map_off, int nv,
+                                                int g32, double2* __restrict__ xg) {
+  const int q = blockIdx.x;
+  const int4 qi = qinfo[q];
+  const int nge = qi.x, ngpsi = qi.y, trs = qi.w;
+  const double2* r = rho + rho_off[q];
+  const int* mi = minv + map_off[q];
+  const double2* p1 = pf1 + map_off[q];
+  const double2* p2 = pf2 + map_off[q];
+  double2* out = xg + map_off[q] * g32;
+  for (int e = threadIdx.x; e < nge * g32; e += blockDim.x) {
+    const int b = e / g32, c = e % g32;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < nv) {
+      const int g = mi[b];
+      const double2 rv = r[(size_t)c * ngpsi + g];
+      const double2 p = trs == 1 ? p2[g] : p1[g];
+      // trs = +1: conj(rho) pf2; trs = -1: rho pf1
+      const double xr = rv.x, xi = trs == 1 ? -rv.y : rv.y;
+      v = make_double2(xr * p.x - xi * p.y, xr * p.y + xi * p.x);
+    }
+    out[e] = v;
+  }
+}
+
+__global__ void __launch_bounds__(512) k_j_contract(
+    const float2* __restrict__ eps, const long long* __restrict__ eps_off,
+    const double2* __restrict__ rho, const long long* __restrict__ rho_off,
+    const int4* __restrict__ qinfo, const int* __restrict__ minv,
+    const double2* __restrict__ pf1, const double2* __restrict__ pf2,
+    const long long* __restrict__ map_off, const double2* __restrict__ xg, int g32,
+    const double2* __restrict__ shell, int nv, int nen, double2* __restrict__ funcw) {
+  const int w = blockIdx.x, q = blockIdx.y, grp = blockIdx.z;
+  const int4 qi = qinfo[q];
+  const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
+  const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
+  const int c0 = grp * 32, nvb = min(32, nv - c0);
+  const double2* Xq = xg + map_off[q] * g32 + c0;  // (b, c) at Xq[b * g32 + c]
+  const double2* r = rho + rho_off[q] + (size_t)c0 * ngpsi;
+  const int* mi = minv + map_off[q];
+  const double2* p1 = pf1 + map_off[q];
+  const double2* p2 = pf2 + map_off[q];
+  __shared__ double red[32][8][2];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int nab = (nge + 15) / 16;
+  double sre[2] = {0.0, 0.0}, sim[2] = {0.0, 0.0};  // this lane's sums for c = 16 cb + li
+  for (int a0 = 0; a0 < nab; a0 += 8 * kJA) {
+    jd4 accr[kJA][2], acci[kJA][2];
+    int arow[kJA];
+#pragma unroll
+    for (int j = 0; j < kJA; ++j) {
+      // rows past nge read row nge - 1; their sums are never used
+      const int a = (a0 + wave + 8 * j) * 16 + li;
+      arow[j] = a < nge ? a : nge - 1;
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) accr[j][cb] = acci[j][cb] = jd4{0.0, 0.0, 0.0, 0.0};
+    }
+#pragma unroll 2
+    for (int b0 = 0; b0 < nge; b0 += 4) {
+      const int b = b0 + lk;
+      const bool bv = b < nge;  // columns past nge contribute zero
+      const int bc = bv ? b : nge - 1;
+      double xr[2], xi[2], nxi[2];
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const double2 xv = Xq[(size_t)bc * g32 + cb * 16 + li];
+        xr[cb] = bv ? xv.x : 0.0;
+        xi[cb] = bv ? xv.y : 0.0;
+        nxi[cb] = -xi[cb];
+      }
+#pragma unroll
+      for (int j = 0; j < kJA; ++j) {
+        const float2 ev = E[(size_t)bc * nge + arow[j]];
+        const double er = ev.x, ei = ev.y;
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          accr[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr[cb], accr[j][cb], 0, 0, 0);
+          accr[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi[cb], accr[j][cb], 0, 0, 0);
+          acci[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xi[cb], acci[j][cb], 0, 0, 0);
+          acci[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, xr[cb], acci[j][cb], 0, 0, 0);
+        }
+      }
+    }
+    // Z_c(a) Y(a, c) for this lane's rows a = 16 (row block) + lk + 4 v and columns c
+#pragma unroll
+    for (int j = 0; j < kJA; ++j) {
+      const int ab = a0 + wave + 8 * j;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int a = ab * 16 + lk + 4 * v;
+        if (ab >= nab || a >= nge) continue;
+        const int g = mi[a];
+        const double2 pp = trs == 1 ? p1[g] : p2[g];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) {
+          const int c = cb * 16 + li;
+          if (c >= nvb) continue;
+          const double2 rv = r[(size_t)c * ngpsi + g];
+          double zr, zi;
+          if (trs == 1) {  // Z = rho pf1
+            zr = rv.x * pp.x - rv.y * pp.y;
+            zi = rv.x * pp.y + rv.y * pp.x;
+          } else {         // Z = conj(rho) pf2
+            zr = rv.x * pp.x + rv.y * pp.y;
+            zi = rv.x * pp.y - rv.y * pp.x;
+          }
+          const double yr = accr[j][cb][v], yi = acci[j][cb][v];
+          sre[cb] += zr * yr - zi * yi;
+          sim[cb] += zr * yi + zi * yr;
+        }
+      }
+    }
+  }
+  // lanes of one column (li) hold their rows' sums: over lk, then the waves
+#pragma unroll
+  for (int cb = 0; cb < 2; ++cb)
+    for (int off = 16; off < 64; off <<= 1) {
+      sre[cb] += __shfl_xor(sre[cb], off);
+      sim[cb] += __shfl_xor(sim[cb], off);
+    }
+  if (lk == 0) {
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) {
+      red[cb * 16 + li][wave][0] = sre[cb];
+      red[cb * 16 + li][wave][1] = sim[cb];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < 2 * nvb; e += blockDim.x) {
+    const int p = e >> 1, part = e & 1;
+    double sum = 0.0;
+#pragma unroll
+    for (int wv = 0; wv < 8; ++wv) sum += red[p][wv][part];
+    const double2 sh = shell[(size_t)q * nv + c0 + p];
+    double2* o = funcw + ((size_t)q * nen + w) * nv + c0 + p;
     if (part == 0) o->x = sum + sh.x;
     else o->y = sum + sh.y;
   }
@@ -450,14 +607,19 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
 void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int npair, int nen, double2* funcw,
-                       hipStream_t s) {
-  const int groups = (npair + kJVec - 1) / kJVec;
-  hipLaunchKernelGGL((k_w_contract<true, kJVec>), dim3(nen, ntq, groups), dim3(kCT), 0, s, eps,
-                     eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, npair, nen,
-                     funcw);
+                       double2* xg, const double2* shell, int ntq, int npair, int nen,
+                       double2* funcw, hipStream_t s) {
+  const int groups = (npair + 31) / 32;
+  const int g32 = 32 * groups;
+  hipLaunchKernelGGL(k_j_xvec, dim3(ntq), dim3(256), 0, s, rho, rho_off, qinfo, minv, pf1, pf2,
+                     map_off, npair, g32, xg);
+  hipLaunchKernelGGL(k_j_contract, dim3(nen, ntq, groups), dim3(512), 0, s, eps, eps_off, rho,
+                     rho_off, qinfo, minv, pf1, pf2, map_off, (const double2*)xg, g32, shell,
+                     npair, nen, funcw);
   RP_HIP(hipGetLastError());
 }
+
+size_t j_contract_scratch(size_t sum_nge, int npair) { return sum_nge * 32 * ((npair + 31) / 32); }
 
 void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
                   int g0, int q0, const double2* pfk, double2* rho, hipStream_t s) {
