@@ -104,20 +104,17 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // eps^-1 E (complex64, column-major), then funcw = sum_a Z(a) Y(a,.) + shell.
 //   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
-// kDiag = false (calc_w3d): the vectors are the nv <= 8 Wannier functions and
-// every (i,j) is formed, out [q][w][j][i].  kDiag = true (calc_j3d): the
-// vectors are the (iw,jw) pair densities, VB per workgroup (blockIdx.z), and
-// only (p,p) is formed, out [q][w][p].
-// Every element of E is read once per workgroup, and a workgroup takes all
-// the vectors of its (q, w) (up to VB), so E is read once per (q, w): 8 B of
-// HBM per 8 VB flops.  512 threads, rows a and a + 512 of E per thread (NG_eps
+// The vectors are the VB = nv <= 8 Wannier functions of calc_w3d and every
+// (i,j) is formed, out [q][w][j][i] (calc_j3d's diagonal pairs: k_j_contract).
+// A workgroup takes all the vectors of its (q, w), so E is read once per
+// (q, w): 8 B of HBM per 8 VB flops.  512 threads, rows a and a + 512 of E per thread (NG_eps
 // <= 1024); the loads of the next PD columns of E are in flight during a
 // column's arithmetic.
-// grid: (nen, ntq, groups).
+// grid: (nen, ntq).
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
 constexpr int kCT = 512;       // threads of the contraction
 constexpr int kTB = 32;        // columns of E staged per pass
-template <bool kDiag, int VB, int PD = (VB > 12 ? 3 : 8)>
+template <int VB, int PD = 8>
 __global__ void __launch_bounds__(512) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
@@ -125,15 +122,14 @@ __global__ void __launch_bounds__(512) k_w_contract(
     const double2* __restrict__ pf1, const double2* __restrict__ pf2,
     const long long* __restrict__ map_off, const double2* __restrict__ shell, int nv, int nen,
     double2* __restrict__ funcw) {
-  constexpr int kPairs = kDiag ? VB : VB * VB;
+  constexpr int kPairs = VB * VB;
   constexpr int kWaves = kCT / 64;
   const int w = blockIdx.x, q = blockIdx.y;
   const int4 qi = qinfo[q];
   const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
-  const int c0 = kDiag ? (int)blockIdx.z * VB : 0;
-  const int nvb = kDiag ? min(VB, nv - c0) : nv;  // vectors of this workgroup
-  const double2* r = rho + rho_off[q] + (size_t)c0 * ngpsi;
+  const int nvb = nv;  // vectors (Wannier functions)
+  const double2* r = rho + rho_off[q];
   const int* mi = minv + map_off[q];
   const double2* p1 = pf1 + map_off[q];
   const double2* p2 = pf2 + map_off[q];
@@ -216,15 +212,15 @@ __global__ void __launch_bounds__(512) k_w_contract(
     }
 #pragma unroll
     for (int yc = 0; yc < VB; ++yc) {
-      if (kDiag ? yc != zi : yc >= nvb) continue;
+      if (yc >= nvb) continue;
       double sr = (z0r * y0r[yc] - z0i * y0i[yc]) + (z1r * y1r[yc] - z1i * y1i[yc]);
       double si = (z0r * y0i[yc] + z0i * y0r[yc]) + (z1r * y1i[yc] + z1i * y1r[yc]);
       for (int off = 32; off > 0; off >>= 1) {
         sr += __shfl_xor(sr, off);
         si += __shfl_xor(si, off);
       }
-      // pair slot: J (p,p) at p; W: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
-      const int slot = kDiag ? zi : (trs == 1 ? zi + VB * yc : yc + VB * zi);
+      // pair slot: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
+      const int slot = trs == 1 ? zi + VB * yc : yc + VB * zi;
       if (lane == 0) {
         red[slot][wave][0] = sr;
         red[slot][wave][1] = si;
@@ -232,30 +228,23 @@ __global__ void __launch_bounds__(512) k_w_contract(
     }
   }
   __syncthreads();
-  const int npairs = kDiag ? nvb : nvb * nvb;
+  const int npairs = nvb * nvb;
   for (int e = t; e < 2 * npairs; e += kCT) {
     const int pr = e >> 1, part = e & 1;
-    const int i = kDiag ? pr : pr % nvb, j = kDiag ? pr : pr / nvb;
-    const int slot = kDiag ? i : i + VB * j;
+    const int i = pr % nvb, j = pr / nvb;
+    const int slot = i + VB * j;
     double sum = 0.0;
 #pragma unroll
     for (int wv = 0; wv < kWaves; ++wv) sum += red[slot][wv][part];
-    double2* o;
-    double2 sh;
-    if (kDiag) {
-      sh = shell[(size_t)q * nv + c0 + i];
-      o = funcw + ((size_t)q * nen + w) * nv + c0 + i;
-    } else {
-      sh = shell[((size_t)q * nv + j) * nv + i];
-      o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
-    }
+    const double2 sh = shell[((size_t)q * nv + j) * nv + i];
+    double2* o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
     if (part == 0) o->x = sum + sh.x;
     else o->y = sum + sh.y;
   }
 }
 
-// calc_j3d's contraction on the FP64 matrix cores: the same funcw_pp as
-// k_w_contract<true>, Y = E X as real products v_mfma_f64_16x16x4 (Re Y =
+// calc_j3d's contraction on the FP64 matrix cores: funcw_pp of the diagonal
+// pairs, by k_w_contract's formula, with Y = E X as real products v_mfma_f64_16x16x4 (Re Y =
 // Er Xr - Ei Xi, Im Y = Er Xi + Ei Xr) over 16-row blocks of E and 16-column
 // blocks of up to 32 pair vectors (blockIdx.z picks the 32).  X comes from
 // k_j_xvec, [b][32 groups] in HBM (zero past the last pair).  A wave takes up
@@ -329,7 +318,6 @@ __global__ void __launch_bounds__(512) k_j_contract(
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) accr[j][cb] = acci[j][cb] = jd4{0.0, 0.0, 0.0, 0.0};
     }
-#pragma unroll 2
     for (int b0 = 0; b0 < nge; b0 += 4) {
       const int b = b0 + lk;
       const bool bv = b < nge;  // columns past nge contribute zero
@@ -594,7 +582,7 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
   switch (nwf) {
 #define RP_WC(NV)                                                                              \
   case NV:                                                                                     \
-    hipLaunchKernelGGL((k_w_contract<false, NV>), dim3(nen, ntq), dim3(kCT), 0, s, eps,        \
+    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ntq), dim3(kCT), 0, s, eps,        \
                        eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, \
                        funcw);                                                                 \
     break;
