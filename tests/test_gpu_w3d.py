@@ -74,3 +74,31 @@ def test_j3d_matches_oracle():
     print(f"X {relf(x, o['X']):.3e} J {relf(j, jo):.3e}", flush=True)
     assert relf(x, o["X"]) <= 1e-6 and relf(j, jo) <= 1e-6
     assert np.abs(x - o["X"]).max() * AU <= 1e-3 and np.abs(j - jo).max() * AU <= 1e-3
+
+
+def test_six_wannier_functions():
+    """Six Wannier functions: W's contraction with six accumulators, and J's
+    36 pair vectors in two workgroup groups of 32 (the second one partial)."""
+    from oracle import pipeline_w3d
+    wl, s, w, W, em, sqi, eps = _w3d_case("random", nwf=6, seed=11)
+    o = pipeline_w3d.run_w3d(w, W, sqi, eps, em, wl["ecut_eps"])
+    rlist = [(0, 0, 0), (0, 1, 0)]
+    oj = pipeline_w3d.run_j3d(w, W, sqi, eps, em, wl["ecut_eps"], rlist)
+    g = W3d(0)
+    try:
+        g.open(w, wl["ecut_eps"], em)
+        g.set_wannier(W)
+        g.set_q(sqi)
+        for iq, e in enumerate(eps):
+            g.set_epsinv(iq, e)
+        func, funcw = g.funcs()
+        v, wm, _, _ = g.vw()
+        x, j, _, _ = g.j3d(rlist)
+    finally:
+        g.close()
+    jo = np.moveaxis(oj["J"], 1, 0)
+    print(f"nwf 6: funcw {relf(funcw, o['funcw']):.3e} W {relf(wm, o['wmat']):.3e} "
+          f"X {relf(x, oj['X']):.3e} J {relf(j, jo):.3e}", flush=True)
+    assert relf(funcw, o["funcw"]) <= 1e-6 and relf(wm, o["wmat"]) <= 1e-6
+    assert relf(x, oj["X"]) <= 1e-6 and relf(j, jo) <= 1e-6
+    assert np.abs(j - jo).max() * AU <= 1e-3
