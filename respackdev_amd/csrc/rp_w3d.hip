@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(512) k_w_contract(
 // D(l / 16 + 4 v, l % 16), v < 4 (checked on the GPU with row- and column-numbered
 // operands).  grid: (nen, ntq, groups), 512 threads.
 typedef double jd4 __attribute__((ext_vector_type(4)));
-constexpr int kJA = 2;
+constexpr int kJA = 1;
 __global__ void __launch_bounds__(256) k_j_xvec(const double2* __restrict__ rho,
                                                 const long long* __restrict__ rho_off,
                                                 const int4* __restrict__ qinfo,
