@@ -8,6 +8,8 @@ the attributes of respackdev_amd.chiqw.WfnData (the dir-wfn contents).
 from __future__ import annotations
 
 import ctypes as C
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -64,8 +66,11 @@ def setup(w, ecut_for_eps=0.0, num_freq_grid=70, n_calc_band=0, shift_ef=0.0,
     kg0 = np.zeros((ntk, ntg, 3), np.int32)
     Ls = [int(np.abs(w.kgi[:, :, d]).max()) + 1 for d in range(3)]
     coef = np.zeros((nb, ntk, ntg), np.complex128)
-    tmp = np.zeros((200000, 3), np.int32)
-    for k in range(ntk):
+
+    def one_k(k):
+        # each k writes its own slices; the C calls release the GIL, so the
+        # k points run in threads
+        tmp = np.zeros((200000, 3), np.int32)
         ik, op = int(numirr[k]), int(numrot[k])
         r = np.asarray(w.rg[op]).reshape(3, 3).T  # r[i,j] = rg(i,j)
         kk = r @ w.ski[ik] + rw[k]
@@ -86,6 +91,8 @@ def setup(w, ecut_for_eps=0.0, num_freq_grid=70, n_calc_band=0, shift_ef=0.0,
             L.ro_make_c0(int(trs[k]), m, ptr(kgk), ptr(rwk), ptr(rinv), ptr(pgk), int(w.nnp),
                          Ls[0], Ls[1], Ls[2], ptr(pack), ptr(occ), ptr(out))
             coef[ib, k, :m] = out
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        list(ex.map(one_k, range(ntk)))
     # band classification
     ef = w.ef + shift_ef / AU
     eig = np.ascontiguousarray(w.eig, np.float64)

@@ -8,6 +8,9 @@ the reference's loop order (calc_intW.F:647-985).
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import lib, ptr
@@ -27,7 +30,8 @@ def setup_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     [ie][jg][ig] complex (already averaged at Gamma); em: nen complex."""
     L = lib()
     # k mesh, G lists, FFT grid: identical to chiqw's setup (calc_intW.F:124-330)
-    s = chiqw_setup(w, ecut_for_eps=ecut_eps, num_freq_grid=1)
+    # (calc_int reads no band wavefunctions: one band keeps the shared setup cheap)
+    s = chiqw_setup(w, ecut_for_eps=ecut_eps, num_freq_grid=1, n_calc_band=1)
     b, vol = s["b"], s["volume"]
     ntk, ntg = s["ntk"], s["ntg"]
     nkb = [int(x) for x in s["nkb"]]
@@ -57,9 +61,9 @@ def setup_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     ntq = len(sq)
     assert ntq == ntk, "NTQ should be NTK"
     # G lists per q (calc_intW.F:500-545)
-    tmp = np.zeros((ntg, 3), np.int32)
-    lg0, nge, ngp = [], [], []
-    for iq in range(ntq):
+    def one_q(iq):
+        # the C calls release the GIL: the q points run in threads
+        tmp = np.zeros((ntg, 3), np.int32)
         if iq < nq_irr:
             q = np.asarray(sqi[iq], np.float64)
         else:
@@ -69,18 +73,24 @@ def setup_w3d(w, wan, sqi, epsinv, em, ecut_eps):
         n = L.ro_make_lg0(ntg, ptr(b[0]), ptr(b[1]), ptr(b[2]), ecut_eps, ecut_psi, q[0], q[1], q[2],
                           ptr(tmp), ptr(ne))
         assert n <= ntg
-        lg = tmp[:n].copy() * trsq[iq]
-        lg0.append(np.ascontiguousarray(lg)); nge.append(int(ne[0])); ngp.append(n)
+        return np.ascontiguousarray(tmp[:n] * trsq[iq]), int(ne[0]), n
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        res = list(ex.map(one_q, range(ntq)))
+    lg0 = [r_[0] for r_ in res]
+    nge = [r_[1] for r_ in res]
+    ngp = [r_[2] for r_ in res]
     ntgq = max(nge)
     # packing of the irreducible eps spheres, epsirr in single precision
     lq = [int(max(np.abs(l[:, d]).max() for l in lg0)) + 1 for d in range(3)]
-    epsirr = []
+    # epsinv: a list, or a function of the irreducible index (then each epsirr
+    # is made when first needed, for workloads too large to hold them all)
+    epsirr = _LazyEps(epsinv, nge, nen, ntgq) if callable(epsinv) else []
     packs = []
     for iq in range(nq_irr):
-        e = np.zeros((nen, ntgq, ntgq), np.complex64)  # Fortran (ig,jg,ie) with ld ntgq
+        if not callable(epsinv):
+            epsirr.append(_epsirr_one(epsinv[iq], nge[iq], nen, ntgq))
         n = nge[iq]
-        e[:, :n, :n] = epsinv[iq].astype(np.complex64)
-        epsirr.append(np.ascontiguousarray(e))
         pk = np.zeros((2 * lq[2] + 1, 2 * lq[1] + 1, 2 * lq[0] + 1), np.int32)
         g = lg0[iq][:n]
         pk[g[:, 2] + lq[2], g[:, 1] + lq[1], g[:, 0] + lq[0]] = np.arange(1, n + 1)
@@ -90,6 +100,27 @@ def setup_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     return dict(L=L, s=s, b=b, vol=vol, ntk=ntk, ntg=ntg, na=na, nwf=nwf, nen=nen, nq_irr=nq_irr,
                 sq=sq, numirrq=numirrq, numrotq=numrotq, trsq=trsq, rwq=rwq, ntq=ntq, lg0=lg0,
                 nge=nge, ngp=ngp, ntgq=ntgq, lq=lq, epsirr=epsirr, packs=packs, rginv=rginv)
+
+
+def _epsirr_one(e_in, n, nen, ntgq):
+    e = np.zeros((nen, ntgq, ntgq), np.complex64)  # Fortran (ig,jg,ie) with ld ntgq
+    e[:, :n, :n] = np.asarray(e_in).astype(np.complex64)
+    return np.ascontiguousarray(e)
+
+
+class _LazyEps:
+    """epsirr made on first use from fn(iq) (the last few kept)."""
+
+    def __init__(self, fn, nge, nen, ntgq, keep=4):
+        self.fn, self.nge, self.nen, self.ntgq, self.keep = fn, nge, nen, ntgq, keep
+        self.cache = {}
+
+    def __getitem__(self, iq):
+        if iq not in self.cache:
+            if len(self.cache) >= self.keep:
+                self.cache.pop(next(iter(self.cache)))
+            self.cache[iq] = _epsirr_one(self.fn(iq), self.nge[iq], self.nen, self.ntgq)
+        return self.cache[iq]
 
 
 def _epsmk(c, w, iq, out):
@@ -112,61 +143,90 @@ def _heads(c, w):
     return chead, cheadw
 
 
+def _rho_q(c, wan, iq):
+    """rho_w(q+G) of full-mesh q iq (calc_intW.F:647-767): [w][G] over NGpsi(q)."""
+    L, s, b = c["L"], c["s"], c["b"]
+    ntk, ntg, nwf = c["ntk"], c["ntg"], c["nwf"]
+    sk0, ng0, kg0, nw = s["sk0"], s["ng0"], s["kg0"], s["nw"]
+    q = c["sq"][iq]
+    q0 = q[0] == 0.0 and q[1] == 0.0 and q[2] == 0.0
+    n = c["ngp"][iq]
+    lg = c["lg0"][iq]
+    qg = np.linalg.norm((q[None, :] + lg) @ b, axis=1)
+    nog0 = int(np.where((lg == 0).all(1))[0][0]) if q0 else -1
+    def per_k(k):
+        # M_ww,k for every w (the C calls release the GIL: k points run in threads)
+        out = np.zeros((nwf, n), np.complex128)
+        m = np.zeros(ntg, np.complex128)
+        sg = np.zeros(3, np.int32)
+        kq = k if q0 else L.ro_search_kq(ntk, ptr(sk0), q[0], q[1], q[2], k, ptr(sg))
+        kgk = np.ascontiguousarray(kg0[k, :ng0[k]])
+        kgq = np.ascontiguousarray(kg0[kq, :ng0[kq]])
+        for iw in range(nwf):
+            ck = np.ascontiguousarray(wan[k, iw, :ng0[k]])
+            cq = np.ascontiguousarray(wan[kq, iw, :ng0[kq]])
+            L.ro_pairdens(int(ng0[k]), ptr(kgk), ptr(ck), int(ng0[kq]), ptr(kgq), ptr(cq),
+                          nw[0], nw[1], nw[2], n, ptr(lg), ptr(sg), ptr(m))
+            if q0:
+                m[nog0] = 0.0
+            out[iw] = m[:n]
+        return out
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        parts = list(ex.map(per_k, range(ntk)))
+    rho = np.zeros((nwf, ntg), np.complex128)
+    for p in parts:  # the k order of the reference's sum
+        rho[:, :n] += p
+    for g in range(n):
+        if g != nog0:
+            rho[:, g] = rho[:, g] / qg[g]
+    return rho / float(ntk)
+
+
+def _func_q(c, w, rho, iq, epsmk):
+    """func and funcw of full-mesh q iq (calc_intW.F:833-912)."""
+    L, nwf, nen, ntg = c["L"], c["nwf"], c["nen"], c["ntg"]
+    r = np.ascontiguousarray(rho)
+    f = np.zeros((nwf, nwf), np.complex128)
+    L.ro_w3d_func(nwf, c["ngp"][iq], ntg, ptr(r), ptr(f))
+    _epsmk(c, w, iq, epsmk)
+    fw = np.zeros((nen, nwf, nwf), np.complex128)
+    L.ro_w3d_funcw(nwf, c["nge"][iq], c["ngp"][iq], ntg, ptr(r), c["ntgq"], nen, ptr(epsmk), ptr(fw))
+    return f, fw
+
+
+def funcs_at(w, wan, sqi, epsinv, em, ecut_eps, iqs):
+    """func and funcw (calc_intW.F:647-912) at the full-mesh q indices iqs only:
+    dict(func [len(iqs)][j][i], funcw [len(iqs)][ie][j][i], sq, trsq, numirrq).
+    epsinv may be a function of the irreducible index; iqs may be a function
+    of the setup dict (keys sq, trsq, numirrq, nge) returning the indices."""
+    c = setup_w3d(w, wan, sqi, epsinv, em, ecut_eps)
+    if callable(iqs):
+        iqs = iqs(c)
+    epsmk = np.zeros((c["nen"], c["ntgq"], c["ntgq"]), np.complex64)
+    func, funcw = [], []
+    for iq in iqs:
+        f, fw = _func_q(c, w, _rho_q(c, wan, iq), iq, epsmk)
+        func.append(f)
+        funcw.append(fw)
+    return dict(func=np.array(func), funcw=np.array(funcw), sq=np.array(c["sq"]), iqs=list(iqs),
+                trsq=list(c["trsq"]), numirrq=list(c["numirrq"]), nge=c["nge"])
+
+
 def run_w3d(w, wan, sqi, epsinv, em, ecut_eps):
     """calc_w3d: rho, func, funcw, V, W (calc_intW.F:647-985)."""
     c = setup_w3d(w, wan, sqi, epsinv, em, ecut_eps)
-    L, s, b = c["L"], c["s"], c["b"]
-    ntk, ntg, ntq, nwf, nen = c["ntk"], c["ntg"], c["ntq"], c["nwf"], c["nen"]
+    L = c["L"]
+    ntg, ntq, nwf, nen = c["ntg"], c["ntq"], c["nwf"], c["nen"]
     sq, lg0, ngp, nge, na, vol = c["sq"], c["lg0"], c["ngp"], c["nge"], c["na"], c["vol"]
     ntgq = c["ntgq"]
-    # rho (calc_intW.F:647-767)
-    sk0 = s["sk0"]
-    ng0 = s["ng0"]
-    kg0 = s["kg0"]
-    nw = s["nw"]
     rho = np.zeros((ntq, nwf, ntg), np.complex128)
-    m = np.zeros(ntg, np.complex128)
-    for iq in range(ntq):
-        q = sq[iq]
-        q0 = q[0] == 0.0 and q[1] == 0.0 and q[2] == 0.0
-        n = ngp[iq]
-        lg = lg0[iq]
-        qg = np.linalg.norm((q[None, :] + lg) @ b, axis=1)
-        nog0 = -1
-        if q0:
-            nog0 = int(np.where((lg == 0).all(1))[0][0])
-        for k in range(ntk):
-            sg = np.zeros(3, np.int32)
-            if q0:
-                kq = k
-            else:
-                kq = L.ro_search_kq(ntk, ptr(sk0), q[0], q[1], q[2], k, ptr(sg))
-            kgk = np.ascontiguousarray(kg0[k, :ng0[k]])
-            kgq = np.ascontiguousarray(kg0[kq, :ng0[kq]])
-            for iw in range(nwf):
-                ck = np.ascontiguousarray(wan[k, iw, :ng0[k]])
-                cq = np.ascontiguousarray(wan[kq, iw, :ng0[kq]])
-                L.ro_pairdens(int(ng0[k]), ptr(kgk), ptr(ck), int(ng0[kq]), ptr(kgq), ptr(cq),
-                              nw[0], nw[1], nw[2], n, ptr(lg), ptr(sg), ptr(m))
-                if q0:
-                    m[nog0] = 0.0
-                rho[iq, iw, :n] += m[:n]
-        for iw in range(nwf):
-            for g in range(n):
-                if g != nog0:
-                    rho[iq, iw, g] = rho[iq, iw, g] / qg[g]
-    rho /= float(ntk)
-    # func / funcw (calc_intW.F:833-912)
     func = np.zeros((ntq, nwf, nwf), np.complex128)
     funcw = np.zeros((ntq, nen, nwf, nwf), np.complex128)
     epsmk = np.zeros((nen, ntgq, ntgq), np.complex64)
     for iq in range(ntq):
-        r = np.ascontiguousarray(rho[iq])
-        L.ro_w3d_func(nwf, ngp[iq], ntg, ptr(r), ptr(func[iq]))
-        _epsmk(c, w, iq, epsmk)
-        fw = np.zeros((nen, nwf, nwf), np.complex128)
-        L.ro_w3d_funcw(nwf, nge[iq], ngp[iq], ntg, ptr(r), ntgq, nen, ptr(epsmk), ptr(fw))
-        funcw[iq] = fw
+        rho[iq] = _rho_q(c, wan, iq)
+        func[iq], funcw[iq] = _func_q(c, w, rho[iq], iq, epsmk)
     chead, cheadw = _heads(c, w)
     n1, n2, n3 = (2 * x + 1 for x in na)
     vmat = np.zeros((n3, n2, n1, nwf, nwf), np.complex128)

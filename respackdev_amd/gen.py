@@ -242,16 +242,23 @@ def make_wannier(sysm: System, nwf: int, seed: int, alpha: float = 1.0, spread: 
     c_w(k+G) ~ exp(-|k+G|^2/(4 alpha)) Y_w(k+G) exp(-i (k+G).tau_w) on every
     full-mesh k, normalised per k.  Returns (wan list per k of (nwf, NG0) complex,
     centres (nwf, 3) bohr)."""
+    sk0, _ = full_kmesh(sysm.ski)
+    return wannier_on(sk0, full_kg(sysm), sysm.bvec, nwf, seed, alpha, spread)
+
+
+def wannier_on(sk0, kgs, bvec: np.ndarray, nwf: int, seed: int, alpha: float = 1.0,
+               spread: float = 0.5):
+    """The model Wannier functions of make_wannier on any full k mesh: sk0
+    (NTK, 3) fractional and kgs the G list of each k (the plane waves k + G)."""
     names = (["dxy", "dyz", "dzx", "dx2y2", "dz2"] if nwf == 5 else
              ["s", "px", "py", "pz", "dxy", "dyz", "dzx", "dx2y2", "dz2"][:nwf])
     if len(names) != nwf:
         raise ValueError("at most 9 model Wannier functions")
     rng = np.random.Generator(np.random.Philox(seed))
     tau = rng.uniform(-spread, spread, (nwf, 3)) / np.sqrt(3.0)
-    sk0, _ = full_kmesh(sysm.ski)
     wan = []
-    for k, kg in zip(sk0, full_kg(sysm)):
-        p = (k[None, :] + kg) @ sysm.bvec
+    for k, kg in zip(sk0, kgs):
+        p = (np.asarray(k)[None, :] + kg) @ bvec
         e2 = np.einsum("ij,ij->i", p, p)
         c = np.empty((nwf, len(kg)), np.complex128)
         for w, nm in enumerate(names):

@@ -124,3 +124,36 @@ def make(nk=4, nocc=2, nvir=4, ecut_psi=4.5, seed=77, a=7.0):
     ident = np.eye(3, dtype=np.int32).reshape(1, 9).copy()
     wB = wfn_data(sk0[:ntk], kgB, coefB, np.array(eigB), ident, np.zeros((1, 3), np.int32), 1)
     return wA, wB
+
+
+
+def w3d_case(nwf=3, ecut_eps=2.0, nen=6, seed=31):
+    """calc_int input on the screw-axis form of the crystal: its WfnData, the
+    irreducible q list (the k list), model Wannier functions on the full mesh
+    in calc_int's unfolding order and G lists (calc_intW.F:208-330), seeded
+    random eps^-1 per irreducible q ([ie][jg][ig], NG_eps of make_LG0) and the
+    frequencies.  calc_int rotates each eps^-1 to the other q of its star
+    with make_eps (calc_int_sub.F:222-311): four rotations, fractional
+    translations (0, 0, j/4) and time reversal."""
+    from oracle.pipeline_w3d import setup_w3d
+    wA, _ = make()
+    sqi = np.array([_fold3(k) for k in wA.ski])
+    lat = np.asarray(wA.lattice)
+    b = 2.0 * np.pi * np.linalg.inv(lat).T
+    em = oracle.gen_grid(nen, 50.0 / gen.AU_EV)
+    rng = np.random.default_rng(seed)
+    eps = []
+    for q in sqi:
+        n = len(oracle.make_kg0(b, ecut_eps, q))
+        e = np.eye(n)[None] + 0.05 * (rng.standard_normal((nen, n, n)) +
+                                      1j * rng.standard_normal((nen, n, n)))
+        eps.append(e)
+    c = setup_w3d(wA, np.zeros((1, nwf, wA.kgi.shape[1])), sqi, eps, em, ecut_eps)
+    s = c["s"]
+    kgs = [s["kg0"][k, :s["ng0"][k]] for k in range(c["ntk"])]
+    wan, _ = gen.wannier_on(s["sk0"], kgs, c["b"], nwf, seed)
+    W = np.zeros((len(wan), nwf, wA.kgi.shape[1]), np.complex128)
+    for k, cw in enumerate(wan):
+        W[k, :, :cw.shape[1]] = cw
+    return dict(w=wA, sqi=sqi, eps=eps, wan=W, em=em, ecut_eps=ecut_eps, sk0=s["sk0"],
+                ng0=s["ng0"], trsq=list(c["trsq"]), numrotq=list(c["numrotq"]))
