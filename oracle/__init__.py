@@ -95,7 +95,9 @@ def have_ref() -> bool:
 def ref():
     global _ref
     if _ref is None:
-        R = C.CDLL(REF_SO, mode=os.RTLD_LAZY)
+        # RTLD_GLOBAL: MKL dlopens its CPU-specific kernels, which resolve symbols
+        # of libmkl_core loaded with this library
+        R = C.CDLL(REF_SO, mode=os.RTLD_LAZY | os.RTLD_GLOBAL)
         R.rpref_gen_grid.argtypes = [I32, D, P]
         R.rpref_algn235.restype = I32
         R.rpref_algn235.argtypes = [I32]
@@ -115,6 +117,8 @@ def ref():
         R.rpref_pairdens.argtypes = [I32, I32, P, P, P, P, I32, I32, I32, I32, I32, P, I32, P, P]
         R.rpref_vmab.argtypes = [I32, I32, P, P, P, P, I32, P, P, P, P]
         R.rpref_vmaa.argtypes = [I32, I32, P, P, P, P, I32, P, P, P, P, P]
+        R.rpref_eps_rpa.argtypes = [I32, I32, P, D, D, D, P, P, P, P, P]
+        R.rpref_eps_rpa_q0.argtypes = [I32, I32, P, P, P, P, P, I32, D, D, D, D, P]
         _ref = R
     return _ref
 
@@ -133,6 +137,25 @@ def ref_int():
         R.rpref_int_pairdens.argtypes = [I32, I32, P, P, P, P, I32, I32, I32, I32, I32, P, I32, P, P]
         _ref_int = R
     return _ref_int
+
+
+_ref_mvmc = {}
+
+
+def have_ref_mvmc() -> bool:
+    return all(os.path.exists(os.path.join(_HERE, "_ref", f"librespack_ref_mvmc{x}.so")) for x in "wj")
+
+
+def ref_mvmc(kind: str):
+    """The reference's mVMC export writer wrt_mvmc of wrt_mvmc_w3d.f90 (kind
+    "w") or wrt_mvmc_j3d.f90 (kind "j"): rpref_wrt_mvmc(nkb1, nkb2, nkb3, NTK,
+    Na1, Na2, Na3, NWF, cut (Ha), HR (NWF, NWF, -Na1:Na1, -Na2:Na2, -Na3:Na3)
+    complex, Ha) writes its .def files into the current directory."""
+    if kind not in _ref_mvmc:
+        R = C.CDLL(os.path.join(_HERE, "_ref", f"librespack_ref_mvmc{kind}.so"))
+        R.rpref_wrt_mvmc.argtypes = [I32, I32, I32, I32, I32, I32, I32, I32, D, P]
+        _ref_mvmc[kind] = R
+    return _ref_mvmc[kind]
 
 
 # ----------------------------------------------------------------- helpers

@@ -3,11 +3,12 @@
 !
 ! TEST INFRASTRUCTURE ONLY.  The resulting library (oracle/_ref/librespack_ref.so)
 ! is used by tests/ as a checker for the oracle's C restatement; nothing in the
-! product links or loads it.  Only files of the reference that need no external
-! library are compiled (no MPI, no LAPACK): m_fft3d_20150826.f90,
-! m_tetrahedron_20170325.F, gen_grid.f90, est_nwx2.f90, est_NTK.f90 and
-! chiqw_sub.F (its LAPACK-calling calc_eps_rpa* are discarded at link time by
-! --gc-sections because nothing exported reaches them).
+! product links or loads it.  The files compiled are m_fft3d_20150826.f90,
+! m_tetrahedron_20170325.F, gen_grid.f90, est_nwx2.f90, est_NTK.f90,
+! chiqw_sub.F and invZGE.F90; the LAPACK routines zgetrf/zgetri that invZGE
+! calls (the Dyson step calc_eps_rpa / calc_eps_rpa_q_0) come from the Intel
+! MKL installed in the image (/opt/conda/lib), as in the reference's own
+! Makefile (src/chiqw/Makefile:10).  No MPI is needed by these files.
 module rpref_shim
   use iso_c_binding
   use fft_3d
@@ -215,6 +216,47 @@ contains
     complex(c_double_complex) :: c0k(ntg), c0kq(ntg), vm(3)
     real(c_double) :: b1(3), b2(3), b3(3), skt(3)
     call calc_VMaa(ntk, ntg, ng0, kg0, c0k, c0kq, ik, b1, b2, b3, skt, vm)
+  end subroutine
+
+  ! calc_eps_rpa (chiqw_sub.F:386-457): chi0 (NG x NG, doubled in place as
+  ! the reference does) -> eps^-1 by zgetrf/zgetri.  Its diagonal text output
+  ! goes to a scratch unit.
+  subroutine rpref_eps_rpa(ntg, ng, lg0, q1, q2, q3, b1, b2, b3, chi0, eps) &
+       bind(C, name="rpref_eps_rpa")
+    integer(c_int), value :: ntg, ng
+    integer(c_int) :: lg0(3, ntg)
+    real(c_double), value :: q1, q2, q3
+    real(c_double) :: b1(3), b2(3), b3(3)
+    complex(c_double_complex) :: chi0(ng, ng), eps(ng, ng)
+    integer :: u
+    real(8) :: tpi, a, b, c
+    tpi = 2.0d0*dacos(-1.0d0)   ! chiqw/config.h:20-21
+    a = q1; b = q2; c = q3
+    open(newunit=u, status='scratch')
+    call calc_eps_rpa(ntg, ng, lg0, a, b, c, b1, b2, b3, chi0, tpi, eps, 0, u)
+    close(u)
+  end subroutine
+
+  ! calc_eps_rpa_q_0 (chiqw_sub.F:459-568): head, wings, Drude term of plasma
+  ! frequency wd at complex frequency wcmplx (only its real part is used).
+  subroutine rpref_eps_rpa_q0(ntg, ng, lg0, b1, b2, b3, chi0, g0, wd, delt, wre, wim, eps) &
+       bind(C, name="rpref_eps_rpa_q0")
+    integer(c_int), value :: ntg, ng, g0
+    integer(c_int) :: lg0(3, ntg)
+    real(c_double) :: b1(3), b2(3), b3(3)
+    real(c_double), value :: wd, delt, wre, wim
+    complex(c_double_complex) :: chi0(ng, ng), eps(ng, ng)
+    integer :: u, ng0
+    real(8) :: tpi, z, wdd, dd
+    complex(8) :: wc
+    tpi = 2.0d0*dacos(-1.0d0)
+    z = 0.0d0
+    wdd = wd; dd = delt
+    wc = cmplx(wre, wim, kind=8)
+    ng0 = g0
+    open(newunit=u, status='scratch')
+    call calc_eps_rpa_q_0(ntg, ng, lg0, z, z, z, b1, b2, b3, chi0, tpi, ng0, eps, wdd, dd, wc, 0, u)
+    close(u)
   end subroutine
 
 end module rpref_shim

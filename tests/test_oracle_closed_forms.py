@@ -1,6 +1,7 @@
-"""Pins the oracle where the reference cannot be compiled here (its LAPACK
-inversion and its MPI driver with the inline chi accumulation, chiqw.F:1397-1410)
-through closed forms, library routines and invariants."""
+"""Pins the oracle where no compiled reference routine can (the inline chi
+accumulation of calc_chiqw's MPI driver, chiqw.F:1397-1476) and adds closed
+forms, library routines and invariants beside the compiled-reference tests
+(test_oracle_vs_ref.py, which also holds the Dyson step calc_eps_rpa)."""
 import numpy as np
 import pytest
 
@@ -23,22 +24,6 @@ def test_zinv_pivots_and_detects_singularity():
     np.testing.assert_allclose(oracle.zinv(a), a)
     with pytest.raises(np.linalg.LinAlgError):
         oracle.zinv(np.zeros((3, 3), np.complex128))
-
-
-def test_eps_rpa_formula_and_inverse():
-    rng = np.random.default_rng(1)
-    lg = oracle.make_kg0(B7, 3.0, (0, 0, 0.25))
-    n = len(lg)
-    chi = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * 1e-3
-    q = np.array([0, 0, 0.25])
-    out = np.zeros((n, n), np.complex128, order="F")
-    chif = np.asfortranarray(chi)
-    b = [np.ascontiguousarray(B7[i]) for i in range(3)]
-    assert oracle.lib().ro_eps_rpa(n, ptr(lg), ptr(q), ptr(b[0]), ptr(b[1]), ptr(b[2]),
-                                   chif.ctypes.data, out.ctypes.data) == 0
-    qg = np.linalg.norm((q + lg) @ B7, axis=1)
-    eps = np.eye(n) - 2 * chi * 4 * np.pi / np.outer(qg, qg)
-    np.testing.assert_allclose(out, np.linalg.inv(eps), rtol=1e-11, atol=1e-13)
 
 
 def test_eps_rpa_q0_is_the_small_q_limit():
@@ -130,3 +115,24 @@ def test_pairdens_single_plane_wave_closed_form():
                                  nw, nw, nw, len(lg), ptr(lg), ptr(sg), ptr(m))
         want = np.zeros(len(lg)); want[j] = 1.0
         np.testing.assert_allclose(m, want, atol=1e-7)
+
+
+def test_chi_of_plane_wave_states_closed_form():
+    """The oracle's chi and eps^-1 (ro_chiqw_q, ro_eps_rpa) for bands made of
+    one or two plane waves with flat energies, against the closed form of
+    tests/plane_wave_case.py: pins 1/NTK, 1/V, the spin factor, 4 pi and the
+    conj placement of chiqw.F:1397-1410."""
+    import plane_wave_case as pw
+    from oracle import pipeline
+    w = pw.wfn()
+    o = pipeline.run_q(w, pw.Q, ecut_for_eps=pw.ECUT_EPS, num_freq_grid=pw.NEN)
+    delt = 0.1 / AU
+    lg, chi, epsinv = pw.closed_form(o["em"], delt)
+    np.testing.assert_array_equal(o["lg"], lg)
+    e_chi = np.linalg.norm(o["chi"][0] - chi) / np.linalg.norm(chi)
+    e_eps = np.linalg.norm(o["epsinv"][0] - epsinv) / np.linalg.norm(epsinv)
+    # the two-plane-wave bands give off-diagonal elements (conj placement)
+    off = chi - np.einsum("eii->ei", chi)[:, :, None] * np.eye(chi.shape[-1])[None]
+    assert np.linalg.norm(off) > 0.3 * np.linalg.norm(chi)
+    assert np.linalg.norm(np.conj(np.swapaxes(off, 1, 2)) - off) > 0.1 * np.linalg.norm(off)
+    assert e_chi <= 1e-6 and e_eps <= 1e-6, (e_chi, e_eps)

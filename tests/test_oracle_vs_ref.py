@@ -17,6 +17,7 @@ from oracle import ptr
 pytestmark = pytest.mark.skipif(not oracle.have_ref(), reason="reference build absent")
 
 B7 = np.eye(3) * (2 * np.pi / 7.0)
+AU = 27.21151  # chiqw/config.h:19
 
 
 def test_gen_grid_matches_reference():
@@ -375,3 +376,66 @@ def test_fermi_surface_weights_on_a_metal():
           f"5 eV, {worst_high:.2e} relative above")
     assert worst_low <= 1e-6
     assert worst_high > 1e-4
+
+
+def _chi_for_dyson(n, seed, scale=0.02):
+    """A chi0 whose eps is well away from singular: chi = A^H D A / n with
+    random A and a complex diagonal D (the structure of chiqw.F:1397-1410)."""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((3 * n, n)) + 1j * rng.standard_normal((3 * n, n))
+    d = -rng.uniform(0.2, 1.0, 3 * n) - 1j * rng.uniform(0.0, 0.3, 3 * n)
+    return scale * (a.conj().T * d) @ a / n
+
+
+@pytest.mark.parametrize("q", [(0.0, 0.0, 0.25), (0.25, -1.0 / 3.0, 0.5)])
+def test_eps_rpa_matches_reference(q):
+    """ro_eps_rpa against calc_eps_rpa (chiqw_sub.F:386-457) with its LAPACK
+    zgetrf/zgetri: spin factor 2, 4 pi / (|q+G||q+G'|), the inversion."""
+    L, R = oracle.lib(), oracle.ref()
+    lg = np.ascontiguousarray(oracle.make_kg0(B7, 3.0, q), np.int32)
+    n = len(lg)
+    chi = np.asfortranarray(_chi_for_dyson(n, 3))
+    qq = np.ascontiguousarray(q, np.float64)
+    mine = np.zeros((n, n), np.complex128, order="F")
+    assert L.ro_eps_rpa(n, ptr(lg), ptr(qq), ptr(B7[0]), ptr(B7[1]), ptr(B7[2]),
+                        chi.ctypes.data, mine.ctypes.data) == 0
+    ref = np.zeros((n, n), np.complex128, order="F")
+    c = chi.copy(order="F")  # the reference doubles chi0 in place
+    R.rpref_eps_rpa(n, n, ptr(lg), *q, ptr(np.ascontiguousarray(B7[0])),
+                    ptr(np.ascontiguousarray(B7[1])), ptr(np.ascontiguousarray(B7[2])),
+                    c.ctypes.data, ref.ctypes.data)
+    err = np.linalg.norm(mine - ref) / np.linalg.norm(ref)
+    assert err <= 1e-13, err
+    # a dropped spin factor or 4 pi would move eps^-1 by far more than the bar
+    assert np.linalg.norm(ref - np.eye(n)) / np.linalg.norm(ref) > 1e-3
+
+
+@pytest.mark.parametrize("wd", [0.0, 0.35])
+def test_eps_rpa_q0_matches_reference_with_drude_term(wd):
+    """ro_eps_rpa_q0 against calc_eps_rpa_q_0 (chiqw_sub.F:459-568): head,
+    wings, the Drude term -wd^2/(w^2+d^2) + i d wd^2/(w^3+w d^2) rounded to
+    single precision (CMPLX without KIND), w = 0 replaced by 1e-5, over a
+    frequency grid."""
+    L, R = oracle.lib(), oracle.ref()
+    lg = np.ascontiguousarray(oracle.make_kg0(B7, 3.0, (0.0, 0.0, 0.0)), np.int32)
+    n = len(lg)
+    g0 = int(np.where((lg == 0).all(1))[0][0])
+    chi = _chi_for_dyson(n, 5)
+    # q -> 0 limits of the head and wings are finite: scale like |q| and |q|^2
+    chi[g0, :] *= 0.3
+    chi[:, g0] *= 0.3
+    chi = np.asfortranarray(chi)
+    delt = 0.1 / AU
+    worst = 0.0
+    for w in oracle.gen_grid(16, 50.0 / AU):
+        mine = np.zeros((n, n), np.complex128, order="F")
+        assert L.ro_eps_rpa_q0(n, ptr(lg), ptr(B7[0]), ptr(B7[1]), ptr(B7[2]), g0, wd, delt,
+                               float(w.real), chi.ctypes.data, mine.ctypes.data) == 0
+        ref = np.zeros((n, n), np.complex128, order="F")
+        c = chi.copy(order="F")
+        R.rpref_eps_rpa_q0(n, n, ptr(lg), ptr(np.ascontiguousarray(B7[0])),
+                           ptr(np.ascontiguousarray(B7[1])), ptr(np.ascontiguousarray(B7[2])),
+                           c.ctypes.data, g0 + 1, wd, delt, float(w.real), float(w.imag),
+                           ref.ctypes.data)
+        worst = max(worst, np.linalg.norm(mine - ref) / np.linalg.norm(ref))
+    assert worst <= 1e-13, worst
