@@ -35,7 +35,7 @@ module respack_gpu
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
   public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
   public :: rp_chiqw_batch_plan, rp_chiqw_batch_run, rp_chiqw_batch_result, rp_chiqw_batch_free
-  public :: rp_chiqw_autotune, rp_chiqw_tune_count, rp_chiqw_tune_info
+  public :: rp_chiqw_autotune, rp_chiqw_tune_count, rp_chiqw_tune_info, rp_chiqw_set_span
   public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
   public :: rp_mlp_gradient, rp_mlp_train, rp_chi_set
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
@@ -195,6 +195,11 @@ module respack_gpu
     integer(c_int) function rp_chiqw_batch_free(h) bind(C, name="rp_chiqw_batch_free")
       import :: c_ptr, c_int
       type(c_ptr), value :: h
+    end function
+    integer(c_int) function rp_chiqw_set_span(h, span) bind(C, name="rp_chiqw_set_span")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: span
     end function
     integer(c_int) function rp_chiqw_autotune(h, enable) bind(C, name="rp_chiqw_autotune")
       import :: c_ptr, c_int, c_int32_t

@@ -140,6 +140,14 @@ int rp_chiqw_npair(rp_handle h, int32_t* npair);
  * N_CALC_BAND, chiqw.F:1267-1295); 0 restores the value of rp_chiqw_open. */
 int rp_chiqw_set_window(rp_handle h, double ecut_for_eps, int32_t n_vir);
 
+/* FP32 accumulation span (in K, a multiple of 32) of the split-BF16 contraction
+ * for the q points run after this call: the FP32 accumulators are added into
+ * FP64 every span K.  Shorter spans lower the accumulation error at the cost
+ * of more read-modify-writes of the FP64 partial tiles.  0 (the default):
+ * 8192, shortened automatically for a q whose eps is ill-conditioned.  No
+ * counterpart in the reference (its accumulation is FP64). */
+int rp_chiqw_set_span(rp_handle h, int32_t span);
+
 /* Batched configuration, captured (no counterpart in the reference, which runs
  * one q per iteration of its q loop, chiqw.F:662-687; for callers that run
  * many q of one system).  rp_chiqw_batch_plan prepares all nq q points

@@ -57,6 +57,7 @@ def lib():
         L.ro_ttrhdrn_chi0.argtypes = [D, D, I32, I32, I32, P, P, P, D, D, D, D, I32, P, P]
         L.ro_ttrhdrn_simple.argtypes = [D, D, I32, I32, I32, P, P, P, P]
         L.ro_ttrhdrn_calc.argtypes = [D, D, P, P, I32, P, P]
+        L.ro_tet_sub.argtypes = [I32, P, I32, P, P]
         L.ro_dft3.argtypes = [I32, I32, I32, I32, P, P]
         L.ro_pairdens.argtypes = [I32, P, P, I32, P, P, I32, I32, I32, I32, P, P, P]
         L.ro_vmab.argtypes = [I32, P, P, P, P, P, P, P]
@@ -118,6 +119,7 @@ def ref():
         R.rpref_vmab.argtypes = [I32, I32, P, P, P, P, I32, P, P, P, P]
         R.rpref_vmaa.argtypes = [I32, I32, P, P, P, P, I32, P, P, P, P, P]
         R.rpref_eps_rpa.argtypes = [I32, I32, P, D, D, D, P, P, P, P, P]
+        R.rpref_tet_sub.argtypes = [I32, P, I32, P, P]
         R.rpref_eps_rpa_q0.argtypes = [I32, I32, P, P, P, P, P, I32, D, D, D, D, P]
         _ref = R
     return _ref

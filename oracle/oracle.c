@@ -369,6 +369,30 @@ static ro_z sub2normal(ro_z g0, ro_z g1, ro_z g2, ro_z g3, ro_z e) {
   return x;
 }
 
+/* The closed forms above with the reference's argument lists (g[0..3] as its
+ * g0..g3; the unused ones ignored): which as in the checker's rpref_tet_sub. */
+int ro_tet_sub(int which, const ro_z *g, int ne, const ro_z *em, ro_z *x) {
+  for (int ie = 0; ie < ne; ++ie) {
+    ro_z e = em[ie];
+    switch (which) {
+    case 0: x[ie] = 1.0 / ((g[0] + e) * 6.0); break;
+    case 1: x[ie] = sub1g30g21(g[0], g[1], e); break;
+    case 2: x[ie] = sub1g30g20(g[0], g[1], e); break;
+    case 3: x[ie] = sub1g30(g[0], g[1], g[2], e); break;
+    case 4: x[ie] = sub1normal(g[0], g[1], g[2], g[3], e); break;
+    case 5: x[ie] = sub2simple(g[0], e); break;
+    case 6: x[ie] = sub2g30g20(g[0], g[1], e); break;
+    case 7: x[ie] = sub2g30g20(g[1], g[0], e); break; /* sub2g30g21(g0,g1,..) */
+    case 8: x[ie] = sub2g30g10(g[0], g[2], e); break;
+    case 9: x[ie] = sub2g10(g[0], g[2], g[3], e); break;
+    case 10: x[ie] = sub2g30(g[0], g[1], g[2], e); break;
+    case 11: x[ie] = sub2normal(g[0], g[1], g[2], g[3], e); break;
+    default: return -1;
+    }
+  }
+  return 0;
+}
+
 /* chk1 (m_tetrahedron_20170325.F:457-503): degeneracy classes of 4 vertices.
  * Vertices are compared in the reference's rotated order p = mod(i,4)+1. */
 static void chk1(double dmna, double dmnr, const ro_z *g, int *ia2, int *ia3,

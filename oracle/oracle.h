@@ -71,6 +71,11 @@ void ro_ttrhdrn_simple(double dmna, double dmnr, int n1, int n2, int n3,
                        const int32_t *idx, const ro_z *fk, const ro_z *gk, ro_z *xo);
 /* m_tetrahedron_20170325.F:209-302 for one tetrahedron: fk[4], gk[4] in,
  * xo[ie + ne*v] accumulated (v = 0..3). Exposed for unit tests. */
+/* One tetrahedron closed form (m_tetrahedron_20170325.F:304-451) at ne
+ * frequencies, g[0..3] the reference's g0..g3; which: 0 sub1simple,
+ * 1 sub1g30g21, 2 sub1g30g20, 3 sub1g30, 4 sub1normal, 5 sub2simple,
+ * 6 sub2g30g20, 7 sub2g30g21, 8 sub2g30g10, 9 sub2g10, 10 sub2g30, 11 sub2normal. */
+int ro_tet_sub(int which, const ro_z *g, int ne, const ro_z *em, ro_z *x);
 void ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk,
                      int ne, const ro_z *em, ro_z *xo);
 

@@ -259,4 +259,28 @@ contains
     close(u)
   end subroutine
 
+  ! One closed form of the generalized tetrahedron method (the public
+  ! ttrhdrn_sub* of m_tetrahedron_20170325.F:304-451) at ne frequencies:
+  ! which = 0 sub1simple, 1 sub1g30g21, 2 sub1g30g20, 3 sub1g30, 4 sub1normal,
+  ! 5 sub2simple, 6 sub2g30g20, 7 sub2g30g21, 8 sub2g30g10, 9 sub2g10,
+  ! 10 sub2g30, 11 sub2normal.
+  subroutine rpref_tet_sub(which, g, ne, em, x) bind(C, name="rpref_tet_sub")
+    integer(c_int), value :: which, ne
+    complex(c_double_complex) :: g(4), em(ne), x(ne)
+    select case (which)
+    case (0); call ttrhdrn_sub1simple(g(1), g(2), g(3), g(4), ne, em, x)
+    case (1); call ttrhdrn_sub1g30g21(g(1), g(2), g(3), g(4), ne, em, x)
+    case (2); call ttrhdrn_sub1g30g20(g(1), g(2), g(3), g(4), ne, em, x)
+    case (3); call ttrhdrn_sub1g30(g(1), g(2), g(3), g(4), ne, em, x)
+    case (4); call ttrhdrn_sub1normal(g(1), g(2), g(3), g(4), ne, em, x)
+    case (5); call ttrhdrn_sub2simple(g(1), g(2), g(3), g(4), ne, em, x)
+    case (6); call ttrhdrn_sub2g30g20(g(1), g(2), g(3), g(4), ne, em, x)
+    case (7); call ttrhdrn_sub2g30g21(g(1), g(2), g(3), g(4), ne, em, x)
+    case (8); call ttrhdrn_sub2g30g10(g(1), g(2), g(3), g(4), ne, em, x)
+    case (9); call ttrhdrn_sub2g10(g(1), g(2), g(3), g(4), ne, em, x)
+    case (10); call ttrhdrn_sub2g30(g(1), g(2), g(3), g(4), ne, em, x)
+    case (11); call ttrhdrn_sub2normal(g(1), g(2), g(3), g(4), ne, em, x)
+    end select
+  end subroutine
+
 end module rpref_shim

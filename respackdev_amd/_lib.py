@@ -30,7 +30,7 @@ EXPORTS = [
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
     "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run",
     "rp_chiqw_batch_plan", "rp_chiqw_batch_run", "rp_chiqw_batch_result", "rp_chiqw_batch_free",
-    "rp_chiqw_autotune", "rp_chiqw_tune_count", "rp_chiqw_tune_info",
+    "rp_chiqw_autotune", "rp_chiqw_tune_count", "rp_chiqw_tune_info", "rp_chiqw_set_span",
 ]
 
 
@@ -79,6 +79,7 @@ def lib():
         L.rp_chiqw_batch_result.argtypes = [P, I32, P, P, P, I32, P]
         L.rp_chiqw_batch_free.argtypes = [P]
         L.rp_chiqw_autotune.argtypes = [P, I32]
+        L.rp_chiqw_set_span.argtypes = [P, I32]
         L.rp_chiqw_tune_count.argtypes = [P, P]
         L.rp_chiqw_tune_info.argtypes = [P, I32, P, P]
         I64, U64 = C.c_int64, C.c_uint64

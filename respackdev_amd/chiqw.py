@@ -288,6 +288,10 @@ class Chiqw:
     def batch_free(self) -> None:
         check(_lib.lib().rp_chiqw_batch_free(self._h))
 
+    def set_span(self, span: int = 0) -> None:
+        """FP32 accumulation span (K) of the split-BF16 contraction (0: automatic)."""
+        check(_lib.lib().rp_chiqw_set_span(self._h, int(span)))
+
     def set_window(self, ecut_for_eps: float = 0.0, n_vir: int = 0):
         """Per-q chi cutoff (Ry) and empty-band window for the q points prepared
         next (0: the values of open)."""

@@ -41,6 +41,7 @@ void swap_q(rp_ctx* h, QSlot& s) {
   std::swap(h->kpad, s.kpad);
   std::swap(h->ecut_eps_win, s.ecut_eps_win);
   std::swap(h->n_vir_win, s.n_vir_win);
+  std::swap(h->q_span, s.q_span);
   h->d_lg.swap(s.d_lg);
   h->d_ikq.swap(s.d_ikq);
   h->d_shift.swap(s.d_shift);
@@ -176,6 +177,14 @@ int rp_chiqw_batch_plan(rp_handle h, int32_t nq, const double* qs, const double*
         h->have_chi = true;
         issue_dyson(h);
         check_status(h, "batch: Dyson step");
+        // the span this q's eps^-1 calls for (split-BF16): the graph uses it
+        if (const int sp = adapt_span(h, precision, b->slots[i].launch.span)) {
+          b->slots[i].launch.span = sp;
+          h->q_span = sp;
+          issue_accumulate(h, precision, b->slots[i].launch, false);
+          issue_dyson(h);
+          check_status(h, "batch: Dyson step");
+        }
         scope.leave();
       }
       // 3. capture every q's launches into one graph

@@ -164,10 +164,8 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
       double dd = std::fabs((h->b[d][0] * h->a[d][0] + h->b[d][1] * h->a[d][1] + h->b[d][2] * h->a[d][2]) / an);
       h->nw[d] = algn235((int)(qwf / dd) + 1);
       RP_CHECK(dft_size_supported(h->nw[d]),
-               "FFT grid dimension " + std::to_string(h->nw[d]) + " is not supported by the LDS transform");
+               "FFT grid dimension " + std::to_string(h->nw[d]) + " is not supported (2,3,5-smooth sizes up to 32)");
     }
-    RP_CHECK((long)h->nw[0] * h->nw[1] * h->nw[2] <= max_grid_points(),
-             "FFT grid too large for the LDS-resident transform");
     // ---- symmetry (rginv by the adjugate: integer matrices of det +-1)
     std::vector<double> rginv(9 * (size_t)nsym);
     for (int op = 0; op < nsym; ++op) {
@@ -342,8 +340,15 @@ int rp_chiqw_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp,
     h->d_ur.reserve((size_t)nb * ntk * nr);
     h->d_tw.reserve(twiddle_entries());
     launch_twiddles(h->d_tw.p, s);
-    launch_band_grids(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, ntk, ntg, nb, h->nw[0], h->nw[1], h->nw[2],
-                      h->d_tw.p, h->d_ur.p, s);
+    {
+      // grids too large for the LDS are transformed over HBM in batches
+      DevBuf<double2> scratch;
+      const size_t se = band_grids_scratch_elems(h->nw[0], h->nw[1], h->nw[2], (size_t)nb * ntk);
+      scratch.reserve(se);
+      launch_band_grids(h->d_coef.p, h->d_kg0.p, h->d_ng0.p, ntk, ntg, nb, h->nw[0], h->nw[1],
+                        h->nw[2], h->d_tw.p, h->d_ur.p, scratch.p, se, s);
+      RP_HIP(hipStreamSynchronize(s));
+    }
     h->d_eband.upload(h->eband, s);
     h->d_pband.upload((const double2*)h->pband.data(), (size_t)nb * ntk, s);
     h->d_imt.upload(h->imt, s);
@@ -417,6 +422,7 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
     const bool same = h->qready && q[0] == h->q[0] && q[1] == h->q[1] && q[2] == h->q[2];
     if (!same) {
       h->qready = h->have_A = h->have_w = h->have_chi = false;
+      h->q_span = 0;
       std::memcpy(h->q, q, sizeof q);
       h->q0 = q[0] == 0.0 && q[1] == 0.0 && q[2] == 0.0;  // chiqw.F:1257
       h->lg = make_kg0(h->b, h->a, h->ecut_eps_win > 0.0 ? h->ecut_eps_win : h->ecut_eps, q);
@@ -479,6 +485,10 @@ int rp_chiqw_prepare_q(rp_handle h, const double* qin, int32_t* ng, int32_t* lgo
       h->d_pvir.upload(h->pvir, s);
       if (!h->intra.empty()) h->d_intra.upload(h->intra, s);
       h->pdplan = pairdens_plan(h->lg, h->nw[0], h->nw[1], h->nw[2]);
+      RP_CHECK(pruned_transform_lds_bytes(h->nw[0], h->nw[1], h->nw[2], h->pdplan[0]) <=
+                   lds_budget_bytes(),
+               "the chi sphere of this q spans " + std::to_string(h->pdplan[0]) +
+                   " z planes of the FFT grid: more than the LDS-resident pair-density transform holds");
       h->d_pdplan.upload(h->pdplan, s);
       h->d_qn.upload(h->qn, s);
       RP_HIP(hipStreamSynchronize(s));
@@ -501,6 +511,15 @@ int rp_chiqw_set_window(rp_handle h, double ecut_for_eps, int32_t n_vir) {
   });
 }
 
+int rp_chiqw_set_span(rp_handle h, int32_t span) {
+  return guarded([&] {
+    RP_CHECK(h, "rp_chiqw_set_span: null handle");
+    RP_CHECK(span == 0 || (span >= 32 && span % 32 == 0),
+             "rp_chiqw_set_span: the span is 0 or a positive multiple of 32");
+    h->span = span;
+  });
+}
+
 int rp_chiqw_npair(rp_handle h, int32_t* npair) {
   return guarded([&] {
     RP_CHECK(h && h->qready, "no q prepared");
@@ -512,14 +531,18 @@ int rp_chiqw_npair(rp_handle h, int32_t* npair) {
 
 namespace rp {
 
+void throw_status(int st, const std::string& what) {
+  if (st & 1) throw Error(what + ": a tetrahedron cut by E_F without partially occupied bands");
+  if (st & 8) throw Error(what + ": too many sub-tetrahedra in a Fermi-surface cut");
+  if (st & 2) throw Error(what + ": bad tetrahedron degeneracy class (errexec)");
+  if (st & 4) throw Error(what + ": singular dielectric matrix (zero pivot)");
+}
+
 void check_status(rp_ctx* h, const char* what) {
   int st = 0;
   RP_HIP(hipMemcpyAsync(&st, h->d_status.p, sizeof(int), hipMemcpyDeviceToHost, h->stream));
   RP_HIP(hipStreamSynchronize(h->stream));
-  if (st & 1) throw Error(std::string(what) + ": a tetrahedron cut by E_F without partially occupied bands");
-  if (st & 8) throw Error(std::string(what) + ": too many sub-tetrahedra in a Fermi-surface cut");
-  if (st & 2) throw Error(std::string(what) + ": bad tetrahedron degeneracy class (errexec)");
-  if (st & 4) throw Error(std::string(what) + ": singular dielectric matrix (zero pivot)");
+  throw_status(st, what);
 }
 
 // ------------------------------------------------------------ stage bodies
@@ -602,6 +625,7 @@ void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events) {
     if (events) RP_HIP(hipEventRecord(h->gev[2 * ix], s));
     if (prec == RP_PREC_BF16X3)
       launch_chi_bf16x3(h->d_Ao.p, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, c.nsplit, c.nwg,
+                        c.span > 0 ? c.span : bf16x3_default_span(),
                         h->d_part.p, h->d_sync.p, s);
     else
       launch_chi_gemm(prec, A, h->lda, h->d_w.p, h->kpad, h->kpad, ng, nen, c.nsplit, h->d_part.p, s);
@@ -689,6 +713,30 @@ static double time_launch(rp_ctx* h, int prec, const ChiLaunch& c) {
 }
 
 ChiLaunch chi_launch(rp_ctx* h, int prec) {
+  ChiLaunch c = chi_launch_tuned(h, prec);
+  if (prec == RP_PREC_BF16X3) c.span = h->span > 0 ? h->span : h->q_span;
+  return c;
+}
+
+double epsinv_kappa(rp_ctx* h) {
+  const int nb = h->nbatch();
+  h->d_kappa.reserve(nb);
+  launch_norm1(h->d_eps.p, h->ng, nb, h->d_kappa.p, h->stream);
+  std::vector<double> k(nb);
+  RP_HIP(hipMemcpyAsync(k.data(), h->d_kappa.p, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  RP_HIP(hipStreamSynchronize(h->stream));
+  return *std::max_element(k.begin(), k.end());
+}
+
+int adapt_span(rp_ctx* h, int prec, int span_used) {
+  if (prec != RP_PREC_BF16X3 || h->span > 0) return 0;
+  h->last_kappa = epsinv_kappa(h);
+  const int sp = bf16x3_span_for_kappa(h->last_kappa);
+  const int used = span_used > 0 ? span_used : bf16x3_default_span();
+  return sp < used ? sp : 0;
+}
+
+ChiLaunch chi_launch_tuned(rp_ctx* h, int prec) {
   const int ndir = h->q0 ? 3 : 1;
   for (const auto& t : h->tune)
     if (t.prec == prec && t.ng == h->ng && t.nen == h->nen && t.kpad == h->kpad && t.ndir == ndir)
@@ -821,6 +869,15 @@ int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_o
     if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
     RP_HIP(hipEventRecord(h->ev[3], s));
     if (rp_dyson(h, epsinv_out)) throw Error(g_err);
+    // an ill-conditioned eps amplifies the FP32 accumulation error of the
+    // split-BF16 contraction: run it again with the span its eps^-1 calls for
+    if (const int sp = adapt_span(h, precision, h->q_span)) {
+      h->q_span = sp;
+      RP_HIP(hipEventRecord(h->ev[2], s));
+      if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
+      RP_HIP(hipEventRecord(h->ev[3], s));
+      if (rp_dyson(h, epsinv_out)) throw Error(g_err);
+    }
     RP_HIP(hipEventRecord(h->ev[4], s));
     RP_HIP(hipEventSynchronize(h->ev[4]));
     if (timings) {

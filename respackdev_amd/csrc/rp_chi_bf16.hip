@@ -31,11 +31,14 @@
 // K range at the same time, so the operand chunks they share (5 row blocks and
 // 5 column blocks serve all 25 tiles at NG ~ 600) come from that XCD's L2
 // instead of HBM.
-// Every kFlushChunks chunks the FP32 accumulators are added into the FP64
-// partial tile of the item in HBM, which the workgroup owns exclusively (FP64
-// accumulation across K blocks of 8192, FP32 within; the first flush of an
-// item stores).  At C2 shapes a span of 4096 K costs 4% more time (the flush
-// read-modify-writes the tile) for chi errors of 4.6e-7 instead of 6.5e-7.
+// Every flush_chunks chunks (the launch's accumulation span, kFlushChunks =
+// 8192 K by default) the FP32 accumulators are added into the FP64 partial
+// tile of the item in HBM, which the workgroup owns exclusively (FP64
+// accumulation across K blocks, FP32 within; the first flush of an item
+// stores).  At the precision workload's shapes a span of 4096 K costs 4% more
+// time (the flush read-modify-writes the tile) for chi errors of 4.6e-7
+// instead of 6.5e-7; the host shortens the span for a q whose eps is
+// ill-conditioned (the FP32 accumulation error is then what eps^-1 amplifies).
 #include <hip/hip_bfloat16.h>
 
 #include <algorithm>
@@ -54,7 +57,7 @@ constexpr int kTM = 128, kTN = 128;   // tile (G x G')
 constexpr int kKC = 32;               // complex K per chunk
 constexpr int kK8 = kKC / 8;          // octets per row per chunk
 constexpr int kRowBytes = kK8 * 64 + 16;  // padded LDS row (bytes)
-constexpr int kFlushChunks = 256;     // FP32 accumulation span: 8192 K
+constexpr int kFlushChunks = 256;     // default FP32 accumulation span: 8192 K
 constexpr int kXcd = 8;               // XCDs of gfx950 (workgroup b -> XCD b % 8)
 constexpr int kSyncChunks = 32;       // chunks between rendezvous of an XCD's workgroups
 constexpr int kSyncSpin = 512;        // bound on the polls of one rendezvous
@@ -130,7 +133,7 @@ __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ df, size_t ldd,
              size_t chunks_per_split, size_t nchunks, int ng, int ntile, int nen, int nsplit,
-             double2* __restrict__ out, unsigned* __restrict__ sync) {
+             int flush_chunks, double2* __restrict__ out, unsigned* __restrict__ sync) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // two buffers of [left tile | right tile], each kTM (kTN) rows of kRowBytes
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
@@ -329,7 +332,7 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
       __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // VALU
     }
     __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);  // LDS writes
-    if (++since == kFlushChunks) {
+    if (++since == flush_chunks) {
       since = 0;
       flush();
     }
@@ -397,9 +400,26 @@ int bf16x3_workgroups(int ncu) { return std::max(kXcd, ncu / kXcd * kXcd); }
 
 size_t bf16x3_sync_words() { return (size_t)kXcd * kSyncStride; }
 
+int bf16x3_default_span() { return kFlushChunks * kKC; }
+
+// The FP32 accumulation error of chi, amplified in eps^-1 by about the
+// condition of eps (~ |eps^-1|_1, as eps is the identity plus a modest term),
+// sets the span.  Measured on the 27 q of the batched workload (NG 434-705):
+// the eps^-1 error against FP64 stays below c(span) kappa with c = 1.7e-7
+// (8192 K), 1.2e-7 (4096), 7.5e-8 (2048), 3.2e-8 (1024).  The spans below keep
+// c kappa under 5e-6, half the mixed path's 1e-5 bar.
+int bf16x3_span_for_kappa(double kappa) {
+  if (kappa <= 25.0) return 8192;
+  if (kappa <= 40.0) return 4096;
+  if (kappa <= 60.0) return 2048;
+  if (kappa <= 150.0) return 1024;
+  return 512;
+}
+
 void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd, size_t kpad, int ng,
-                       int nen, int nsplit, int nwg, double2* partial, unsigned* sync,
+                       int nen, int nsplit, int nwg, int span, double2* partial, unsigned* sync,
                        hipStream_t s) {
+  RP_CHECK(span >= kKC && span % kKC == 0, "bf16x3 contraction: span must be a multiple of 32 K");
   RP_CHECK(lda % kTM == 0, "bf16x3 contraction needs lda a multiple of 128");
   RP_CHECK(kpad % kKC == 0, "bf16x3 contraction needs K padded to a multiple of 32");
   RP_CHECK(nwg % kXcd == 0 && nwg > 0, "bf16x3 contraction needs a multiple of 8 workgroups");
@@ -412,7 +432,7 @@ void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd,
   RP_HIP(hipFuncSetAttribute((const void*)k_chi_bf16x3, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
   hipLaunchKernelGGL(k_chi_bf16x3, dim3(nwg), dim3(512), lds, s, (const u4v*)Ao, lda, df, ldd, per,
-                     nchunks, ng, ntile, nen, nsplit, partial, sync);
+                     nchunks, ng, ntile, nen, nsplit, span / kKC, partial, sync);
   RP_HIP(hipGetLastError());
 }
 

@@ -520,6 +520,36 @@ __global__ void __launch_bounds__(256) k_gj_colswap(double2* __restrict__ a, int
   }
 }
 
+// kappa[b] = max_j sum_i |a_b(i, j)| (the 1-norm of each matrix of the batch).
+// grid: nbatch blocks of 256.
+__global__ void __launch_bounds__(256) k_norm1(const double2* __restrict__ a, int n,
+                                               double* __restrict__ kappa) {
+  const int b = blockIdx.x;
+  const double2* m = a + (size_t)b * n * n;
+  double best = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double2 z = m[i + (size_t)n * j];
+      s += sqrt(z.x * z.x + z.y * z.y);
+    }
+    best = fmax(best, s);
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) kappa[b] = red[0];
+}
+
+void launch_norm1(const double2* a, int n, int nbatch, double* kappa, hipStream_t s) {
+  hipLaunchKernelGGL(k_norm1, dim3(nbatch), dim3(256), 0, s, a, n, kappa);
+  RP_HIP(hipGetLastError());
+}
+
 void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
                       int g0, const double* wd, const double2* em, double delt, double2* eps,
                       hipStream_t s) {

@@ -59,6 +59,7 @@ struct Mlp;  // band-tail closure network (rp_mlp_state.h)
 struct ChiLaunch {
   int nsplit = 1;
   int nwg = 0;
+  int span = 0;  // split-BF16: FP32 accumulation span in K (0: the default)
 };
 
 // One autotuned contraction shape and the launch chosen for it.
@@ -85,6 +86,7 @@ struct QSlot {
   DevBuf<double> d_qn;
   DevBuf<double2> d_eps;  // eps^-1 of this q (a batched run's output)
   ChiLaunch launch;
+  int q_span = 0;
 };
 
 // Work buffers of the stages (shared by all q of one launch sequence).
@@ -172,6 +174,10 @@ struct rp_ctx {
   // read-back, buffers must already be large enough
   bool capturing = false;
   int autotune = 0;              // time the contraction's launch candidates per shape
+  int span = 0;                  // split-BF16 FP32 accumulation span (K) set by the caller, 0: automatic
+  int q_span = 0;                // the span chosen for the current q (0: the default)
+  double last_kappa = 0;         // |eps^-1|_1 of the last q checked by adapt_span
+  DevBuf<double> d_kappa;
   std::vector<rp::TuneEntry> tune;
   std::unique_ptr<rp::Batch> batch;
 
@@ -190,6 +196,8 @@ struct rp_ctx {
 namespace rp {
 // Stage bodies (rp_capi.cpp): enqueue only, on h->stream.
 void check_status(rp_ctx* h, const char* what);
+// Throws the error a non-zero status word of the kernels stands for.
+void throw_status(int st, const std::string& what);
 void issue_pairdens(rp_ctx* h);
 void issue_weights(rp_ctx* h);
 void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events);
@@ -198,6 +206,14 @@ ChiLaunch chi_launch_heuristic(const rp_ctx* h, int prec);
 // The launch for the current q: the tuned one for its shape, tuning it now
 // when autotuning is on and the inputs are ready, else the heuristic.
 ChiLaunch chi_launch(rp_ctx* h, int prec);
+// The same without the span of the caller or of the q.
+ChiLaunch chi_launch_tuned(rp_ctx* h, int prec);
+// max over the batch of |eps^-1|_1 of the current q (after the Dyson step)
+double epsinv_kappa(rp_ctx* h);
+// The split-BF16 span the current q's eps^-1 calls for (bf16x3_span_for_kappa)
+// when it is shorter than span_used (0: the default), else 0; automatic spans
+// only (no caller's span), split-BF16 only.
+int adapt_span(rp_ctx* h, int prec, int span_used);
 void swap_q(rp_ctx* h, QSlot& s);
 void swap_scratch(rp_ctx* h, Scratch& s);
 }  // namespace rp

@@ -11,12 +11,22 @@ namespace rp {
 
 // rp_pairdens.hip
 bool dft_size_supported(int n);
-int max_grid_points();
+// LDS one workgroup may use (the bound of the LDS-resident transforms)
+size_t lds_budget_bytes();
 size_t twiddle_entries();
 void launch_twiddles(double2* tab, hipStream_t s);
+// Elements of FP64 scratch the band grids of ngrids (band, k) need when the
+// grid is too large for the LDS (0 when it fits): batches of at most 1 GiB.
+size_t band_grids_scratch_elems(int nx, int ny, int nz, size_t ngrids);
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                        int nband, int nx, int ny, int nz, const double2* twtab, float2* ur,
-                       hipStream_t s);
+                       double2* scratch, size_t scratch_elems, hipStream_t s);
+// In-place 3D DFT (z, y, x) of nb grids in HBM as line passes; sign +1: e^{+i}
+// unscaled (fft3_bw), -1: e^{-i} unscaled.
+void launch_line_dft3(double2* g, long long nb, int nx, int ny, int nz, int sign,
+                      const double2* tab, hipStream_t s);
+// LDS of the pruned forward transform for nzz kept z planes
+size_t pruned_transform_lds_bytes(int nx, int ny, int nz, int nzz);
 // Plan of the pruned forward transform for a chi sphere lg (see rp_pairdens.hip);
 // plan[0] is the number of kept z planes.
 std::vector<int> pairdens_plan(const std::vector<int>& lg, int nx, int ny, int nz);
@@ -26,6 +36,27 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
                                   const int* ikq, const int* shift, int nx, int ny, int nz,
                                   const int* plan, int nzz, int ng, size_t lda, int nK,
                                   const double2* twtab, float2* A, hipStream_t s);
+// calc_int: P[q][p][r] = sum_k wk[k] conj(u_iw,k+q(r) e^{-i 2pi r.dG/N}) u_jw,k(r)
+// (FP64) for the q of ikq[nq][ntk] / shift[nq][ntk][3]; full: every pair
+// p = iw + nwf*jw, else the diagonal p = w.  wk may be null (1).
+void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int ny, int nz,
+                      const int* ikq, const int* shift, const double2* wk, int nq, double2* P,
+                      hipStream_t s);
+// rho[rho_off[q] + p*ngpsi[q] + i] from P of q = q_offset .. q_offset+nq-1:
+// pruned forward transform read at -G of the psi sphere (plans at plan_off),
+// / |q+G| (qg at qg_off), / NTK, zero at g0s[q].
+void launch_rho_from_grid(const double2* P, int npair, int nx, int ny, int nz, const int* plan,
+                          const long long* plan_off, int max_nzz, const double* qg,
+                          const long long* qg_off, const int* g0s, const int* ngpsi,
+                          const long long* rho_off, int q_offset, int nq, int ntk,
+                          const double2* twtab, double2* rho, hipStream_t s);
+// The same from grids transformed over HBM (grids beyond the LDS): P is
+// transformed in place, then the psi-sphere points (lg: 3 ints per point at
+// the offsets off of qg) are gathered.
+void launch_rho_from_grid_hbm(double2* P, int npair, int nx, int ny, int nz, const int* lg,
+                              const double* qg, const long long* off, const int* g0s,
+                              const int* ngpsi, const long long* rho_off, int q_offset, int nq,
+                              int ntk, const double2* twtab, double2* rho, hipStream_t s);
 void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                      const int* pocc, const int* pvir, const double* eband, const double* bvec,
                      double delt, int g0, int ng, size_t lda, size_t rows, int nK, const float2* A,
@@ -63,14 +94,21 @@ void launch_weights_to_f32(const double2* dw, size_t n, float2* df, hipStream_t 
 int bf16x3_nsplit(int ng, int nen, size_t kpad, int nwg);
 int bf16x3_workgroups(int ncu);
 size_t bf16x3_sync_words();
+// default FP32 accumulation span (K) of the split-BF16 contraction
+int bf16x3_default_span();
+// the span that keeps eps^-1 within the mixed path's bar for a q whose eps^-1
+// has 1-norm kappa (the amplification of chi's error)
+int bf16x3_span_for_kappa(double kappa);
 void launch_chi_bf16x3(const void* Ao, size_t lda, const float2* df, size_t ldd, size_t kpad, int ng,
-                       int nen, int nsplit, int nwg, double2* partial, unsigned* sync,
+                       int nen, int nsplit, int nwg, int span, double2* partial, unsigned* sync,
                        hipStream_t s);
 
 // rp_dyson.hip
 void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, int nen, int q0,
                       int g0, const double* wd, const double2* em, double delt, double2* eps,
                       hipStream_t s);
+// kappa[b] = the 1-norm of matrix b of the batch (max column sum of |a|)
+void launch_norm1(const double2* a, int n, int nbatch, double* kappa, hipStream_t s);
 size_t gauss_jordan_work_ints(int nbatch);
 void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
                          hipStream_t s);
@@ -78,8 +116,6 @@ void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, in
 // rp_w3d.hip
 int w3d_max_wannier();
 int w3d_max_eps_g();
-void launch_w_rho(const float2* A, size_t lda, int ntk, int nwf, int ngpsi, const double* qg,
-                  int g0, double2* rho, hipStream_t s);
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s);
 void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
@@ -94,8 +130,6 @@ void launch_j_contract(const float2* eps, const long long* eps_off, const double
                        double2* xg, const double2* shell, int ntq, int npair, int nen,
                        double2* funcw, hipStream_t s);
 size_t j_contract_scratch(size_t sum_nge, int npair);
-void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
-                  int g0, int q0, const double2* pfk, double2* rho, hipStream_t s);
 void launch_j_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int npair, double2* func, double2* shell, hipStream_t s);
 void launch_j_sum(const double2* func, const double2* funcw, int ntq, int nwf, int nen, int r0,

@@ -10,12 +10,16 @@
 // (fft3_fw), read at -G for the chi sphere and rounded to single precision
 // (chiqw_sub.F:192-202): A[K][lda] complex64, K = pair*NTK + k.
 //
-// Both transforms are LDS-resident: the FP64 grid (<= 10000 points) lives in
-// one workgroup's LDS and each axis is done in place as independent per-thread
-// line DFTs (the axis lengths are the reference's 2,3,5-smooth sizes, <= 32),
-// so HBM sees one read of the inputs and one write of the outputs per
-// transform.  The forward transform of stage 2 is pruned to the outputs the
-// sphere needs (below).
+// Both transforms are LDS-resident where the FP64 grid fits one workgroup's
+// LDS: each axis is done in place as independent per-thread line DFTs (the
+// axis lengths are the reference's 2,3,5-smooth sizes, <= 32), so HBM sees one
+// read of the inputs and one write of the outputs per transform.  The forward
+// transform of stage 2 is pruned to the outputs the sphere needs (below), so
+// it holds only the z planes with sphere points.  Stage 1 on grids too large
+// for the LDS (e.g. 24^3 = 13824 points) runs as three passes of line DFTs
+// over HBM (k_line_dft).
+#include <algorithm>
+#include <string>
 #include <vector>
 
 #include "rp_common.h"
@@ -24,9 +28,9 @@
 
 namespace rp {
 
-// Largest grid the LDS-resident transform holds (the FP64 grid within the
-// 160 KB of LDS a gfx950 workgroup may use): 20*20*25 points.
-constexpr int kMaxGrid = 10000;
+// LDS a gfx950 workgroup may use; the FP64 grid of an LDS-resident transform
+// must fit in it (10240 points).
+constexpr size_t kLdsBytes = 160 * 1024;
 
 // Twiddle tables in HBM for every line length N = 1..32, one row per output
 // index: T_N[k][j] = w^{(jk) mod N} with w = exp(-2 pi i/N) (forward table) or
@@ -37,7 +41,7 @@ constexpr int kMaxGrid = 10000;
 constexpr int kTwMax = 32;
 constexpr int kTwHalf = (kTwMax - 1) * kTwMax * (2 * kTwMax - 1) / 6 + kTwMax * kTwMax;
 __host__ __device__ constexpr int tw_off(int n) { return (n - 1) * n * (2 * n - 1) / 6; }
-__device__ __forceinline__ const double2* twiddles_of(const double2* tab, int n, int sign) {
+__host__ __device__ __forceinline__ const double2* twiddles_of(const double2* tab, int n, int sign) {
   return tab + (sign > 0 ? kTwHalf : 0) + tw_off(n);
 }
 __global__ void k_fill_twiddles(double2* __restrict__ tab) {
@@ -119,7 +123,7 @@ bool dft_size_supported(int n) {
       return false;
   }
 }
-int max_grid_points() { return kMaxGrid; }
+size_t lds_budget_bytes() { return kLdsBytes; }
 
 // z, then y, then x (the order of fft3_fw/fft3_bw, m_fft3d_20150826.f90:1598-
 // 1720), in place.  Ends synchronised.
@@ -162,6 +166,72 @@ __global__ void __launch_bounds__(256) k_band_grids(const double2* __restrict__ 
   float2* out = ur + ((size_t)b * ntk + k) * nr;
   for (int r = threadIdx.x; r < nr; r += blockDim.x)
     out[r] = make_float2((float)g[r].x, (float)g[r].y);  // CMPLX(): single precision
+}
+
+// ------------------------------------------ grids beyond the LDS: HBM passes
+// In place along one axis of nb grids g[b][r] (complex FP64): every line is
+// one thread's, its N points in registers; sign as the twiddle table tw.
+template <int N>
+__global__ void __launch_bounds__(256) k_line_dft(double2* __restrict__ g, long long nlines, int axis,
+                                                  int nx, int ny, int nz,
+                                                  const double2* __restrict__ tw) {
+  const long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (idx >= nlines) return;
+  const int nr = nx * ny * nz, per = nr / N;
+  const long long b = idx / per;
+  const int l = (int)(idx % per);
+  int base, stride;
+  if (axis == 0) { base = nx * l; stride = 1; }
+  else if (axis == 1) { base = (l % nx) + nx * ny * (l / nx); stride = nx; }
+  else { base = l; stride = nx * ny; }
+  double2* p = g + (size_t)b * nr + base;
+  double2 v[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = p[j * stride];
+  if constexpr (N <= 16) {
+    double2 y[N];
+    fft_reg<N, 1, N>(v, y, tw + N);  // row 1 of the table: w^j
+#pragma unroll
+    for (int k = 0; k < N; ++k) p[k * stride] = y[k];
+  } else {
+#pragma unroll 1
+    for (int k = 0; k < N; ++k) {
+      const double2* row = tw + k * N;
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        const double2 w = row[j];
+        re = fma(v[j].x, w.x, re);
+        re = fma(-v[j].y, w.y, re);
+        im = fma(v[j].x, w.y, im);
+        im = fma(v[j].y, w.x, im);
+      }
+      p[k * stride] = make_double2(re, im);
+    }
+  }
+}
+
+// Coefficients of the grids (b, k) = gb0 .. gb0+nb-1 (flattened b*ntk + k)
+// scattered onto zeroed grids g[gb - gb0][r].  grid (ceil(ntg/256), nb).
+__global__ void __launch_bounds__(256) k_scatter_coef(const double2* __restrict__ coef,
+                                                      const int* __restrict__ kg,
+                                                      const int* __restrict__ ng0, int ntk, int ntg,
+                                                      int nx, int ny, int nz, long long gb0,
+                                                      double2* __restrict__ g) {
+  const long long gb = gb0 + blockIdx.y;
+  const int k = (int)(gb % ntk);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng0[k]) return;
+  const int* kk = kg + (size_t)k * ntg * 3;
+  const int r = wrap(kk[3 * i], nx) + nx * (wrap(kk[3 * i + 1], ny) + ny * wrap(kk[3 * i + 2], nz));
+  g[(size_t)blockIdx.y * nx * ny * nz + r] = coef[(size_t)gb * ntg + i];
+}
+
+// CMPLX(): the FP64 grids rounded to single precision into ur.
+__global__ void k_round_grids(const double2* __restrict__ g, size_t n, float2* __restrict__ ur) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    ur[i] = make_float2((float)g[i].x, (float)g[i].y);
 }
 
 // ----------------------------------------------- pruned forward transform
@@ -392,6 +462,165 @@ __global__ void __launch_bounds__(256) k_pair_densities_pruned(
     }
   }
 }
+// ------------------------------------- Wannier pair densities summed over k
+// calc_int needs the pair densities only through their k sums
+//   rho_p(q+G) = (1/NTK) sum_k w_k M_p,k(G) / |q+G|
+// (calc_intW.F:647-767 with w_k = 1; calc_intJ.F:618-720 with
+// w_k = e^{-i 2pi k.R}), and M_p,k(G) is the forward transform of the product
+// conj(u_iw,k+q(r) e^{-i 2pi r.dG_k/N}) u_jw,k(r).  The transform is linear, so
+// the products are summed over k on the real-space grid first (FP64), and one
+// pruned transform per (q, pair) replaces NTK of them.  The only arithmetic the
+// reference does differently is the single-precision rounding of each M_k
+// before the sum (CMPLX, calc_int_sub.F:214): about 1e-8 relative in rho.
+//
+// k_pair_rsum, grid (ceil(nr/256), nq, ngroups), 256 threads, one grid point r
+// per thread:
+//   FULL = false: pairs p = w (iw = jw = w), w < nwf <= 8, one group;
+//   FULL = true:  pairs p = iw + nwf*jw, group z takes jw in {2z, 2z + 1}.
+// P[q][p][r] complex FP64.  ikq[q][k], shift[q][k][3] (the umklapp vectors),
+// wk[k] (may be null: 1).
+constexpr int kRsumJB = 2;
+template <bool FULL>
+__global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur, int ntk, int nwf,
+                                                   int nx, int ny, int nz, const int* __restrict__ ikq,
+                                                   const int* __restrict__ shift,
+                                                   const double2* __restrict__ wk,
+                                                   double2* __restrict__ P) {
+  __shared__ double2 tab[3 * kTwMax];  // e^{-i 2pi m/n} per axis (n = nx, ny, nz)
+  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+  for (int i = threadIdx.x; i < nx + ny + nz; i += blockDim.x) {
+    const int ax = i < nx ? 0 : (i < nx + ny ? 1 : 2);
+    const int m = ax == 0 ? i : (ax == 1 ? i - nx : i - nx - ny);
+    const int n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+    double sn, cs;
+    sincos(tpi * ((double)m / (double)n), &sn, &cs);
+    tab[i] = make_double2(cs, -sn);
+  }
+  __syncthreads();
+  const int nr = nx * ny * nz;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = blockIdx.y, z = blockIdx.z;
+  const int npair = FULL ? nwf * nwf : nwf;
+  if (r >= nr) return;
+  const int x = r % nx, y = (r / nx) % ny, zz = r / (nx * ny);
+  constexpr int NA = FULL ? kRsumJB : 8;  // accumulator rows (jw, or w on the diagonal)
+  double2 acc[NA][FULL ? 8 : 1];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int b = 0; b < (FULL ? 8 : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
+  const int* kq_of = ikq + (size_t)q * ntk;
+  const int* sh_of = shift + (size_t)3 * q * ntk;
+  for (int k = 0; k < ntk; ++k) {
+    const int kq = kq_of[k];
+    const int d1 = sh_of[3 * k], d2 = sh_of[3 * k + 1], d3 = sh_of[3 * k + 2];
+    double2 ph = make_double2(1.0, 0.0);
+    if (d1 != 0 || d2 != 0 || d3 != 0) {  // uniform over the grid
+      const double2 px = tab[((d1 * x) % nx + nx) % nx];
+      const double2 py = tab[nx + ((d2 * y) % ny + ny) % ny];
+      const double2 pz = tab[nx + ny + ((d3 * zz) % nz + nz) % nz];
+      const double2 t = make_double2(px.x * py.x - px.y * py.y, px.x * py.y + px.y * py.x);
+      ph = make_double2(t.x * pz.x - t.y * pz.y, t.x * pz.y + t.y * pz.x);
+    }
+    const double2 wv = wk ? wk[k] : make_double2(1.0, 0.0);
+    // b_iw = u_iw,k+q(r) e^{-i 2pi r.dG/N} (FP64 product of the single-precision grid)
+    double2 bq[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      bq[i] = make_double2(0.0, 0.0);
+      if (i < nwf) {
+        const float2 v = ur[((size_t)i * ntk + kq) * nr + r];
+        bq[i] = make_double2((double)v.x * ph.x - (double)v.y * ph.y,
+                             (double)v.x * ph.y + (double)v.y * ph.x);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int jw = FULL ? z * kRsumJB + a : a;
+      if (jw >= nwf) continue;
+      const float2 v = ur[((size_t)jw * ntk + k) * nr + r];
+      // w_k u_jw,k(r)
+      const double2 aw = make_double2((double)v.x * wv.x - (double)v.y * wv.y,
+                                      (double)v.x * wv.y + (double)v.y * wv.x);
+      if constexpr (FULL) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {  // conj(b_iw) (w_k a_jw)
+          acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
+          acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
+        }
+      } else {
+        const double2 b = bq[a];  // a is a compile-time index here (unrolled)
+        acc[a][0].x += b.x * aw.x + b.y * aw.y;
+        acc[a][0].y += b.x * aw.y - b.y * aw.x;
+      }
+    }
+  }
+  double2* out = P + (size_t)q * npair * nr + r;
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int jw = FULL ? z * kRsumJB + a : a;
+    if (jw >= nwf) continue;
+    if constexpr (FULL) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (i < nwf) out[(size_t)(i + nwf * jw) * nr] = acc[a][i];
+    } else {
+      out[(size_t)jw * nr] = acc[a][0];
+    }
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void pd_zaxis_grid(const double2* __restrict__ src, int nx, int ny,
+                                              unsigned zmask, const double2* tw, double2* g) {
+  const int nl = nx * ny;
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    double2 v[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = src[l + nl * j];
+    masked_line<N, true>(v, zmask, tw, g + l, nl);
+  }
+}
+
+// rho_p(q+G) from the summed product grid P[q][p][r]: the pruned forward
+// transform (e^{-i}, 1/N) read at -G for the psi-sphere points of q, / |q+G|,
+// / NTK, G = 0 at q = 0 set to zero (calc_intW.F:720-767).  grid (npair, nq);
+// rho[rho_off[q] + p*ngpsi + i].
+template <int MAXN>
+__global__ void __launch_bounds__(256) k_rho_from_grid(
+    const double2* __restrict__ P, int npair, int nx, int ny, int nz, const int* __restrict__ plan,
+    const long long* __restrict__ plan_off, const double* __restrict__ qg,
+    const long long* __restrict__ qg_off, const int* __restrict__ g0s,
+    const int* __restrict__ ngpsi, const long long* __restrict__ rho_off, int q_offset, int ntk,
+    const double2* __restrict__ twtab, double2* __restrict__ rho) {
+  extern __shared__ double2 lds[];
+  const int p = blockIdx.x, ql = blockIdx.y, q = ql + q_offset;
+  const int* pl = plan + plan_off[q];
+  const int nzz = pl[0], nxl = pl[1];
+  const unsigned zmask = (unsigned)pl[2];
+  const int* ymask = pl + 4;
+  const int* xline = ymask + nzz;
+  const int* xmask = xline + nxl;
+  const int* pos = xmask + nxl;
+  double2* g = lds;  // [nzz][ny][nx]
+  const int nr = nx * ny * nz;
+  const double2* src = P + ((size_t)ql * npair + p) * nr;
+  RP_PD_SWITCH(nz, MAXN, (pd_zaxis_grid<N>(src, nx, ny, zmask, twiddles_of(twtab, N, -1), g)))
+  __syncthreads();
+  RP_PD_SWITCH(ny, MAXN, (pd_yaxis<N>(g, nx, nzz, ymask, twiddles_of(twtab, N, -1))))
+  __syncthreads();
+  RP_PD_SWITCH(nx, MAXN, (pd_xaxis<N>(g, ny, nxl, xline, xmask, twiddles_of(twtab, N, -1))))
+  __syncthreads();
+  const int np = ngpsi[q], g0 = g0s[q];
+  const double* len = qg + qg_off[q];
+  const double scl = 1.0 / ((double)nr * (double)ntk);
+  double2* out = rho + rho_off[q] + (size_t)p * np;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    const double2 v = g[pos[i]];
+    out[i] = i == g0 ? make_double2(0.0, 0.0)
+                     : make_double2(v.x * scl / len[i], v.y * scl / len[i]);
+  }
+}
 #undef RP_PD_CASE
 #undef RP_PD_SWITCH
 
@@ -454,10 +683,62 @@ void launch_twiddles(double2* tab, hipStream_t s) {
   RP_HIP(hipGetLastError());
 }
 
+#define RP_LD_CASE(NN)                                                                       \
+  case NN:                                                                                   \
+    hipLaunchKernelGGL(k_line_dft<NN>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, s, g, \
+                       nl, axis, nx, ny, nz, twiddles_of(tab, NN, sign));                    \
+    break;
+void launch_line_dft3(double2* g, long long nb, int nx, int ny, int nz, int sign,
+                      const double2* tab, hipStream_t s) {
+  const long long nr = (long long)nx * ny * nz;
+  const int axes[3] = {2, 1, 0};  // z, then y, then x (m_fft3d_20150826.f90:1598-1720)
+  for (int axis : axes) {
+    const int n = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const long long nl = nb * (nr / n);
+    switch (n) {
+      RP_LD_CASE(1) RP_LD_CASE(2) RP_LD_CASE(3) RP_LD_CASE(4) RP_LD_CASE(5) RP_LD_CASE(6)
+      RP_LD_CASE(8) RP_LD_CASE(9) RP_LD_CASE(10) RP_LD_CASE(12) RP_LD_CASE(15) RP_LD_CASE(16)
+      RP_LD_CASE(18) RP_LD_CASE(20) RP_LD_CASE(24) RP_LD_CASE(25) RP_LD_CASE(27) RP_LD_CASE(30)
+      RP_LD_CASE(32)
+      default: throw Error("FFT axis length " + std::to_string(n) + " not supported");
+    }
+    RP_HIP(hipGetLastError());
+  }
+}
+#undef RP_LD_CASE
+
+// Grids that do not fit the LDS go through HBM in batches of at most this size.
+constexpr size_t kHbmBatchBytes = size_t(1) << 30;
+
+size_t band_grids_scratch_elems(int nx, int ny, int nz, size_t ngrids) {
+  const size_t nr = (size_t)nx * ny * nz;
+  if (nr * sizeof(double2) <= kLdsBytes) return 0;
+  return std::min<size_t>(ngrids, std::max<size_t>(1, kHbmBatchBytes / (nr * sizeof(double2)))) * nr;
+}
+
 void launch_band_grids(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                        int nband, int nx, int ny, int nz, const double2* twtab, float2* ur,
-                       hipStream_t s) {
-  size_t lds = sizeof(double2) * ((size_t)nx * ny * nz);
+                       double2* scratch, size_t scratch_elems, hipStream_t s) {
+  const size_t nr = (size_t)nx * ny * nz;
+  if (nr * sizeof(double2) > kLdsBytes) {
+    // backward transform as line passes over HBM, in batches of grids
+    RP_CHECK(scratch && scratch_elems >= nr, "band grids: HBM scratch missing");
+    const long long total = (long long)nband * ntk;
+    const long long per = (long long)(scratch_elems / nr);
+    for (long long gb0 = 0; gb0 < total; gb0 += per) {
+      const long long nb = std::min(per, total - gb0);
+      launch_zero(scratch, (size_t)nb * nr * sizeof(double2), s);
+      hipLaunchKernelGGL(k_scatter_coef, dim3((ntg + 255) / 256, (unsigned)nb), dim3(256), 0, s, coef,
+                         kg, ng0, ntk, ntg, nx, ny, nz, gb0, scratch);
+      RP_HIP(hipGetLastError());
+      launch_line_dft3(scratch, nb, nx, ny, nz, +1, twtab, s);
+      hipLaunchKernelGGL(k_round_grids, dim3(4096), dim3(256), 0, s, scratch, (size_t)nb * nr,
+                         ur + (size_t)gb0 * nr);
+      RP_HIP(hipGetLastError());
+    }
+    return;
+  }
+  size_t lds = sizeof(double2) * nr;
   const bool small = nx <= 16 && ny <= 16 && nz <= 16;
   const void* fn = small ? (const void*)k_band_grids<16> : (const void*)k_band_grids<32>;
   if (lds > 65536)
@@ -493,6 +774,79 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
   }
 }
 
+void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int ny, int nz,
+                      const int* ikq, const int* shift, const double2* wk, int nq, double2* P,
+                      hipStream_t s) {
+  RP_CHECK(nwf >= 1 && nwf <= 8, "Wannier pair sums take at most 8 Wannier functions");
+  RP_CHECK(nx <= kTwMax && ny <= kTwMax && nz <= kTwMax, "FFT axis too long for the phase tables");
+  const int nr = nx * ny * nz;
+  const int groups = full ? (nwf + kRsumJB - 1) / kRsumJB : 1;
+  dim3 grid((nr + 255) / 256, nq, groups);
+  if (full)
+    hipLaunchKernelGGL(k_pair_rsum<true>, grid, dim3(256), 0, s, ur, ntk, nwf, nx, ny, nz, ikq,
+                       shift, wk, P);
+  else
+    hipLaunchKernelGGL(k_pair_rsum<false>, grid, dim3(256), 0, s, ur, ntk, nwf, nx, ny, nz, ikq,
+                       shift, wk, P);
+  RP_HIP(hipGetLastError());
+}
+
+void launch_rho_from_grid(const double2* P, int npair, int nx, int ny, int nz, const int* plan,
+                          const long long* plan_off, int max_nzz, const double* qg,
+                          const long long* qg_off, const int* g0s, const int* ngpsi,
+                          const long long* rho_off, int q_offset, int nq, int ntk,
+                          const double2* twtab, double2* rho, hipStream_t s) {
+  const size_t lds = sizeof(double2) * ((size_t)nx * ny * max_nzz);
+  const bool small = nx <= 16 && ny <= 16 && nz <= 16;
+  const void* fn = small ? (const void*)k_rho_from_grid<16> : (const void*)k_rho_from_grid<32>;
+  if (lds > 65536)
+    RP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  dim3 grid(npair, nq);
+  if (small)
+    hipLaunchKernelGGL(k_rho_from_grid<16>, grid, dim3(256), lds, s, P, npair, nx, ny, nz, plan,
+                       plan_off, qg, qg_off, g0s, ngpsi, rho_off, q_offset, ntk, twtab, rho);
+  else
+    hipLaunchKernelGGL(k_rho_from_grid<32>, grid, dim3(256), lds, s, P, npair, nx, ny, nz, plan,
+                       plan_off, qg, qg_off, g0s, ngpsi, rho_off, q_offset, ntk, twtab, rho);
+  RP_HIP(hipGetLastError());
+}
+
+// rho from transformed grids P[q][p][r] (the HBM path of k_rho_from_grid): the
+// points -G of the psi sphere (lg at lg_off, 3 ints per point), same scaling.
+__global__ void __launch_bounds__(256) k_rho_gather(
+    const double2* __restrict__ P, int npair, int nx, int ny, int nz, const int* __restrict__ lg,
+    const double* __restrict__ qg, const long long* __restrict__ off, const int* __restrict__ g0s,
+    const int* __restrict__ ngpsi, const long long* __restrict__ rho_off, int q_offset, int ntk,
+    double2* __restrict__ rho) {
+  const int p = blockIdx.x, ql = blockIdx.y, q = ql + q_offset;
+  const int np = ngpsi[q], g0 = g0s[q];
+  const int nr = nx * ny * nz;
+  const double2* src = P + ((size_t)ql * npair + p) * nr;
+  const int* l = lg + 3 * off[q];
+  const double* len = qg + off[q];
+  const double scl = 1.0 / ((double)nr * (double)ntk);
+  double2* out = rho + rho_off[q] + (size_t)p * np;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    const int r = wrap(-l[3 * i], nx) + nx * (wrap(-l[3 * i + 1], ny) + ny * wrap(-l[3 * i + 2], nz));
+    const double2 v = src[r];
+    out[i] = i == g0 ? make_double2(0.0, 0.0)
+                     : make_double2(v.x * scl / len[i], v.y * scl / len[i]);
+  }
+}
+
+void launch_rho_from_grid_hbm(double2* P, int npair, int nx, int ny, int nz, const int* lg,
+                              const double* qg, const long long* off, const int* g0s,
+                              const int* ngpsi, const long long* rho_off, int q_offset, int nq,
+                              int ntk, const double2* twtab, double2* rho, hipStream_t s) {
+  launch_line_dft3(P, (long long)nq * npair, nx, ny, nz, -1, twtab, s);
+  hipLaunchKernelGGL(k_rho_gather, dim3(npair, nq), dim3(256), 0, s, P, npair, nx, ny, nz, lg, qg,
+                     off, g0s, ngpsi, rho_off, q_offset, ntk, rho);
+  RP_HIP(hipGetLastError());
+}
+
+size_t pruned_transform_lds_bytes(int nx, int ny, int nz, int nzz) {
+  return sizeof(double2) * ((size_t)nx * ny * nzz + nx + ny + nz);
+}
 
 void launch_q0_heads(const double2* coef, const int* kg, const int* ng0, int ntk, int ntg,
                      const int* pocc, const int* pvir, const double* eband, const double* bvec,

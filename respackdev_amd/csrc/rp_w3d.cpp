@@ -48,18 +48,22 @@ struct W3d {
   // device
   DevBuf<double2> d_coef, d_tw;
   DevBuf<int> d_kg0, d_ng0;
-  DevBuf<float2> d_ur, d_eps, d_A;
+  DevBuf<float2> d_ur, d_eps;
   DevBuf<long long> d_eps_off, d_rho_off, d_map_off;
   DevBuf<int4> d_qinfo;
-  DevBuf<int> d_minv, d_pw;
+  DevBuf<int> d_minv;
   DevBuf<double> d_sq;
   // per-q geometry of the pair densities, every q at once (built on first use
   // after rp_w3d_set_q): |q+G|, k+q index and umklapp per k, pruned-transform plan
   bool geo_ready = false;
   std::vector<long long> geo_qg_off, geo_plan_off;
   std::vector<int> geo_g0, geo_q0, geo_nzz;
+  int geo_max_nzz = 0;
   DevBuf<double> d_geo_qg;
-  DevBuf<int> d_geo_ikq, d_geo_shift, d_geo_plan;
+  DevBuf<int> d_geo_ikq, d_geo_shift, d_geo_plan, d_geo_g0, d_ngpsi, d_geo_lg;
+  DevBuf<long long> d_geo_qg_off, d_geo_plan_off;
+  // summed pair-density grids P[q][p][r] of a chunk of q (launch_pair_rsum)
+  DevBuf<double2> d_P;
   DevBuf<double2> d_rho, d_pf1, d_pf2, d_func, d_shell, d_funcw, d_cheadw, d_vmat, d_wmat;
 };
 
@@ -166,7 +170,7 @@ QGeom q_geometry(const W3d& w, int iq) {
 void ensure_geometry(W3d& w, hipStream_t s) {
   if (w.geo_ready) return;
   std::vector<double> qg;
-  std::vector<int> ikq, shift, plan;
+  std::vector<int> ikq, shift, plan, lgs;
   w.geo_qg_off.assign(w.ntq, 0);
   w.geo_plan_off.assign(w.ntq, 0);
   w.geo_g0.assign(w.ntq, -1);
@@ -177,6 +181,7 @@ void ensure_geometry(W3d& w, hipStream_t s) {
     const int np = w.ngpsi[iq];
     w.geo_qg_off[iq] = (long long)qg.size();
     qg.insert(qg.end(), geo.qg.begin(), geo.qg.end());
+    lgs.insert(lgs.end(), w.lg0[iq].begin(), w.lg0[iq].begin() + 3 * (size_t)np);
     ikq.insert(ikq.end(), geo.ikq.begin(), geo.ikq.end());
     shift.insert(shift.end(), geo.shift.begin(), geo.shift.end());
     const std::vector<int> pl = pairdens_plan(
@@ -188,12 +193,47 @@ void ensure_geometry(W3d& w, hipStream_t s) {
     w.geo_g0[iq] = geo.g0;
     w.geo_q0[iq] = geo.q0 ? 1 : 0;
   }
+  w.geo_max_nzz = *std::max_element(w.geo_nzz.begin(), w.geo_nzz.end());
   w.d_geo_qg.upload(qg, s);
+  w.d_geo_lg.upload(lgs, s);
   w.d_geo_ikq.upload(ikq, s);
   w.d_geo_shift.upload(shift, s);
   w.d_geo_plan.upload(plan, s);
+  w.d_geo_qg_off.upload(w.geo_qg_off, s);
+  w.d_geo_plan_off.upload(w.geo_plan_off, s);
+  w.d_geo_g0.upload(w.geo_g0, s);
+  w.d_ngpsi.upload(w.ngpsi, s);
   RP_HIP(hipStreamSynchronize(s));  // the host vectors end here
   w.geo_ready = true;
+}
+
+// Bound on the summed pair-density grids held at once (q chunks of launch_pair_rsum).
+constexpr size_t kPBudgetBytes = size_t(1) << 30;
+
+// rho of every q (calc_intW.F:647-767 with wk null, calc_intJ.F:618-720 with
+// wk = e^{-i 2pi k.R}): the products summed over k on the grid, then one
+// pruned transform per (q, pair), in q chunks of at most kPBudgetBytes.
+void rho_all_q(W3d& w, bool full, const double2* wk, const long long* d_rho_off, double2* rho,
+               hipStream_t s) {
+  const int npair = full ? w.nwf * w.nwf : w.nwf;
+  const size_t nr = (size_t)w.nw[0] * w.nw[1] * w.nw[2];
+  const size_t per_q = (size_t)npair * nr * sizeof(double2);
+  const int qchunk = (int)std::max<size_t>(1, std::min<size_t>(w.ntq, kPBudgetBytes / per_q));
+  w.d_P.reserve((size_t)qchunk * npair * nr);
+  for (int q0 = 0; q0 < w.ntq; q0 += qchunk) {
+    const int nq = std::min(qchunk, w.ntq - q0);
+    launch_pair_rsum(w.d_ur.p, w.ntk, w.nwf, full, w.nw[0], w.nw[1], w.nw[2],
+                     w.d_geo_ikq.p + (size_t)q0 * w.ntk, w.d_geo_shift.p + (size_t)3 * q0 * w.ntk,
+                     wk, nq, w.d_P.p, s);
+    if (pruned_transform_lds_bytes(w.nw[0], w.nw[1], w.nw[2], w.geo_max_nzz) <= lds_budget_bytes())
+      launch_rho_from_grid(w.d_P.p, npair, w.nw[0], w.nw[1], w.nw[2], w.d_geo_plan.p,
+                           w.d_geo_plan_off.p, w.geo_max_nzz, w.d_geo_qg.p, w.d_geo_qg_off.p,
+                           w.d_geo_g0.p, w.d_ngpsi.p, d_rho_off, q0, nq, w.ntk, w.d_tw.p, rho, s);
+    else  // the psi sphere's z planes exceed the LDS: transform over HBM, then gather
+      launch_rho_from_grid_hbm(w.d_P.p, npair, w.nw[0], w.nw[1], w.nw[2], w.d_geo_lg.p,
+                               w.d_geo_qg.p, w.d_geo_qg_off.p, w.d_geo_g0.p, w.d_ngpsi.p, d_rho_off,
+                               q0, nq, w.ntk, w.d_tw.p, rho, s);
+  }
 }
 
 }  // namespace
@@ -228,9 +268,7 @@ int rp_w3d_open(rp_handle h, const double* lattice, int32_t nsym, int32_t nnp, c
     fft_grid(w.a, w.b, w.ecut_psi, w.nw);
     for (int d = 0; d < 3; ++d)
       RP_CHECK(dft_size_supported(w.nw[d]), "FFT grid dimension " + std::to_string(w.nw[d]) +
-                                                " is not supported by the LDS transform");
-    RP_CHECK((long)w.nw[0] * w.nw[1] * w.nw[2] <= max_grid_points(),
-             "FFT grid too large for the LDS-resident transform");
+                                                " is not supported (2,3,5-smooth sizes up to 32)");
     // rginv (invmat of rg; integer matrices of det +-1)
     w.rginv.assign(9 * (size_t)nsym, 0.0);
     for (int op = 0; op < nsym; ++op) {
@@ -352,8 +390,14 @@ int rp_w3d_set_wannier(rp_handle h, int32_t nwf, const double* wan) {
     w.d_ur.reserve((size_t)nwf * ntk * nr);
     w.d_tw.reserve(twiddle_entries());
     launch_twiddles(w.d_tw.p, s);
-    launch_band_grids(w.d_coef.p, w.d_kg0.p, w.d_ng0.p, ntk, ntg, nwf, w.nw[0], w.nw[1], w.nw[2],
-                      w.d_tw.p, w.d_ur.p, s);
+    {
+      DevBuf<double2> scratch;
+      const size_t se = band_grids_scratch_elems(w.nw[0], w.nw[1], w.nw[2], (size_t)nwf * ntk);
+      scratch.reserve(se);
+      launch_band_grids(w.d_coef.p, w.d_kg0.p, w.d_ng0.p, ntk, ntg, nwf, w.nw[0], w.nw[1], w.nw[2],
+                        w.d_tw.p, w.d_ur.p, scratch.p, se, s);
+      RP_HIP(hipStreamSynchronize(s));
+    }
     RP_HIP(hipStreamSynchronize(s));
     w.have_wan = true;
     w.have_funcs = false;
@@ -522,23 +566,9 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     upload_eps_maps(w, s);
     w.d_rho_off.upload(w.rho_off, s);
     w.d_rho.reserve((size_t)(w.rho_off.back() + (long long)w.ngpsi.back() * w3d_max_wannier()));
-    // rho for every q (calc_intW.F:647-767): diagonal Wannier pair densities
-    std::vector<int> pw(nwf);
-    for (int i = 0; i < nwf; ++i) pw[i] = i;
-    w.d_pw.upload(pw, s);
-    int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
-    const size_t lda = roundup((size_t)maxpsi, 64);
-    w.d_A.reserve((size_t)nwf * ntk * lda);
+    // rho for every q (calc_intW.F:647-767): the diagonal Wannier pairs
     ensure_geometry(w, s);
-    for (int iq = 0; iq < w.ntq; ++iq) {
-      const int np = w.ngpsi[iq];
-      launch_pair_densities_pruned(w.d_ur.p, ntk, w.d_pw.p, w.d_pw.p, w.d_geo_ikq.p + (size_t)iq * ntk,
-                                   w.d_geo_shift.p + (size_t)3 * iq * ntk, w.nw[0], w.nw[1], w.nw[2],
-                                   w.d_geo_plan.p + w.geo_plan_off[iq], w.geo_nzz[iq], np, lda,
-                                   nwf * ntk, w.d_tw.p, w.d_A.p, s);
-      launch_w_rho(w.d_A.p, lda, ntk, nwf, np, w.d_geo_qg.p + w.geo_qg_off[iq], w.geo_g0[iq],
-                   w.d_rho.p + w.rho_off[iq], s);
-    }
+    rho_all_q(w, false, nullptr, w.d_rho_off.p, w.d_rho.p, s);
     w.d_func.reserve((size_t)w.ntq * nwf * nwf);
     w.d_shell.reserve((size_t)w.ntq * nwf * nwf);
     w.d_funcw.reserve((size_t)w.ntq * nen * nwf * nwf);
@@ -600,13 +630,8 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     const int ntk = w.ntk, nwf = w.nwf, nen = w.nen, npair = nwf * nwf;
     upload_eps_maps(w, s);
     compute_heads(w);
-    // pair p = iw + nwf*jw: C0_K = C0(:,jw,k), C0_KQ = C0(:,iw,k+q) (calc_intJ.F:651-657)
-    std::vector<int> po(npair), pv(npair);
-    for (int jw = 0; jw < nwf; ++jw)
-      for (int iw = 0; iw < nwf; ++iw) { po[iw + nwf * jw] = jw; pv[iw + nwf * jw] = iw; }
-    DevBuf<int> d_po, d_pv;
-    d_po.upload(po, s);
-    d_pv.upload(pv, s);
+    // pair p = iw + nwf*jw: C0_K = C0(:,jw,k), C0_KQ = C0(:,iw,k+q) (calc_intJ.F:651-657),
+    // the pair order of launch_pair_rsum's full mode
     // e^{-i 2pi k.R} per R and k (calc_intJ.F:646-650)
     const double tpi = 2.0 * std::acos(-1.0);
     std::vector<double2> pfk((size_t)nr * ntk);
@@ -626,20 +651,9 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     d_joff.upload(joff, s);
     DevBuf<double2> d_jrho, d_jfunc, d_jshell, d_jfuncw, d_x, d_j, d_cw, d_xg;
     d_jrho.reserve((size_t)tot * nr);
-    const int maxpsi = *std::max_element(w.ngpsi.begin(), w.ngpsi.end());
-    const size_t lda = roundup((size_t)maxpsi, 64);
-    w.d_A.reserve((size_t)npair * ntk * lda);
     ensure_geometry(w, s);
-    for (int iq = 0; iq < w.ntq; ++iq) {
-      const int np = w.ngpsi[iq];
-      launch_pair_densities_pruned(w.d_ur.p, ntk, d_po.p, d_pv.p, w.d_geo_ikq.p + (size_t)iq * ntk,
-                                   w.d_geo_shift.p + (size_t)3 * iq * ntk, w.nw[0], w.nw[1], w.nw[2],
-                                   w.d_geo_plan.p + w.geo_plan_off[iq], w.geo_nzz[iq], np, lda,
-                                   npair * ntk, w.d_tw.p, w.d_A.p, s);
-      for (int r = 0; r < nr; ++r)
-        launch_j_rho(w.d_A.p, lda, ntk, npair, np, w.d_geo_qg.p + w.geo_qg_off[iq], w.geo_g0[iq],
-                     w.geo_q0[iq], d_pfk.p + (size_t)r * ntk, d_jrho.p + (size_t)r * tot + joff[iq], s);
-    }
+    for (int r = 0; r < nr; ++r)
+      rho_all_q(w, true, d_pfk.p + (size_t)r * ntk, d_joff.p, d_jrho.p + (size_t)r * tot, s);
     d_jfunc.reserve((size_t)w.ntq * npair);
     d_jshell.reserve((size_t)w.ntq * npair);
     d_jfuncw.reserve((size_t)w.ntq * nen * npair);

@@ -2,8 +2,9 @@
 // integrals V_ij(R), W_ij(R,w) of calc_intW.F:636-985.
 //
 // rho_w(q+G) = (1/NTK) sum_k M_ww,k(G) / |q+G| comes from the pair-density
-// kernels (the diagonal Wannier pairs) followed by k_w_rho.  Then per full-mesh
-// q (all q of one launch):
+// kernels of rp_pairdens.hip (k_pair_rsum, k_rho_from_grid: the products summed
+// over k on the grid, one transform per (q, pair)).  Then per full-mesh q (all
+// q of one launch):
 //   func_ij(q)     = sum_{G in psi} rho_i(G) conj(rho_j(G))                (k_w_func)
 //   funcw_ij(q,w)  = sum_{G,G' in eps} rho_i(G) eps^-1_q(G,G',w) conj(rho_j(G'))
 //                    + sum_{G in psi \ eps} rho_i(G) conj(rho_j(G))        (k_w_contract)
@@ -16,47 +17,6 @@
 #include "rp_kernels.h"
 
 namespace rp {
-
-// rho[q][w][G] (complex FP64) from the pair densities A[(w*ntk + k)][lda]
-// (complex64): sum over k, / |q+G|, / NTK; G = 0 at q = 0 is zero
-// (calc_intW.F:681-767).  The k sum is split over kRhoSplit chunks (threadIdx.y),
-// whose partial sums are added in chunk order.  grid: (ceil(ngpsi/64), nwf),
-// block (64, kRhoSplit).
-constexpr int kRhoSplit = 8;
-__global__ void __launch_bounds__(512) k_w_rho(const float2* __restrict__ A, size_t lda, int ntk,
-                                               int ngpsi, const double* __restrict__ qg, int g0,
-                                               double2* __restrict__ rho) {
-  __shared__ double2 part[kRhoSplit][64];
-  const int g = blockIdx.x * 64 + threadIdx.x;
-  const int w = blockIdx.y, c = threadIdx.y;
-  const int per = (ntk + kRhoSplit - 1) / kRhoSplit;
-  const int k0 = c * per, k1 = min(ntk, k0 + per);
-  double sr = 0.0, si = 0.0;
-  if (g < ngpsi) {
-    const float2* a = A + (size_t)w * ntk * lda + g;
-    for (int k = k0; k < k1; ++k) {
-      const float2 v = a[(size_t)k * lda];
-      sr += (double)v.x;
-      si += (double)v.y;
-    }
-  }
-  part[c][threadIdx.x] = make_double2(sr, si);
-  __syncthreads();
-  if (c != 0 || g >= ngpsi) return;
-  sr = si = 0.0;
-#pragma unroll
-  for (int j = 0; j < kRhoSplit; ++j) {
-    sr += part[j][threadIdx.x].x;
-    si += part[j][threadIdx.x].y;
-  }
-  double2 r;
-  if (g == g0) {
-    r = make_double2(0.0, 0.0);
-  } else {
-    r = make_double2(sr / qg[g] / (double)ntk, si / qg[g] / (double)ntk);
-  }
-  rho[(size_t)w * ngpsi + g] = r;
-}
 
 // Per q: func (all G of the psi list) and the shell part of funcw (G beyond the
 // eps sphere).  grid: ntq blocks of 256; out [q][j][i].
@@ -400,51 +360,6 @@ __global__ void __launch_bounds__(512) k_j_contract(
   }
 }
 
-// calc_j3d rho (calc_intJ.F:634-790): rho_p(G) for the pair p = iw + nwf*jw,
-// sum_k M_(jw at k, iw at k+q)(G) e^{-i 2pi k.R} from A[(p*ntk + k)][lda]: at
-// q != 0 each term is divided by |q+G|, at q = 0 the sum is (G = 0 dropped);
-// then / NTK.  The k sum is split as in k_w_rho.  grid: (ceil(ngpsi/64),
-// npair), block (64, kRhoSplit).
-__global__ void __launch_bounds__(512) k_j_rho(const float2* __restrict__ A, size_t lda, int ntk,
-                                               int ngpsi, const double* __restrict__ qg, int g0,
-                                               int q0, const double2* __restrict__ pfk,
-                                               double2* __restrict__ rho) {
-  __shared__ double2 part[kRhoSplit][64];
-  const int g = blockIdx.x * 64 + threadIdx.x;
-  const int p = blockIdx.y, c = threadIdx.y;
-  const int per = (ntk + kRhoSplit - 1) / kRhoSplit;
-  const int k0 = c * per, k1 = min(ntk, k0 + per);
-  double sr = 0.0, si = 0.0;
-  double len = 1.0;
-  if (g < ngpsi) {
-    const float2* a = A + (size_t)p * ntk * lda + g;
-    len = qg[g];
-    for (int k = k0; k < k1; ++k) {
-      const float2 v = a[(size_t)k * lda];
-      const double2 f = pfk[k];
-      double tr = (double)v.x * f.x - (double)v.y * f.y;
-      double ti = (double)v.x * f.y + (double)v.y * f.x;
-      if (!q0) { tr = tr / len; ti = ti / len; }
-      sr += tr;
-      si += ti;
-    }
-  }
-  part[c][threadIdx.x] = make_double2(sr, si);
-  __syncthreads();
-  if (c != 0 || g >= ngpsi) return;
-  sr = si = 0.0;
-#pragma unroll
-  for (int j = 0; j < kRhoSplit; ++j) {
-    sr += part[j][threadIdx.x].x;
-    si += part[j][threadIdx.x].y;
-  }
-  double2 r;
-  if (q0 && g == g0) r = make_double2(0.0, 0.0);
-  else if (q0) r = make_double2(sr / len / (double)ntk, si / len / (double)ntk);
-  else r = make_double2(sr / (double)ntk, si / (double)ntk);
-  rho[(size_t)p * ngpsi + g] = r;
-}
-
 // calc_j3d func / shell (calc_intJ.F:680-690): per (q, p) sum_G |rho_p(G)|^2 as
 // rho conj(rho).  grid: (ntq, npair), 256 threads; out [q][p].
 __global__ void __launch_bounds__(256) k_j_func(const double2* __restrict__ rho,
@@ -559,13 +474,6 @@ __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ fu
 int w3d_max_wannier() { return kMaxWf; }
 int w3d_max_eps_g() { return 2 * kCT; }
 
-void launch_w_rho(const float2* A, size_t lda, int ntk, int nwf, int ngpsi, const double* qg,
-                  int g0, double2* rho, hipStream_t s) {
-  hipLaunchKernelGGL(k_w_rho, dim3((ngpsi + 63) / 64, nwf), dim3(64, kRhoSplit), 0, s, A, lda, ntk,
-                     ngpsi, qg, g0, rho);
-  RP_HIP(hipGetLastError());
-}
-
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s) {
   hipLaunchKernelGGL(k_w_func, dim3(ntq), dim3(256), 0, s, rho, rho_off, qinfo, nwf, func, shell);
@@ -608,13 +516,6 @@ void launch_j_contract(const float2* eps, const long long* eps_off, const double
 }
 
 size_t j_contract_scratch(size_t sum_nge, int npair) { return sum_nge * 32 * ((npair + 31) / 32); }
-
-void launch_j_rho(const float2* A, size_t lda, int ntk, int npair, int ngpsi, const double* qg,
-                  int g0, int q0, const double2* pfk, double2* rho, hipStream_t s) {
-  hipLaunchKernelGGL(k_j_rho, dim3((ngpsi + 63) / 64, npair), dim3(64, kRhoSplit), 0, s, A, lda,
-                     ntk, ngpsi, qg, g0, q0, pfk, rho);
-  RP_HIP(hipGetLastError());
-}
 
 void launch_j_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int npair, double2* func, double2* shell, hipStream_t s) {

@@ -137,3 +137,19 @@ def test_chi_invariants(gpu):
         h2 = (c - c.conj().T) / 2j
         ev = np.linalg.eigvalsh(h2)
         assert ev.max() <= 1e-12 * abs(ev).max()
+
+
+@pytest.mark.parametrize("prec,bar", [("fp64", 1e-6), ("bf16x3", 1e-5)])
+def test_plane_wave_states_closed_form(gpu, prec, bar):
+    """chi and eps^-1 of plane-wave bands with flat energies against the closed
+    form of tests/plane_wave_case.py (normalisation, spin factor, 4 pi and the
+    conj placement of chiqw.F:1397-1410 on the GPU path itself)."""
+    import plane_wave_case as pw
+    w = pw.wfn()
+    info = gpu.open(w, ecut_for_eps=pw.ECUT_EPS, num_freq_grid=pw.NEN)
+    r = gpu.run_q(pw.Q, prec, want_chi=True)
+    lg, chi, epsinv = pw.closed_form(info["em"], 0.1 / 27.21151)
+    np.testing.assert_array_equal(r["lg"], lg)
+    e_chi, e_eps = relf(r["chi"][0], chi), relf(r["epsinv"][0], epsinv)
+    print(f"plane waves {prec}: chi {e_chi:.3e} eps^-1 {e_eps:.3e}", flush=True)
+    assert e_chi <= bar and e_eps <= bar
