@@ -33,7 +33,7 @@ module respack_gpu
   public :: rp_algn235, rp_gen_grid, rp_make_kg0, rp_ttrhdrn_mkidx
   public :: rp_chiqw_open, rp_chiqw_set_crpa, rp_chiqw_info, rp_chiqw_kmesh
   public :: rp_chiqw_prepare_q, rp_pairdens_batch, rp_tetra_weights_batch
-  public :: rp_chi_accumulate, rp_dyson, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
+  public :: rp_chi_accumulate, rp_dyson, rp_dyson_noalloc, rp_chiqw_q, rp_chiqw_npair, rp_chiqw_set_window
   public :: rp_chiqw_batch_plan, rp_chiqw_batch_run, rp_chiqw_batch_result, rp_chiqw_batch_free
   public :: rp_chiqw_autotune, rp_chiqw_tune_count, rp_chiqw_tune_info, rp_chiqw_set_span
   public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
@@ -149,6 +149,13 @@ module respack_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: h
       type(c_ptr), value :: epsinv_out
+    end function
+    ! eps^-1 from chi, both in device memory, with no allocation (see respack_gpu.h)
+    integer(c_int) function rp_dyson_noalloc(h, chi_dev, epsinv_dev) bind(C, name="rp_dyson_noalloc")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      type(c_ptr), value :: chi_dev
+      type(c_ptr), value :: epsinv_dev
     end function
     integer(c_int) function rp_chiqw_q(h, q, precision, epsinv_out, chi_out, timings) &
         bind(C, name="rp_chiqw_q")

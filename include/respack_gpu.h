@@ -125,6 +125,17 @@ int rp_chi_set(rp_handle h, const double* chi_in);
 /* Dyson step calc_eps_rpa / calc_eps_rpa_q_0 (chiqw_sub.F:386-568) on the GPU
  * in FP64: eps^-1, Fortran order (ig,jg,ie) (times 3 at q = 0). */
 int rp_dyson(rp_handle h, double* epsinv_out);
+/* The Dyson step of rp_dyson for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  chi_dev: device memory
+ * holding chi of the prepared q in the layout rp_chi_accumulate writes
+ * (complex (ig, jg, ie[, dir])); epsinv_dev: device memory of the same size,
+ * receives eps^-1 as rp_dyson's output.  The step's scratch is a fixed 4 MiB
+ * device array per GPU (status, Drude terms, pivots, Gauss-Jordan move lists):
+ * sizes that need more (about NG x 3 nen above 10^6) are rejected.  The call
+ * makes no allocation and synchronises the handle's stream to check the
+ * status.  Calls that overlap on one GPU share that scratch and race. */
+int rp_dyson_noalloc(rp_handle h, const double* chi_dev, double* epsinv_dev);
 /* All four stages for one q.  timings (optional, 8 doubles): seconds of pair
  * densities, weights, contraction, Dyson, total device time, host prepare,
  * the main contraction kernel alone (summed over its launches), and the

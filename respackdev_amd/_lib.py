@@ -24,7 +24,7 @@ EXPORTS = [
     "rp_init", "rp_finalize", "rp_last_error", "rp_algn235", "rp_gen_grid", "rp_make_kg0",
     "rp_ttrhdrn_mkidx", "rp_chiqw_open", "rp_chiqw_set_crpa", "rp_chiqw_info", "rp_chiqw_kmesh",
     "rp_chiqw_prepare_q", "rp_pairdens_batch", "rp_tetra_weights_batch", "rp_chi_accumulate",
-    "rp_dyson", "rp_chiqw_q", "rp_chiqw_npair", "rp_chiqw_set_window",
+    "rp_dyson", "rp_dyson_noalloc", "rp_chiqw_q", "rp_chiqw_npair", "rp_chiqw_set_window",
     "rp_mlp_open", "rp_mlp_nparams", "rp_mlp_get_params", "rp_mlp_set_params", "rp_mlp_forward",
     "rp_mlp_gradient", "rp_mlp_train", "rp_chi_set",
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
@@ -71,6 +71,7 @@ def lib():
         L.rp_tetra_weights_batch.argtypes = [P, P]
         L.rp_chi_accumulate.argtypes = [P, I32, P]
         L.rp_dyson.argtypes = [P, P]
+        L.rp_dyson_noalloc.argtypes = [P, P, P]
         L.rp_chiqw_q.argtypes = [P, P, I32, P, P, P]
         L.rp_chiqw_npair.argtypes = [P, P]
         L.rp_chiqw_set_window.argtypes = [P, D, I32]

@@ -28,7 +28,7 @@ HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-Wall",
 
 SOURCES = ["rp_setup.cpp", "rp_capi.cpp", "rp_w3d.cpp", "rp_batch.cpp", "rp_pairdens.hip", "rp_tetra.hip",
            "rp_chi_gemm.hip", "rp_chi_bf16.hip", "rp_dyson.hip", "rp_w3d.hip",
-           "rp_mlp.cpp", "rp_mlp.hip"]
+           "rp_mlp.cpp", "rp_mlp.hip", "rp_noalloc.hip"]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -31,6 +31,7 @@ int rp_init(int32_t device, rp_handle* out) {
     RP_CHECK(n > 0, "rp_init: no HIP device visible");
     RP_CHECK(device >= 0 && device < n, "rp_init: device ordinal out of range");
     RP_HIP(hipSetDevice(device));
+    noalloc_init(device);
     hipDeviceProp_t prop;
     RP_HIP(hipGetDeviceProperties(&prop, device));
     RP_CHECK(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0,

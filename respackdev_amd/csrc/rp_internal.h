@@ -198,6 +198,9 @@ namespace rp {
 void check_status(rp_ctx* h, const char* what);
 // Throws the error a non-zero status word of the kernels stands for.
 void throw_status(int st, const std::string& what);
+// Looks up this device's address of the fixed scratch of rp_dyson_noalloc
+// (rp_noalloc.hip); called by rp_init with the device current.
+void noalloc_init(int device);
 void issue_pairdens(rp_ctx* h);
 void issue_weights(rp_ctx* h);
 void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events);
