@@ -15,9 +15,7 @@
 //   FP64   v_mfma_f64_16x16x4f64, FP64 accumulators.
 //   FP32   v_mfma_f32_16x16x4f32; FP32 accumulators flushed into FP64 ones
 //          every kFlush chunks.
-//   BF16X3 v_mfma_f32_16x16x32_bf16 with each real operand split x = hi + lo
-//          (two bf16) and the products hi*hi + hi*lo + lo*hi; FP32
-//          accumulators flushed into FP64 every kFlush chunks.
+// (The split-BF16 precision has its own kernel, k_chi_bf16x3.)
 // The K range may be split over gridDim.z; partial tiles are summed in fixed
 // order by k_chi_reduce (deterministic), which also divides by the volume.
 #include <algorithm>
@@ -29,7 +27,6 @@ namespace rp {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 typedef float f4v __attribute__((ext_vector_type(4)));
-typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 constexpr int kTile = 64;   // G and G' per workgroup
 constexpr int kKC = 32;     // complex K rows per LDS chunk
@@ -127,7 +124,8 @@ __global__ void __launch_bounds__(256) k_chi_gemm(const float2* __restrict__ A, 
           accd_re[a][b][v] = acc_re[a][b][v];
           accd_im[a][b][v] = acc_im[a][b][v];
         }
-  } else if constexpr (P == RP_PREC_FP32) {
+  } else {
+    static_assert(P == RP_PREC_FP32, "k_chi_gemm: FP64 or FP32");
     f4v acc_re[2][2], acc_im[2][2];
     for (int a = 0; a < 2; ++a)
       for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
@@ -156,68 +154,6 @@ __global__ void __launch_bounds__(256) k_chi_gemm(const float2* __restrict__ A, 
             if (!colok[b]) continue;
             acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(L[a], Rre[b], acc_re[a][b], 0, 0, 0);
             acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(L[a], Rim[b], acc_im[a][b], 0, 0, 0);
-          }
-        }
-      }
-      __syncthreads();
-      if (++since == kFlush || c + 1 == c_end) {
-        since = 0;
-        for (int a = 0; a < 2; ++a)
-          for (int b = 0; b < 2; ++b) {
-            for (int v = 0; v < 4; ++v) {
-              accd_re[a][b][v] += (double)acc_re[a][b][v];
-              accd_im[a][b][v] += (double)acc_im[a][b][v];
-            }
-            acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
-          }
-      }
-    }
-  } else {  // RP_PREC_BF16X3
-    f4v acc_re[2][2], acc_im[2][2];
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) acc_re[a][b] = acc_im[a][b] = f4v{0, 0, 0, 0};
-    const int hh = q & 1;
-    int since = 0;
-    for (size_t c = c_begin; c < c_end; ++c) {
-      stage_chunk(A, lda, c * kKC, g0, h0, dww, sL, sR, sD);
-      __syncthreads();
-      for (int s = 0; s < 2; ++s) {
-        const int kb = 16 * s + 8 * hh;
-        bf8v Lh[2], Ll[2], Rrh[2], Rrl[2], Rih[2], Ril[2];
-        for (int b = 0; b < 2; ++b) {
-          const float2* lrow = sL[wm * 32 + b * 16 + li] + kb;
-          const float2* rrow = sR[wn * 32 + b * 16 + li] + kb;
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            float2 la = lrow[e];
-            float lv = part ? la.y : la.x;
-            __bf16 lh = (__bf16)lv;
-            Lh[b][e] = lh;
-            Ll[b][e] = (__bf16)(lv - (float)lh);
-            const double2 dd = sD[kb + e];
-            const float dx = (float)dd.x, dy = (float)dd.y;
-            float2 ra = rrow[e];
-            float bx = dx * ra.x - dy * ra.y;
-            float by = dx * ra.y + dy * ra.x;
-            float rre = part ? by : bx;
-            float rim = part ? -bx : by;
-            __bf16 a1 = (__bf16)rre, a2 = (__bf16)rim;
-            Rrh[b][e] = a1;
-            Rrl[b][e] = (__bf16)(rre - (float)a1);
-            Rih[b][e] = a2;
-            Ril[b][e] = (__bf16)(rim - (float)a2);
-          }
-        }
-        for (int a = 0; a < 2; ++a) {
-          if (!rowok[a]) continue;
-          for (int b = 0; b < 2; ++b) {
-            if (!colok[b]) continue;
-            acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrh[b], acc_re[a][b], 0, 0, 0);
-            acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rrl[b], acc_re[a][b], 0, 0, 0);
-            acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rrh[b], acc_re[a][b], 0, 0, 0);
-            acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Rih[b], acc_im[a][b], 0, 0, 0);
-            acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Lh[a], Ril[b], acc_im[a][b], 0, 0, 0);
-            acc_im[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Rih[b], acc_im[a][b], 0, 0, 0);
           }
         }
       }
@@ -284,12 +220,8 @@ void launch_chi_gemm(int prec, const float2* A, size_t lda, const double2* dw, s
       hipLaunchKernelGGL(k_chi_gemm<RP_PREC_FP32>, grid, dim3(256), 0, s, A, lda, dw, ldd, per,
                          kchunks, ng, ntile, nen, partial);
       break;
-    case RP_PREC_BF16X3:
-      hipLaunchKernelGGL(k_chi_gemm<RP_PREC_BF16X3>, grid, dim3(256), 0, s, A, lda, dw, ldd, per,
-                         kchunks, ng, ntile, nen, partial);
-      break;
     default:
-      throw Error("unknown precision mode");
+      throw Error("k_chi_gemm: FP64 or FP32 only (split-BF16 runs k_chi_bf16x3)");
   }
   RP_HIP(hipGetLastError());
 }

@@ -464,7 +464,6 @@ int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps
       w.nge[iq] = ne;
       w.ngpsi[iq] = (int)(lg.size() / 3);
       RP_CHECK(w.ngpsi[iq] <= w.ntg, "NG_for_psi(q) exceeds NTG (LG0 is sized NTG in the reference)");
-      RP_CHECK(ne <= w3d_max_eps_g(), "eps sphere too large for the W contraction kernel");
       w.lg0[iq] = lg;
     }
     // Gamma: the irreducible index (calc_intW.F:595-597) and G = 0 in its list
@@ -563,6 +562,13 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const int ntk = w.ntk, nwf = w.nwf, nen = w.nen;
+    // the W contraction holds two rows of eps^-1 per thread (the J contraction
+    // has no such limit)
+    for (int iq = 0; iq < w.ntq; ++iq)
+      RP_CHECK(w.nge[iq] <= w3d_max_eps_g(), "rp_w3d_funcs: eps sphere of " +
+                                                 std::to_string(w.nge[iq]) + " G exceeds the " +
+                                                 std::to_string(w3d_max_eps_g()) +
+                                                 " of the W contraction kernel");
     upload_eps_maps(w, s);
     w.d_rho_off.upload(w.rho_off, s);
     w.d_rho.reserve((size_t)(w.rho_off.back() + (long long)w.ngpsi.back() * w3d_max_wannier()));

@@ -208,7 +208,9 @@ __global__ void __launch_bounds__(512) k_w_contract(
 // Er Xr - Ei Xi, Im Y = Er Xi + Ei Xr) over 16-row blocks of E and 16-column
 // blocks of up to 32 pair vectors (blockIdx.z picks the 32).  X comes from
 // k_j_xvec, [b][32 groups] in HBM (zero past the last pair).  A wave takes up
-// to kJA row blocks per pass (wave + 8 j), so E is read once per (q, w).  Lane
+// to kJA row blocks per pass (wave + 8 j) and the passes cover every row, so
+// a workgroup reads E once: E is read once per (q, w) for up to 32 pair vectors
+// and once per (q, w, group of 32) beyond (NWF >= 6).  Lane
 // l supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds
 // D(l / 16 + 4 v, l % 16), v < 4 (checked on the GPU with row- and column-numbered
 // operands).  grid: (nen, ntq, groups), 512 threads.

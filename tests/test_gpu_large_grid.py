@@ -10,14 +10,19 @@ sphere spans every z plane) too.  The chi sphere at the default cutoff
 Reduced band set and k mesh (2^3 k, 2 occupied + 3 empty bands, ~4700 plane
 waves): FP64 chi and eps^-1 against the oracle within 1e-6; calc_w3d's func
 and funcw (two model Wannier functions, random eps^-1) within 1e-6.
+
+The same cell with an eps cutoff of 32 Ry (NG_eps 1166-1189): calc_j3d's X and
+J against the oracle within 1e-6 (the J contraction has no row limit), and
+calc_w3d's contraction, which holds at most 1024 rows, refuses it.
 """
 import numpy as np
 import pytest
 
 import oracle
 from respackdev_amd import gen
+from respackdev_amd._lib import RpError
 from respackdev_amd.chiqw import Chiqw, from_system
-from respackdev_amd.w3d import W3d
+from respackdev_amd.w3d import AU, W3d
 
 pytestmark = pytest.mark.gpu
 
@@ -56,11 +61,8 @@ def test_chiqw_on_a_24_cubed_grid(srvo3_sized):
     assert e_chi <= 1e-6 and e_eps <= 1e-6
 
 
-def test_w3d_on_a_24_cubed_grid(srvo3_sized):
-    from oracle import pipeline_w3d
-    s = srvo3_sized
+def _w3d_inputs(s, nwf, nen, ecut_eps, seed):
     w = from_system(s)
-    nwf, nen, ecut_eps = 2, 6, 8.1
     wan, _ = gen.make_wannier(s, nwf, 7)
     ntg = max(c.shape[1] for c in wan)
     W = np.zeros((len(wan), nwf, ntg), np.complex128)
@@ -68,13 +70,20 @@ def test_w3d_on_a_24_cubed_grid(srvo3_sized):
         W[k, :, :c.shape[1]] = c
     em = oracle.gen_grid(nen, 50.0 / 27.21151)
     sqi = np.array([[gen._fold(x) for x in k] for k in w.ski])
-    rng = np.random.default_rng(3)
+    rng = np.random.default_rng(seed)
     eps = []
     for q in sqi:
         n = len(oracle.make_kg0(s.bvec, ecut_eps, q))
         e = np.eye(n)[None] + 0.05 * (rng.standard_normal((nen, n, n)) +
                                       1j * rng.standard_normal((nen, n, n)))
         eps.append(e)
+    return w, W, em, sqi, eps
+
+
+def test_w3d_on_a_24_cubed_grid(srvo3_sized):
+    from oracle import pipeline_w3d
+    nwf, nen, ecut_eps = 2, 6, 8.1
+    w, W, em, sqi, eps = _w3d_inputs(srvo3_sized, nwf, nen, ecut_eps, 3)
     o = pipeline_w3d.run_w3d(w, W, sqi, eps, em, ecut_eps)
     g = W3d(0)
     try:
@@ -90,3 +99,29 @@ def test_w3d_on_a_24_cubed_grid(srvo3_sized):
     ef, ew = relf(func, o["func"]), relf(funcw, o["funcw"])
     print(f"w3d on 24^3: func {ef:.3e} funcw {ew:.3e}", flush=True)
     assert ef <= 1e-6 and ew <= 1e-6
+
+
+def test_j3d_with_more_than_1024_eps_rows(srvo3_sized):
+    from oracle import pipeline_w3d
+    nwf, nen, ecut_eps = 2, 2, 32.0
+    rlist = [(0, 0, 0), (1, 0, 0)]
+    w, W, em, sqi, eps = _w3d_inputs(srvo3_sized, nwf, nen, ecut_eps, 5)
+    assert min(e.shape[-1] for e in eps) > 1024
+    oj = pipeline_w3d.run_j3d(w, W, sqi, eps, em, ecut_eps, rlist)
+    g = W3d(0)
+    try:
+        g.open(w, ecut_eps, em)
+        g.set_wannier(W)
+        g.set_q(sqi)
+        for iq, e in enumerate(eps):
+            g.set_epsinv(iq, e)
+        x, j, _, _ = g.j3d(rlist)
+        with pytest.raises(RpError, match="exceeds the 1024"):
+            g.funcs()
+    finally:
+        g.close()
+    jo = np.moveaxis(oj["J"], 1, 0)  # [ie][r][j][i]
+    ex, ej = relf(x, oj["X"]), relf(j, jo)
+    print(f"j3d at NG_eps {max(e.shape[-1] for e in eps)}: X {ex:.3e} J {ej:.3e}", flush=True)
+    assert ex <= 1e-6 and ej <= 1e-6
+    assert np.abs(j - jo).max() * AU <= 1e-3
