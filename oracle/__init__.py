@@ -56,7 +56,11 @@ def lib():
         L.ro_ttrhdrn_mkidx.argtypes = [I32, I32, I32, P, P, P, P]
         L.ro_ttrhdrn_chi0.argtypes = [D, D, I32, I32, I32, P, P, P, D, D, D, D, I32, P, P]
         L.ro_ttrhdrn_simple.argtypes = [D, D, I32, I32, I32, P, P, P, P]
-        L.ro_ttrhdrn_calc.argtypes = [D, D, P, P, I32, P, P]
+        L.ro_ttrhdrn_calc.argtypes = [D, D, P, P, I32, P, P, I32]
+        L.ro_ttrhdrn_calc_quad.argtypes = [D, D, P, P, I32, P, P]
+        L.ro_ttrhdrn_chi0_closed.argtypes = L.ro_ttrhdrn_chi0.argtypes
+        L.ro_ttrhdrn_chi0_quad.argtypes = L.ro_ttrhdrn_chi0.argtypes
+        L.ro_tet_series.argtypes = [P, I32, P, P, P]
         L.ro_tet_sub.argtypes = [I32, P, I32, P, P]
         L.ro_dft3.argtypes = [I32, I32, I32, I32, P, P]
         L.ro_pairdens.argtypes = [I32, P, P, I32, P, P, I32, I32, I32, I32, P, P, P]
@@ -184,14 +188,21 @@ def mkidx(nkb, b: np.ndarray) -> np.ndarray:
     return idx
 
 
-def ttrhdrn_chi0(nkb, idx, e1, e2, ef, delt, dsgn, em, dmna=1e-3, dmnr=1e-3, xo=None):
+def ttrhdrn_chi0(nkb, idx, e1, e2, ef, delt, dsgn, em, dmna=1e-3, dmnr=1e-3, xo=None,
+                 mode="stable"):
+    """ttrhdrn_chi0 accumulated into xo [nk3][ne].  mode "stable": the series
+    where it converges, else the closed forms (the oracle's evaluation);
+    "closed": the reference's closed forms everywhere in double; "quad": the
+    closed forms in __float128 (the arbiter)."""
+    fn = {"stable": "ro_ttrhdrn_chi0", "closed": "ro_ttrhdrn_chi0_closed",
+          "quad": "ro_ttrhdrn_chi0_quad"}[mode]
     nk3 = nkb[0] * nkb[1] * nkb[2]
     em = np.ascontiguousarray(em, np.complex128)
     if xo is None:
         xo = np.zeros((nk3, len(em)), np.complex128)
     e1 = np.ascontiguousarray(e1, np.complex128)
     e2 = np.ascontiguousarray(e2, np.complex128)
-    lib().ro_ttrhdrn_chi0(dmna, dmnr, nkb[0], nkb[1], nkb[2], ptr(idx), ptr(e1), ptr(e2),
+    getattr(lib(), fn)(dmna, dmnr, nkb[0], nkb[1], nkb[2], ptr(idx), ptr(e1), ptr(e2),
                           ef, ef, delt, dsgn, len(em), ptr(em), ptr(xo))
     return xo
 

@@ -293,81 +293,13 @@ void ro_ttrhdrn_mkidx(int n1, int n2, int n3, const double *b1, const double *b2
           }
 }
 
-/* Closed forms of the generalized tetrahedron method (Fujiwara, Yamamoto,
- * Ishii, J. Phys. Soc. Jpn. 72, 777 (2003)) exactly as evaluated at
- * m_tetrahedron_20170325.F:304-451, for one frequency e. */
-static ro_z sub1g30g21(ro_z g0, ro_z g1, ro_z e) {
-  ro_z d = g0 - g1;
-  ro_z x = (0.5 * (g0 + g1) + e) / (d * d);
-  x = x - clog((g0 + e) / (g1 + e)) * (g0 + e) * (g1 + e) / (d * d * d);
-  return x;
-}
-static ro_z sub1g30g20(ro_z g0, ro_z g1, ro_z e) {
-  ro_z d = g0 - g1;
-  ro_z x = (g0 - 3.0 * g1 - 2.0 * e) * 0.25 / (d * d);
-  x = x + (g1 + e) * (g1 + e) * clog((g0 + e) / (g1 + e)) * 0.5 / (d * d * d);
-  return x;
-}
-static ro_z sub1g30(ro_z g0, ro_z g1, ro_z g2, ro_z e) {
-  ro_z x = 0.5 / (g0 - g2);
-  ro_z p = (g0 - g2) * (g0 - g1);
-  x = x - 0.5 / (p * p) * (g0 + e) *
-              (-2.0 * (g1 + e) * (g2 + e) + (g0 + e) * (g1 + g2 + 2.0 * e)) *
-              clog((g0 + e) / (g1 + e));
-  x = x + 0.5 / ((g0 - g2) * (g0 - g2)) * ((g2 + e) * (g2 + e)) *
-              clog((g1 + e) / (g2 + e)) / (g1 - g2);
-  x = x + 0.5 / ((g0 - g2) * (g0 - g1)) * (g1 + e);
-  return x;
-}
-static ro_z sub1normal(ro_z g0, ro_z g1, ro_z g2, ro_z g3, ro_z e) {
-  ro_z l1 = (g1 + e) * (g1 + e) * clog((g1 + e) / (g0 + e));
-  ro_z x = (-l1) / ((g0 - g2) * (g0 - g3) * (g0 - g1)) * 0.5;
-  x = x - (l1 - (g2 + e) * (g2 + e) * clog((g2 + e) / (g0 + e))) /
-              ((g0 - g2) * (g1 - g2) * (g2 - g3)) * 0.5;
-  x = x + (l1 - (g3 + e) * (g3 + e) * clog((g3 + e) / (g0 + e))) /
-              ((g1 - g3) * (g0 - g3) * (g2 - g3)) * 0.5;
-  return x;
-}
-static ro_z sub2simple(ro_z g0, ro_z e) { return 1.0 / ((g0 + e) * 24.0); }
-static ro_z sub2g30g20(ro_z g0, ro_z g1, ro_z e) {
-  ro_z d = g0 - g1, d2 = d * d;
-  ro_z x = (d * (g0 + 2.0 * g1 + 3.0 * e) - 6.0 * (g0 + e) * (g1 + e)) / (12.0 * (d2 * d));
-  x = x + (g0 + e) * ((g1 + e) * (g1 + e)) * clog((g0 + e) / (g1 + e)) / (d2 * d2) * 0.5;
-  return x;
-}
-static ro_z sub2g30g10(ro_z g0, ro_z g2, ro_z e) {
-  ro_z a = g0 + e, c = g2 + e, d = g0 - g2, d2 = d * d;
-  ro_z x = (2.0 * (a * a) - 7.0 * a * c + 11.0 * (c * c)) / (36.0 * (d2 * d));
-  x = x - (c * c * c) * clog(a / c) / (6.0 * (d2 * d2));
-  return x;
-}
-static ro_z sub2g10(ro_z g0, ro_z g2, ro_z g3, ro_z e) {
-  ro_z a = g0 + e, c2 = g2 + e, c3 = g3 + e;
-  ro_z x = a * ((a * a) + 5.0 * c2 * c3 - 3.0 * a * (g2 + g3 + 2.0 * e)) /
-           (12.0 * ((g0 - g2) * (g0 - g2)) * ((g0 - g3) * (g0 - g3)));
-  ro_z d02 = g0 - g2, d03 = g0 - g3;
-  x = x - (c2 * c2 * c2) * clog(c2 / a) / (6.0 * (d02 * d02 * d02) * (g2 - g3));
-  x = x + (c3 * c3 * c3) * clog(c3 / a) / (6.0 * (d03 * d03 * d03) * (g2 - g3));
-  return x;
-}
-static ro_z sub2g30(ro_z g0, ro_z g1, ro_z g2, ro_z e) {
-  ro_z a = g0 + e, b = g1 + e, c = g2 + e;
-  ro_z d01 = g0 - g1, d02 = g0 - g2, d12 = g1 - g2;
-  ro_z x = ((a * a) * d12 + (b * b) * d02) / (6.0 * d02 * (d01 * d01) * d12);
-  x = x - (a * a) * (2.0 * b * d02 + c * d01) * clog(a / c) /
-              (6.0 * (d02 * d02) * (d01 * d01 * d01));
-  x = x + (b * b) * (2.0 * a * d12 - c * d01) * clog(b / c) /
-              (6.0 * (d01 * d01 * d01) * (d12 * d12));
-  return x;
-}
-static ro_z sub2normal(ro_z g0, ro_z g1, ro_z g2, ro_z g3, ro_z e) {
-  ro_z a = g0 + e, b = g1 + e, c = g2 + e, d = g3 + e;
-  ro_z x = -(b * b) / (6.0 * (g0 - g1) * (g1 - g2) * (g1 - g3));
-  x = x - (c * c * c) * clog(c / b) / (6.0 * (g0 - g2) * ((g1 - g2) * (g1 - g2)) * (g2 - g3));
-  x = x + (d * d * d) * clog(d / b) / (6.0 * (g0 - g3) * ((g1 - g3) * (g1 - g3)) * (g2 - g3));
-  x = x + (a * a * a) * clog(a / b) / (6.0 * ((g0 - g1) * (g0 - g1)) * (g0 - g2) * (g0 - g3));
-  return x;
-}
+/* The generalized-tetrahedron weights (closed forms, chk1, calc, ttct,
+ * ttrhdrn_chi0) in double: see tet_forms.h, which tet_quad.c instantiates in
+ * __float128 as well. */
+#define TF_Z ro_z
+#define TF_LOG clog
+#define TF(name) d_##name
+#include "tet_forms.h"
 
 /* The closed forms above with the reference's argument lists (g[0..3] as its
  * g0..g3; the unused ones ignored): which as in the checker's rpref_tet_sub. */
@@ -376,229 +308,50 @@ int ro_tet_sub(int which, const ro_z *g, int ne, const ro_z *em, ro_z *x) {
     ro_z e = em[ie];
     switch (which) {
     case 0: x[ie] = 1.0 / ((g[0] + e) * 6.0); break;
-    case 1: x[ie] = sub1g30g21(g[0], g[1], e); break;
-    case 2: x[ie] = sub1g30g20(g[0], g[1], e); break;
-    case 3: x[ie] = sub1g30(g[0], g[1], g[2], e); break;
-    case 4: x[ie] = sub1normal(g[0], g[1], g[2], g[3], e); break;
-    case 5: x[ie] = sub2simple(g[0], e); break;
-    case 6: x[ie] = sub2g30g20(g[0], g[1], e); break;
-    case 7: x[ie] = sub2g30g20(g[1], g[0], e); break; /* sub2g30g21(g0,g1,..) */
-    case 8: x[ie] = sub2g30g10(g[0], g[2], e); break;
-    case 9: x[ie] = sub2g10(g[0], g[2], g[3], e); break;
-    case 10: x[ie] = sub2g30(g[0], g[1], g[2], e); break;
-    case 11: x[ie] = sub2normal(g[0], g[1], g[2], g[3], e); break;
+    case 1: x[ie] = d_sub1g30g21(g[0], g[1], e); break;
+    case 2: x[ie] = d_sub1g30g20(g[0], g[1], e); break;
+    case 3: x[ie] = d_sub1g30(g[0], g[1], g[2], e); break;
+    case 4: x[ie] = d_sub1normal(g[0], g[1], g[2], g[3], e); break;
+    case 5: x[ie] = d_sub2simple(g[0], e); break;
+    case 6: x[ie] = d_sub2g30g20(g[0], g[1], e); break;
+    case 7: x[ie] = d_sub2g30g20(g[1], g[0], e); break; /* sub2g30g21(g0,g1,..) */
+    case 8: x[ie] = d_sub2g30g10(g[0], g[2], e); break;
+    case 9: x[ie] = d_sub2g10(g[0], g[2], g[3], e); break;
+    case 10: x[ie] = d_sub2g30(g[0], g[1], g[2], e); break;
+    case 11: x[ie] = d_sub2normal(g[0], g[1], g[2], g[3], e); break;
     default: return -1;
     }
   }
   return 0;
 }
 
-/* chk1 (m_tetrahedron_20170325.F:457-503): degeneracy classes of 4 vertices.
- * Vertices are compared in the reference's rotated order p = mod(i,4)+1. */
-static void chk1(double dmna, double dmnr, const ro_z *g, int *ia2, int *ia3,
-                 int *ia4, int *i3out, int *n2) {
-  int i3 = 0, i4 = 0;
-  for (int v = 0; v < 4; ++v) { ia2[v] = 0; ia3[v] = -1; ia4[v] = -1; }
-  for (int i1 = 1; i1 <= 3; ++i1)
-    for (int i2 = i1 + 1; i2 <= 4; ++i2) {
-      int p = i1 % 4, q = i2 % 4; /* 0-based form of mod(i,4)+1 */
-      double d1 = cabs(g[p] - g[q]);
-      double d2 = cabs(g[p] + g[q]) * 0.5;
-      if (d1 < dmna || d1 / d2 < dmnr) {
-        ++i3;
-        if (!ia2[p]) { ia3[i4++] = p; ia2[p] = 1; }
-        if (!ia2[q]) { ia3[i4++] = q; ia2[q] = 1; }
-      }
-    }
-  int i5 = 0;
-  for (int i1 = 1; i1 <= 4; ++i1) {
-    int p = i1 % 4;
-    if (!ia2[p]) ia4[i5++] = p;
-  }
-  *i3out = i3;
-  *n2 = i4;
-}
-
-/* calc (m_tetrahedron_20170325.F:209-302) over corner slots c[4] of arrays
- * fk/gk; xo + ne*c[v] receives the corner contributions. */
-static int calc_slots(double dmna, double dmnr, const int *c, const ro_z *fk,
-                      const ro_z *gk, int ne, const ro_z *em, ro_z *xo) {
-  ro_z g[4], f[4];
-  for (int v = 0; v < 4; ++v) { g[v] = gk[c[v]]; f[v] = fk[c[v]]; }
-  int ia2[4], a3[4], a4[4], i3, n2;
-  chk1(dmna, dmnr, g, ia2, a3, a4, &i3, &n2);
+int ro_tet_series(const ro_z *gt, int ne, const ro_z *em, ro_z *w, int32_t *used) {
+  int n = 0;
   for (int ie = 0; ie < ne; ++ie) {
-    ro_z e = em[ie];
-#define XO(v) xo[(size_t)ne * c[v] + ie]
-    switch (n2) {
-    case 0: {
-      ro_z c1 = sub2normal(g[a4[1]], g[a4[0]], g[a4[2]], g[a4[3]], e);
-      ro_z c2 = sub2normal(g[a4[0]], g[a4[1]], g[a4[2]], g[a4[3]], e);
-      ro_z c3 = sub2normal(g[a4[0]], g[a4[2]], g[a4[1]], g[a4[3]], e);
-      ro_z c4 = sub1normal(g[a4[0]], g[a4[3]], g[a4[1]], g[a4[2]], e);
-      XO(a4[0]) += f[a4[0]] * c1;
-      XO(a4[1]) += f[a4[1]] * c2;
-      XO(a4[2]) += f[a4[2]] * c3;
-      XO(a4[3]) += f[a4[3]] * (c4 - c3 - c2 - c1);
-      break;
-    }
-    case 2: {
-      ro_z q1 = (g[a3[0]] + g[a3[1]]) / 2.0;
-      ro_z c1 = sub2g10(q1, g[a4[0]], g[a4[1]], e);
-      ro_z c2 = sub2g30(q1, g[a4[0]], g[a4[1]], e);
-      ro_z c3 = sub1g30(q1, g[a4[1]], g[a4[0]], e);
-      XO(a3[0]) += f[a3[0]] * c1;
-      XO(a3[1]) += f[a3[1]] * c1;
-      XO(a4[0]) += f[a4[0]] * c2;
-      XO(a4[1]) += f[a4[1]] * (c3 - c2 - c1 - c1);
-      break;
-    }
-    case 3: {
-      ro_z q1 = (g[a3[0]] + g[a3[1]] + g[a3[2]]) / 3.0;
-      ro_z c1 = sub2g30g10(q1, g[a4[0]], e);
-      ro_z c2 = sub1g30g20(q1, g[a4[0]], e);
-      XO(a3[0]) += f[a3[0]] * c1;
-      XO(a3[1]) += f[a3[1]] * c1;
-      XO(a3[2]) += f[a3[2]] * c1;
-      XO(a4[0]) += f[a4[0]] * (c2 - c1 - c1 - c1);
-      break;
-    }
-    case 4:
-      if (i3 == 2) {
-        ro_z q1 = (g[a3[0]] + g[a3[1]]) / 2.0, q2 = (g[a3[2]] + g[a3[3]]) / 2.0;
-        ro_z c1 = sub2g30g20(q1, q2, e); /* sub2g30g21(q2,q1,..) */
-        ro_z c2 = sub1g30g21(q1, q2, e);
-        XO(a3[0]) += f[a3[0]] * c1;
-        XO(a3[1]) += f[a3[1]] * c1;
-        XO(a3[2]) += f[a3[2]] * (c2 - c1 - c1) * 0.5;
-        XO(a3[3]) += f[a3[3]] * (c2 - c1 - c1) * 0.5;
-      } else if (i3 >= 3 && i3 <= 6) {
-        ro_z q1 = (g[a3[0]] + g[a3[1]] + g[a3[2]] + g[a3[3]]) / 4.0;
-        ro_z c1 = sub2simple(q1, e);
-        XO(a3[0]) += c1 * f[a3[0]];
-        XO(a3[1]) += c1 * f[a3[1]];
-        XO(a3[2]) += c1 * f[a3[2]];
-        XO(a3[3]) += c1 * f[a3[3]];
-      } else {
-        return -1;
-      }
-      break;
-    default:
-      return -1;
-    }
-#undef XO
+    used[ie] = tet_series(gt, em[ie], w + 4 * ie);
+    n += used[ie];
   }
-  return 0;
+  return n;
 }
 
-void ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk,
-                     int ne, const ro_z *em, ro_z *xo) {
+int ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk, int ne,
+                    const ro_z *em, ro_z *xo, int use_series) {
   const int c[4] = {0, 1, 2, 3};
-  calc_slots(dmna, dmnr, c, fk, gk, ne, em, xo);
+  return d_calc(dmna, dmnr, c, fk, gk, ne, em, xo, use_series);
 }
 
-#define TT_NMAX 99
-/* ttct (m_tetrahedron_20170325.F:143-207): split sub-tetrahedra by the plane
- * dvtx = dval along edges, then keep those on the imod side. */
-static int ttct(double dval, const double *dvtx, int imod, int *ntet,
-                double dwgt[][4][4], double *dvol) {
-  unsigned char bvtx[TT_NMAX][4], btsk[TT_NMAX];
-  memset(bvtx, 0, sizeof bvtx);
-  memset(btsk, 0, sizeof btsk);
-  int itet = 0;
-  while (itet < *ntet) {
-    ++itet;
-    int t = itet - 1;
-    double v0[4];
-    unsigned char b0[4];
-    for (int i = 0; i < 4; ++i) {
-      v0[i] = dwgt[t][i][0] * dvtx[0] + dwgt[t][i][1] * dvtx[1] +
-              dwgt[t][i][2] * dvtx[2] + dwgt[t][i][3] * dvtx[3];
-      b0[i] = bvtx[t][i];
-    }
-    int b1 = 0, i1 = 0, i2 = 0;
-    for (i1 = 0; i1 < 3 && !b1; ++i1) {
-      if (b0[i1]) continue;
-      for (i2 = i1 + 1; i2 < 4; ++i2) {
-        if (b0[i2]) continue;
-        b1 = (v0[i1] - dval) * (dval - v0[i2]) > 0.0;
-        if (b1) break;
-      }
-      if (b1) break;
-    }
-    if (b1) {
-      double d1 = (v0[i2] - dval) / (v0[i2] - v0[i1]);
-      double d2 = (dval - v0[i1]) / (v0[i2] - v0[i1]);
-      if (*ntet + 1 > TT_NMAX) return -1; /* "ntet.gt.nmax" */
-      int n = (*ntet)++;
-      memcpy(bvtx[n], bvtx[t], 4);
-      memcpy(dwgt[n], dwgt[t], sizeof dwgt[0]);
-      bvtx[n][i1] = 1;
-      for (int j = 0; j < 4; ++j) dwgt[n][i1][j] = dwgt[t][i1][j] * d1 + dwgt[t][i2][j] * d2;
-      dvol[n] = dvol[t] * d1;
-      bvtx[t][i2] = 1;
-      for (int j = 0; j < 4; ++j) dwgt[t][i2][j] = dwgt[t][i1][j] * d1 + dwgt[t][i2][j] * d2;
-      dvol[t] = dvol[t] * d2;
-      --itet;
-    } else {
-      for (int i = 0; i < 4; ++i) {
-        if (b0[i]) continue;
-        btsk[t] = btsk[t] || ((double)imod * (v0[i] - dval) > 0.0);
-      }
-    }
-  }
-  int k = 0;
-  for (int i = 0; i < *ntet; ++i) {
-    if (!btsk[i]) continue;
-    if (k != i) { memcpy(dwgt[k], dwgt[i], sizeof dwgt[0]); dvol[k] = dvol[i]; }
-    ++k;
-  }
-  *ntet = k;
-  return 0;
-}
-
-/* dwgt[t][v][j]: weight of original corner j in vertex v of sub-tetrahedron t
- * (the reference's dwgt(j, v, t)). */
 void ro_ttrhdrn_chi0(double dmna, double dmnr, int n1, int n2, int n3,
                      const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
                      double f2, double delt, double dsgn, int ne, const ro_z *em,
                      ro_z *xo) {
-  int nt = n1 * n2 * n3 * 6;
-  double (*dwgt)[4][4] = malloc(sizeof(double[TT_NMAX][4][4]));
-  double dvol[TT_NMAX];
-  ro_z *xxo = malloc(sizeof(ro_z) * 4 * ne);
-  const int slots[4] = {0, 1, 2, 3};
-  for (int it = 0; it < nt; ++it) {
-    const int32_t *c = idx + 4 * it;
-    memset(dwgt[0], 0, sizeof dwgt[0]);
-    for (int v = 0; v < 4; ++v) dwgt[0][v][v] = 1.0;
-    dvol[0] = 1.0;
-    int ntet = 1;
-    double ee1[4], ee2[4];
-    for (int v = 0; v < 4; ++v) ee2[v] = creal(e2[c[v]]);
-    ttct(f2, ee2, +1, &ntet, dwgt, dvol);
-    for (int v = 0; v < 4; ++v) ee1[v] = creal(e1[c[v]]);
-    ttct(f1, ee1, -1, &ntet, dwgt, dvol);
-    for (int t = 0; t < ntet; ++t) {
-      ro_z ffk[4], ggk[4];
-      for (int i = 0; i < 4 * ne; ++i) xxo[i] = 0.0;
-      for (int v = 0; v < 4; ++v) {
-        double s = 0.0;
-        for (int j = 0; j < 4; ++j) s += (ee2[j] - ee1[j]) * dwgt[t][v][j];
-        ffk[v] = dvol[t] * dsgn;
-        ggk[v] = -(s - I * delt) * dsgn;
-      }
-      calc_slots(dmna, dmnr, slots, ffk, ggk, ne, em, xxo);
-      for (int v = 0; v < 4; ++v)
-        for (int j = 0; j < 4; ++j) {
-          double w = dwgt[t][v][j];
-          ro_z *dst = xo + (size_t)ne * c[j];
-          for (int ie = 0; ie < ne; ++ie) dst[ie] += xxo[ne * v + ie] * w;
-        }
-    }
-  }
-  free(xxo);
-  free(dwgt);
+  d_chi0(dmna, dmnr, n1, n2, n3, idx, e1, e2, f1, f2, delt, dsgn, ne, em, xo, 1);
+}
+
+void ro_ttrhdrn_chi0_closed(double dmna, double dmnr, int n1, int n2, int n3,
+                            const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
+                            double f2, double delt, double dsgn, int ne, const ro_z *em,
+                            ro_z *xo) {
+  d_chi0(dmna, dmnr, n1, n2, n3, idx, e1, e2, f1, f2, delt, dsgn, ne, em, xo, 0);
 }
 
 void ro_ttrhdrn_simple(double dmna, double dmnr, int n1, int n2, int n3,
@@ -608,7 +361,7 @@ void ro_ttrhdrn_simple(double dmna, double dmnr, int n1, int n2, int n3,
   int nt = n1 * n2 * n3 * 6;
   for (int it = 0; it < nt; ++it) {
     int c[4] = {idx[4 * it], idx[4 * it + 1], idx[4 * it + 2], idx[4 * it + 3]};
-    calc_slots(dmna, dmnr, c, fk, gk, 1, &em0, xo);
+    d_calc(dmna, dmnr, c, fk, gk, 1, &em0, xo, 1);
   }
 }
 

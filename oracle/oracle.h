@@ -61,23 +61,45 @@ void ro_make_c0(int itrs, int ng, const int32_t *kg, const int32_t *rw,
  * the 0-based k index of corner v of tetrahedron t (6 per k-cube). */
 void ro_ttrhdrn_mkidx(int n1, int n2, int n3, const double *b1, const double *b2,
                       const double *b3, int32_t *idx);
-/* m_tetrahedron_20170325.F:99-141; xo[ie + ne*ik] is ACCUMULATED. */
+/* m_tetrahedron_20170325.F:99-141; xo[ie + ne*ik] is ACCUMULATED.  The
+ * vertex weights by the series of tet_forms.h where it converges
+ * (|g - c| < 0.2 |e + c|), else by the reference's closed forms. */
 void ro_ttrhdrn_chi0(double dmna, double dmnr, int n1, int n2, int n3,
                      const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
                      double f2, double delt, double dsgn, int ne, const ro_z *em,
                      ro_z *xo);
+/* The same with the reference's closed forms everywhere (as it evaluates). */
+void ro_ttrhdrn_chi0_closed(double dmna, double dmnr, int n1, int n2, int n3,
+                            const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
+                            double f2, double delt, double dsgn, int ne, const ro_z *em,
+                            ro_z *xo);
+/* The closed forms evaluated in __float128 (tet_quad.c; cuts, classes and
+ * merged nodes in double as the reference forms them); xo accumulated in
+ * __float128 from its input and rounded back. */
+void ro_ttrhdrn_chi0_quad(double dmna, double dmnr, int n1, int n2, int n3,
+                          const int32_t *idx, const ro_z *e1, const ro_z *e2, double f1,
+                          double f2, double delt, double dsgn, int ne, const ro_z *em,
+                          ro_z *xo);
 /* m_tetrahedron_20170325.F:63-76 (ne = 1, em = 0); xo accumulated */
 void ro_ttrhdrn_simple(double dmna, double dmnr, int n1, int n2, int n3,
                        const int32_t *idx, const ro_z *fk, const ro_z *gk, ro_z *xo);
-/* m_tetrahedron_20170325.F:209-302 for one tetrahedron: fk[4], gk[4] in,
- * xo[ie + ne*v] accumulated (v = 0..3). Exposed for unit tests. */
 /* One tetrahedron closed form (m_tetrahedron_20170325.F:304-451) at ne
  * frequencies, g[0..3] the reference's g0..g3; which: 0 sub1simple,
  * 1 sub1g30g21, 2 sub1g30g20, 3 sub1g30, 4 sub1normal, 5 sub2simple,
  * 6 sub2g30g20, 7 sub2g30g21, 8 sub2g30g10, 9 sub2g10, 10 sub2g30, 11 sub2normal. */
 int ro_tet_sub(int which, const ro_z *g, int ne, const ro_z *em, ro_z *x);
-void ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk,
-                     int ne, const ro_z *em, ro_z *xo);
+/* calc (m_tetrahedron_20170325.F:209-302) for one tetrahedron: fk[4], gk[4]
+ * in, xo[ie + ne*v] accumulated (v = 0..3); use_series as ro_ttrhdrn_chi0
+ * (1) or ro_ttrhdrn_chi0_closed (0).  Returns -1 for a rejected class. */
+int ro_ttrhdrn_calc(double dmna, double dmnr, const ro_z *fk, const ro_z *gk, int ne,
+                    const ro_z *em, ro_z *xo, int use_series);
+/* The same calc in __float128, closed forms only (tet_quad.c). */
+int ro_ttrhdrn_calc_quad(double dmna, double dmnr, const ro_z *fk, const ro_z *gk, int ne,
+                         const ro_z *em, ro_z *xo);
+/* W_v of the node set gt[4] by the series of tet_forms.h at ne frequencies:
+ * w[4*ie + v]; used[ie] = 0 where it does not converge fast enough (w unset).
+ * Returns the number of frequencies evaluated. */
+int ro_tet_series(const ro_z *gt, int ne, const ro_z *em, ro_z *w, int32_t *used);
 
 /* Brute-force 3D DFT on an nx*ny*nz grid (x fastest).  dir = -1 matches
  * fft3_fw (e^{-i}, scaled 1/N), dir = +1 matches fft3_bw (e^{+i}, unscaled),

@@ -193,6 +193,7 @@ struct GjRegShared {
   int wi[16];
   double2 rowp[kPanelMax], rowc[kPanelMax], prow[kPanelMax];
   int orig[768];
+  int origb[768];  // the rows' sources since the last reset (one half of a 32-column panel)
   int nmove;
   int singular;
 };
@@ -248,6 +249,9 @@ __device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared
     const int t = sh.orig[c];
     sh.orig[c] = sh.orig[p];
     sh.orig[p] = t;
+    const int tb = sh.origb[c];
+    sh.origb[c] = sh.origb[p];
+    sh.origb[p] = tb;
   }
   if (i == p) {
 #pragma unroll
@@ -453,6 +457,228 @@ __global__ void __launch_bounds__(768) k_gj_step(double2* __restrict__ a, int n,
   gj_reg_factor(r, sh.p, i, n, cj, mine, b, m, ipiv, moves_next + (size_t)b * kMoveInts, status);
 }
 
+// ---------------------------------------------------------------------------
+// Register path with 32-column steps (n <= 768).  Each launch applies the
+// rank-32 transform of one 32-column panel (two 16-column halves factored by
+// the previous launch) to every other column: half the passes over the
+// matrices of 16-column steps.  One block per matrix brings the next panel up
+// to date and factors it in the same launch (look-ahead), as two halves in
+// registers: half A, then half B after A's transform, then A after B's.
+// The transform of the two halves together differs from the identity in the
+// 32 panel columns only, so every other column j becomes
+//   a'(:,j) = P a(:,j) with the panel rows zeroed + W (P a)(panel rows, j),
+// W the panel's 32 final columns and P the panel's row interchanges (B's act
+// on rows below A's panel rows only, so A's transform commutes past them).
+constexpr int kSuper = 32;
+constexpr int kMoveMax32 = 2 * kSuper;          // panel rows and rows they displaced
+constexpr int kMoveInts32 = 1 + 2 * kMoveMax32;  // {count, dst[64], src[64]}
+
+struct Gj32Shared {
+  GjRegShared p;
+  double2 vals[kMoveMax32][kPanelMax];  // old values of the moved rows in 16 columns
+  int dst[kMoveMax32], src[kMoveMax32];
+  int nm;
+};
+
+// Loads into sh.vals the old values of the nm moved rows (sources sh.src) in
+// columns [cj, cj + ntj) and ends with a barrier.
+__device__ __forceinline__ void gj32_stage(Gj32Shared& sh, const double2* __restrict__ m, int n,
+                                           int cj, int ntj, int nm) {
+  for (int x = threadIdx.x; x < nm * kPanelMax; x += blockDim.x) {
+    const int e = x / kPanelMax, t = x % kPanelMax;
+    sh.vals[e][t] = t < ntj ? m[sh.src[e] + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+// Row i of columns [cj, cj + ntj) after the rank-W transform of the panel
+// [ck, ck + W): the staged move list (sh.dst/sh.vals, its first W entries the
+// panel rows in order) and the panel columns W(i, q) read from m.
+template <int W>
+__device__ __forceinline__ void gj32_apply(double2 (&r)[kPanelMax], const Gj32Shared& sh,
+                                           const double2* __restrict__ m, int n, int ck, int cj,
+                                           int ntj, int nm, int i, bool mine) {
+  const bool inpanel = i >= ck && i < ck + W;
+  int slot = -1;
+  if (!inpanel)
+    for (int e = W; e < nm; ++e)
+      if (sh.dst[e] == i) slot = e;
+#pragma unroll
+  for (int t = 0; t < kPanelMax; ++t)
+    r[t] = (!mine || inpanel || t >= ntj) ? make_double2(0.0, 0.0)
+           : slot >= 0                    ? sh.vals[slot][t]
+                                          : m[i + (size_t)n * (cj + t)];
+  if (!mine) return;
+  for (int q = 0; q < W; ++q) {
+    const double2 wq = m[i + (size_t)n * (ck + q)];
+    const zd w = mkz(wq.x, wq.y);
+#pragma unroll
+    for (int t = 0; t < kPanelMax; ++t) {
+      const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.vals[q][t].x, sh.vals[q][t].y);
+      r[t] = make_double2(u.x, u.y);
+    }
+  }
+}
+
+// The move list of rows whose source differs from themselves in map (orig or
+// origb of sh.p), the nb panel rows [c0, c0 + nb) first, into sh.dst/src/nm.
+__device__ __forceinline__ void gj32_list(Gj32Shared& sh, const int* map, int n, int c0, int nb,
+                                          int i, bool mine) {
+  if (i == 0) sh.nm = nb;
+  if (i < nb) {
+    sh.dst[i] = c0 + i;
+    sh.src[i] = map[c0 + i];
+  }
+  __syncthreads();
+  if (mine && (i < c0 || i >= c0 + nb) && map[i] != i) {
+    const int e = atomicAdd(&sh.nm, 1);
+    sh.dst[e] = i;
+    sh.src[e] = map[i];
+  }
+  __syncthreads();
+}
+
+// Factors the 32-column panel [c, c + 32) of matrix b: half A's columns are
+// in r (up to date), half B's in m (up to date).  Leaves the panel's final
+// columns in m and its move list in mv.
+__device__ __forceinline__ void gj32_chain(double2 (&r)[kPanelMax], Gj32Shared& sh, double2* m,
+                                           int n, int c, int b, int* __restrict__ ipiv,
+                                           int* __restrict__ mv, int* __restrict__ status) {
+  const int i = threadIdx.x;
+  const bool mine = i < n;
+  if (mine) sh.p.orig[i] = sh.p.origb[i] = i;
+  if (i == 0) sh.p.singular = 0;
+  __syncthreads();
+  // half A
+  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, sh.p, i, n, c, mine, b, ipiv,
+               status);
+  __syncthreads();
+  if (sh.p.singular) {
+    if (i == 0) mv[0] = -1;
+    return;  // uniform
+  }
+  if (mine)
+#pragma unroll
+    for (int q = 0; q < kPanelMax; ++q) m[i + (size_t)n * (c + q)] = r[q];
+  // half B after A's transform (A's moves: origb since the reset)
+  gj32_list(sh, sh.p.origb, n, c, kPanelMax, i, mine);
+  gj32_stage(sh, m, n, c + kPanelMax, kPanelMax, sh.nm);
+  gj32_apply<kPanelMax>(r, sh, m, n, c, c + kPanelMax, kPanelMax, sh.nm, i, mine);
+  if (mine) sh.p.origb[i] = i;
+  __syncthreads();
+  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, sh.p, i, n, c + kPanelMax, mine, b,
+               ipiv, status);
+  __syncthreads();
+  if (sh.p.singular) {
+    if (i == 0) mv[0] = -1;
+    return;  // uniform
+  }
+  if (mine)
+#pragma unroll
+    for (int q = 0; q < kPanelMax; ++q) m[i + (size_t)n * (c + kPanelMax + q)] = r[q];
+  // half A after B's transform (B's moves: origb since its reset)
+  gj32_list(sh, sh.p.origb, n, c + kPanelMax, kPanelMax, i, mine);
+  gj32_stage(sh, m, n, c, kPanelMax, sh.nm);
+  gj32_apply<kPanelMax>(r, sh, m, n, c + kPanelMax, c, kPanelMax, sh.nm, i, mine);
+  __syncthreads();  // every staged read of A is done before A is overwritten
+  if (mine)
+#pragma unroll
+    for (int q = 0; q < kPanelMax; ++q) m[i + (size_t)n * (c + q)] = r[q];
+  // the panel's move list: its 32 rows, then the rows they displaced
+  gj32_list(sh, sh.p.orig, n, c, kSuper, i, mine);
+  for (int e = i; e < sh.nm; e += blockDim.x) {
+    mv[1 + e] = sh.dst[e];
+    mv[1 + kMoveMax32 + e] = sh.src[e];
+  }
+  if (i == 0) mv[0] = sh.nm;
+}
+
+// The first panel [0, 32): one block per matrix.
+__global__ void __launch_bounds__(768) k_gj_first32(double2* __restrict__ a, int n,
+                                                    int* __restrict__ ipiv, int* __restrict__ moves,
+                                                    int* __restrict__ status) {
+  __shared__ Gj32Shared sh;
+  const int b = blockIdx.x, i = threadIdx.x;
+  double2* m = a + (size_t)b * n * n;
+  double2 r[kPanelMax];
+#pragma unroll
+  for (int q = 0; q < kPanelMax; ++q) r[q] = i < n ? m[i + (size_t)n * q] : make_double2(0.0, 0.0);
+  gj32_chain(r, sh, m, n, 0, b, ipiv, moves + (size_t)b * kMoveInts32, status);
+}
+
+// One 32-column step: the rank-32 transform of panel k (columns [ck, ck+32),
+// move list moves_k) on every other column and, when next is set, the
+// factoring of panel k+1.  Blocks [0, nbatch) (next set) are panel blocks:
+// they update the 32 columns of panel k+1 (half B to HBM, half A kept in
+// registers) and factor it (gj32_chain).  Every other block updates 16 other
+// columns of one matrix: those outside [ck, ck + 32 (1 + next)), ngroups per
+// matrix.  The two kinds touch disjoint columns and read panel k's columns
+// only.  With nbatch a multiple of 8 the update blocks of matrix b run on XCD
+// b mod 8.
+__global__ void __launch_bounds__(768) k_gj_step32(double2* __restrict__ a, int n, int ck, int next,
+                                                   int nbatch, int ngroups, int* __restrict__ ipiv,
+                                                   const int* __restrict__ moves_k,
+                                                   int* __restrict__ moves_next,
+                                                   int* __restrict__ status) {
+  __shared__ Gj32Shared sh;
+  const int i = threadIdx.x;
+  const bool mine = i < n;
+  const bool panel = next && (int)blockIdx.x < nbatch;
+  int b, cj = 0, ntj = kPanelMax;
+  if (panel) {
+    b = blockIdx.x;
+  } else {
+    const int u = (int)blockIdx.x - (next ? nbatch : 0);
+    int t;
+    if (nbatch % 8 == 0) {
+      const int x = u % 8, s = u / 8;
+      b = x + 8 * (s / ngroups);
+      t = s % ngroups;
+    } else {
+      b = u / ngroups;
+      t = u % ngroups;
+    }
+    const int skip = kSuper * (1 + next);  // columns [ck, ck + skip) are not in the groups
+    const int t0 = t * kPanelMax;          // groups never straddle ck (multiples of 16)
+    cj = t0 < ck ? t0 : t0 + skip;
+    ntj = min(kPanelMax, n - skip - t0);
+  }
+  double2* m = a + (size_t)b * n * n;
+  const int* mv = moves_k + (size_t)b * kMoveInts32;
+  const int nm = mv[0];
+  if (nm < 0) {  // a zero pivot earlier (the host reports it): nothing more for this matrix
+    if (panel && i == 0) moves_next[(size_t)b * kMoveInts32] = -1;
+    return;  // uniform: every thread read the same word
+  }
+  if (i < nm) {
+    sh.dst[i] = mv[1 + i];
+    sh.src[i] = mv[1 + kMoveMax32 + i];
+  }
+  __syncthreads();
+  double2 r[kPanelMax];
+  if (!panel) {
+    gj32_stage(sh, m, n, cj, ntj, nm);
+    gj32_apply<kSuper>(r, sh, m, n, ck, cj, ntj, nm, i, mine);
+    if (mine)
+#pragma unroll
+      for (int t = 0; t < kPanelMax; ++t)
+        if (t < ntj) m[i + (size_t)n * (cj + t)] = r[t];
+    return;  // uniform per block
+  }
+  // panel k+1: half B brought up to date into HBM, then half A into registers
+  const int c = ck + kSuper;
+  gj32_stage(sh, m, n, c + kPanelMax, kPanelMax, nm);
+  gj32_apply<kSuper>(r, sh, m, n, ck, c + kPanelMax, kPanelMax, nm, i, mine);
+  __syncthreads();  // staged reads of B done before B is overwritten
+  if (mine)
+#pragma unroll
+    for (int t = 0; t < kPanelMax; ++t) m[i + (size_t)n * (c + kPanelMax + t)] = r[t];
+  gj32_stage(sh, m, n, c, kPanelMax, nm);
+  gj32_apply<kSuper>(r, sh, m, n, ck, c, kPanelMax, nm, i, mine);
+  __syncthreads();  // every thread's B row is written before the chain stages B rows
+  gj32_chain(r, sh, m, n, c, b, ipiv, moves_next + (size_t)b * kMoveInts32, status);
+}
+
 // Rank-nb update of every column outside the panel.  grid: (ceil((n-nb)/TJ),
 // nbatch), 256 threads; workgroup = TJ whole columns of one matrix, so the
 // reads of moved rows (staged in LDS before the barrier) never race with
@@ -560,14 +786,31 @@ void launch_eps_build(const double2* chi, const double* qn, int n, int nbatch, i
 
 // Two move lists per matrix: the fused step reads panel k's while its panel
 // blocks write panel k+1's.
-size_t gauss_jordan_work_ints(int nbatch) { return 2 * (size_t)nbatch * kMoveInts; }
+size_t gauss_jordan_work_ints(int nbatch) { return 2 * (size_t)nbatch * kMoveInts32; }
 
 void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, int* status,
                          hipStream_t s) {
   // move list of panel number k: work + (k & 1) * nbatch * kMoveInts
   auto moves = [&](int k) { return work + (size_t)(k & 1) * nbatch * kMoveInts; };
   int c0 = 0, panel = 0;
-  if (n <= 768 && n >= kPanelMax) {
+  if (n <= 768 && n >= kSuper) {
+    // a thread per row (at least 256: the steps stage 64 x 16 moved-row values)
+    const int nt = std::max(256, (n + 63) / 64 * 64);
+    auto moves32 = [&](int k) { return work + (size_t)(k & 1) * nbatch * kMoveInts32; };
+    const int nfull = n / kSuper;
+    hipLaunchKernelGGL(k_gj_first32, dim3(nbatch), dim3(nt), 0, s, a, n, ipiv, moves32(0), status);
+    for (int k = 0; k < nfull; ++k) {
+      const int next = k + 1 < nfull ? 1 : 0;
+      const int nrest = n - kSuper * (1 + next);
+      const int ngroups = (nrest + kPanelMax - 1) / kPanelMax;
+      const int nblk = (next ? nbatch : 0) + nbatch * ngroups;
+      if (nblk > 0)
+        hipLaunchKernelGGL(k_gj_step32, dim3(nblk), dim3(nt), 0, s, a, n, k * kSuper, next, nbatch,
+                           ngroups, ipiv, (const int*)moves32(k), moves32(k + 1), status);
+    }
+    c0 = nfull * kSuper;
+    panel = nfull;  // the LDS path's move lists continue the alternation
+  } else if (n <= 768 && n >= kPanelMax) {
     // a thread per row (at least 256: the step stages 16 x 16 pivot-row values)
     const int nt = std::max(256, (n + 63) / 64 * 64);
     // full panels in registers, each launch updating panel k and factoring k+1

@@ -13,6 +13,15 @@
 // FP64 atomics.  Tetrahedra cut by E_F (metals: partially occupied bands)
 // are left to k_tetra_cut, which runs after this kernel on the same output.
 //
+// Each closed form is the exact integral W_v = int lambda_v / (e + g(lambda))
+// over the unit simplex for its class's nodes (near-equal vertex values
+// replaced by their mean).  Where the nodes are close together against
+// |e + c| (c their mean) the closed forms cancel: their terms are ~24 r^-3
+// times W, r = max_j |g_j - c| / |e + c|, so at high frequency a 1-ulp error
+// of a logarithm becomes 1e-5 of W on small Fermi-surface pieces and more on
+// tiny ones.  For r < kSeriesR every kernel here evaluates the same integral
+// by its convergent series instead (series_w), which loses nothing.
+//
 // The output is the contraction weight d[w][K] = x(k,w) (1 - P_occ P_vir)/NTK
 // with K = pair*NTK + k (the factors of chiqw.F:1404-1406).
 #include <algorithm>
@@ -144,6 +153,70 @@ __device__ __forceinline__ int sel4(int i, int v0, int v1, int v2, int v3) {
 }
 __device__ __forceinline__ zd sel4(int i, zd v0, zd v1, zd v2, zd v3) {
   return mkz(sel4d(i, v0.x, v1.x, v2.x, v3.x), sel4d(i, v0.y, v1.y, v2.y, v3.y));
+}
+
+// The vertex weights of nodes g0..g3 (one shared imaginary part, the
+// broadening) at frequency e by the series of the integral around the mean
+// node c: with s = e + c and d_j = Re g_j - Re c,
+//   W_v = (1/s) sum_n (-1/s)^n n!/(n+4)! P_{v,n},
+//   P_{v,n} = [z^n] prod_j 1/(1 - d_j z) * 1/(1 - d_v z)
+// (the Dirichlet moments of the simplex).  Term n is at most r^n / (24 |s|),
+// r = max_j |d_j| / |s|: the sum runs until r^n < 2^-54 and loses nothing to
+// cancellation.  Returns false, outputs untouched, for r >= kSeriesR, where
+// the closed forms lose at most ~24 r^-3 = 200 ulps.
+constexpr double kSeriesR = 0.5;
+constexpr int kSeriesMax = 64;  // terms at r just below kSeriesR: 55
+// the last index n of the sum for ratio r (r^n < 2^-54)
+__device__ __forceinline__ int series_terms(double r) {
+  return r > 0.0 ? min(kSeriesMax - 1, (int)ceil(-37.42994775023705 / log(r))) : 0;
+}
+__constant__ double kMomentInv[kSeriesMax] = {
+#define RP_MI(n) 1.0 / ((double)((n) + 1) * ((n) + 2) * ((n) + 3) * ((n) + 4))
+    RP_MI(0),  RP_MI(1),  RP_MI(2),  RP_MI(3),  RP_MI(4),  RP_MI(5),  RP_MI(6),  RP_MI(7),
+    RP_MI(8),  RP_MI(9),  RP_MI(10), RP_MI(11), RP_MI(12), RP_MI(13), RP_MI(14), RP_MI(15),
+    RP_MI(16), RP_MI(17), RP_MI(18), RP_MI(19), RP_MI(20), RP_MI(21), RP_MI(22), RP_MI(23),
+    RP_MI(24), RP_MI(25), RP_MI(26), RP_MI(27), RP_MI(28), RP_MI(29), RP_MI(30), RP_MI(31),
+    RP_MI(32), RP_MI(33), RP_MI(34), RP_MI(35), RP_MI(36), RP_MI(37), RP_MI(38), RP_MI(39),
+    RP_MI(40), RP_MI(41), RP_MI(42), RP_MI(43), RP_MI(44), RP_MI(45), RP_MI(46), RP_MI(47),
+    RP_MI(48), RP_MI(49), RP_MI(50), RP_MI(51), RP_MI(52), RP_MI(53), RP_MI(54), RP_MI(55),
+    RP_MI(56), RP_MI(57), RP_MI(58), RP_MI(59), RP_MI(60), RP_MI(61), RP_MI(62), RP_MI(63),
+#undef RP_MI
+};
+__device__ __forceinline__ bool series_w(zd g0, zd g1, zd g2, zd g3, zd e, zd& w0, zd& w1,
+                                         zd& w2, zd& w3) {
+  const zd c = (g0 + g1 + g2 + g3) * 0.25;
+  const zd s = e + c;
+  const double d0 = g0.x - c.x, d1 = g1.x - c.x, d2 = g2.x - c.x, d3 = g3.x - c.x;
+  const double dm = fmax(fmax(fabs(d0), fabs(d1)), fmax(fabs(d2), fabs(d3)));
+  const double as = absz(s);
+  if (!(dm < kSeriesR * as)) return false;
+  const int nmax = series_terms(dm / as);
+  const zd t = -1.0 / s;
+  zd tn = 1.0 / s;
+  double h1 = 1.0, h2 = 1.0, h3 = 1.0, h = 1.0;
+  double p0 = 1.0, p1 = 1.0, p2 = 1.0, p3 = 1.0;
+  zd a0 = mkz(0.0, 0.0), a1 = a0, a2 = a0, a3 = a0;
+  for (int n = 0; n <= nmax; ++n) {
+    const double cf = kMomentInv[n];
+    a0 = a0 + tn * (cf * p0);
+    a1 = a1 + tn * (cf * p1);
+    a2 = a2 + tn * (cf * p2);
+    a3 = a3 + tn * (cf * p3);
+    tn = tn * t;
+    h1 *= d0;
+    h2 = h1 + d1 * h2;
+    h3 = h2 + d2 * h3;
+    h = h3 + d3 * h;
+    p0 = h + d0 * p0;
+    p1 = h + d1 * p1;
+    p2 = h + d2 * p2;
+    p3 = h + d3 * p3;
+  }
+  w0 = a0;
+  w1 = a1;
+  w2 = a2;
+  w3 = a3;
+  return true;
 }
 
 // Log(a/b) from Log a and Log b, on the principal branch (-pi, pi].
@@ -295,6 +368,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const zd K31 = P2 * -s6, K32 = r12 * P1 * s6, K33 = r23 * P3 * s6, K34 = r02 * P0 * s6;
     // sub1normal(A0,A3,A1,A2)
     const zd J1 = P0 * 0.5, J2 = P1 * -0.5, J3 = P2 * -0.5;
+    // the series' nodes for dsgn = +1 (series_w's operations): mean sc, offsets sd
+    const zd sc = (A0 + A1 + A2 + A3) * 0.25;
+    const double sd0 = A0.x - sc.x, sd1 = A1.x - sc.x, sd2 = A2.x - sc.x, sd3 = A3.x - sc.x;
+    const double sdm = fmax(fmax(fabs(sd0), fabs(sd1)), fmax(fabs(sd2), fabs(sd3)));
     // Both items of the tetrahedron: dsgn = s takes the vertex values s*A, so
     // every r_ij takes the factor s: the factors of three of them (K11, K21,
     // K31, J1..J3) change sign, those of four do not (exactly, in IEEE
@@ -304,9 +381,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       const double2 ev = em[f0 + f];
       const zd e = mkz(ev.x, ev.y);
       zd T0 = mkz(0.0, 0.0), T1 = mkz(0.0, 0.0), T2 = mkz(0.0, 0.0), T3 = mkz(0.0, 0.0);
+      // Both signs by the series (the usual case): the nodes of dsgn = -1 are
+      // -A, so d_j and the mean change sign and P_{v,n} takes (-1)^n: one set of
+      // recurrences serves both, W(+) = sum (1/s+)(-1/s+)^n c_n P_n and
+      // W(-) = sum (1/s-)(1/s-)^n c_n P_n with s+- = e +- c (the same values as
+      // two calls of series_w, the last terms aside).
+      const zd sp = e + sc, sm = e - sc;
+      const double rp = sdm / absz(sp), rm = sdm / absz(sm);
+      if (rp < kSeriesR && rm < kSeriesR) {
+        const int nmax = max(series_terms(rp), series_terms(rm));
+        const zd tp = -1.0 / sp, tm = 1.0 / sm;
+        zd np = 1.0 / sp, nm = 1.0 / sm;
+        double h1 = 1.0, h2 = 1.0, h3 = 1.0, h = 1.0;
+        double p0 = 1.0, p1 = 1.0, p2 = 1.0, p3 = 1.0;
+        for (int n = 0; n <= nmax; ++n) {
+          const double cf = kMomentInv[n];
+          const double c0 = cf * p0, c1 = cf * p1, c2 = cf * p2, c3 = cf * p3;
+          T0 = T0 + (np - nm) * c0;
+          T1 = T1 + (np - nm) * c1;
+          T2 = T2 + (np - nm) * c2;
+          T3 = T3 + (np - nm) * c3;
+          np = np * tp;
+          nm = nm * tm;
+          h1 *= sd0;
+          h2 = h1 + sd1 * h2;
+          h3 = h2 + sd2 * h3;
+          h = h3 + sd3 * h;
+          p0 = h + sd0 * p0;
+          p1 = h + sd1 * p1;
+          p2 = h + sd2 * p2;
+          p3 = h + sd3 * p3;
+        }
+        lds_add(x0 + f, T0);
+        lds_add(x1 + f, T1);
+        lds_add(x2 + f, T2);
+        lds_add(x3 + f, T3);
+        continue;
+      }
 #pragma unroll
       for (int sg = 0; sg < 2; ++sg) {
         const double s = sg ? -1.0 : 1.0;
+        {
+          zd w0, w1, w2, w3;
+          if (series_w(A0 * s, A1 * s, A2 * s, A3 * s, e, w0, w1, w2, w3)) {
+            T0 = T0 + w0 * s;
+            T1 = T1 + w1 * s;
+            T2 = T2 + w2 * s;
+            T3 = T3 + w3 * s;
+            continue;
+          }
+        }
         const zd v0 = A0 * s + e, v1 = A1 * s + e, v2 = A2 * s + e, v3 = A3 * s + e;
         // log(v_i), v_i = s A_i + e, from the vertex table (the same operations)
         const double2 w0 = l0[2 * f + sg], w1 = l1[2 * f + sg], w2 = l2[2 * f + sg],
@@ -365,11 +489,50 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     double2* y40 = x + (size_t)sel4(o0, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
     double2* y41 = x + (size_t)sel4(o1, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
     const double ds = m.dsgn;
+    // the class's nodes (near-equal values replaced by their mean) and the
+    // corners they belong to, for the series; kAll's closed form is exact
+    zd n2 = G40, n3 = G41;
+    double2* z2 = y40;
+    double2* z3 = y41;
+    zd m1 = (G30 + G31) / 2.0;
+    if (kind == kTriple) {
+      m1 = (G30 + G31 + G32) / 3.0;
+      n2 = m1;
+      n3 = G40;
+      z2 = y32;
+      z3 = y40;
+    } else if (kind == kTwoPairs) {
+      n2 = n3 = (G32 + G33) / 2.0;
+      z2 = y32;
+      z3 = y33;
+    }
+    if (kind != kAll) {
+      // frequencies where the series converges; the closed forms take the rest
+      for (int f = 0; f < nfb; ++f) {
+        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
+        zd w0, w1, w2, w3;
+        if (series_w(m1, m1, n2, n3, e, w0, w1, w2, w3)) {
+          lds_add(y30 + f, w0 * ds);
+          lds_add(y31 + f, w1 * ds);
+          lds_add(z2 + f, w2 * ds);
+          lds_add(z3 + f, w3 * ds);
+        }
+      }
+    }
+    // true where the series was taken above (the same test, no arithmetic)
+    auto by_series = [&](zd e) {
+      const zd c = (m1 + m1 + n2 + n3) * 0.25;
+      const zd s = e + c;
+      const double dm = fmax(fmax(fabs(m1.x - c.x), fabs(m1.x - c.x)),
+                             fmax(fabs(n2.x - c.x), fabs(n3.x - c.x)));
+      return kind != kAll && dm < kSeriesR * absz(s);
+    };
     if (kind == kPair) {
       const zd q1 = (G30 + G31) / 2.0;
 #pragma unroll 1
       for (int f = 0; f < nfb; ++f) {
         const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
+        if (by_series(e)) continue;
         const zd c1 = f_sub2g10(q1, G40, G41, e);
         const zd c2 = f_sub2g30(q1, G40, G41, e);
         const zd c3 = f_sub1g30(q1, G41, G40, e);
@@ -383,6 +546,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll 1
       for (int f = 0; f < nfb; ++f) {
         const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
+        if (by_series(e)) continue;
         const zd c1 = f_sub2g30g10(q1, G40, e);
         const zd c2 = f_sub1g30g20(q1, G40, e);
         lds_add(y30 + f, c1 * ds);
@@ -395,6 +559,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 #pragma unroll 1
       for (int f = 0; f < nfb; ++f) {
         const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
+        if (by_series(e)) continue;
         const zd c1 = f_sub2g30g20(q1, q2, e);
         const zd c2 = f_sub1g30g21(q1, q2, e);
         lds_add(y30 + f, c1 * ds);
@@ -449,6 +614,35 @@ __device__ __forceinline__ void calc_terms(int code, zd g0, zd g1, zd g2, zd g3,
                                            zd& t1, zd& t2, zd& t3) {
   t0 = t1 = t2 = t3 = mkz(0.0, 0.0);
   const int kind = code & 15;
+  {
+    // the class's nodes by vertex (near-equal values replaced by their mean,
+    // as the closed forms below take them), then the series where it converges
+    const int b0 = vtx(code, 0, 0), b1 = vtx(code, 0, 1), b2 = vtx(code, 0, 2), b3 = vtx(code, 0, 3);
+    const zd G30 = sel4(b0, g0, g1, g2, g3), G31 = sel4(b1, g0, g1, g2, g3);
+    const zd G32 = sel4(b2, g0, g1, g2, g3), G33 = sel4(b3, g0, g1, g2, g3);
+    zd q1 = mkz(0.0, 0.0), q2 = q1;
+    unsigned in1 = 0, in2 = 0;  // vertex masks of the classes with means q1, q2
+    if (kind == kPair) {
+      q1 = (G30 + G31) / 2.0;
+      in1 = (1u << b0) | (1u << b1);
+    } else if (kind == kTriple) {
+      q1 = (G30 + G31 + G32) / 3.0;
+      in1 = (1u << b0) | (1u << b1) | (1u << b2);
+    } else if (kind == kTwoPairs) {
+      q1 = (G30 + G31) / 2.0;
+      q2 = (G32 + G33) / 2.0;
+      in1 = (1u << b0) | (1u << b1);
+      in2 = (1u << b2) | (1u << b3);
+    } else if (kind == kAll) {
+      q1 = (G30 + G31 + G32 + G33) / 4.0;
+      in1 = 15u;
+    }
+    const zd n0 = (in1 & 1u) ? q1 : (in2 & 1u) ? q2 : g0;
+    const zd n1 = (in1 & 2u) ? q1 : (in2 & 2u) ? q2 : g1;
+    const zd n2 = (in1 & 4u) ? q1 : (in2 & 4u) ? q2 : g2;
+    const zd n3 = (in1 & 8u) ? q1 : (in2 & 8u) ? q2 : g3;
+    if (series_w(n0, n1, n2, n3, e, t0, t1, t2, t3)) return;
+  }
   if (kind == kNormal) {  // case(0)
     const int i0 = vtx(code, 1, 0), i1 = vtx(code, 1, 1), i2 = vtx(code, 1, 2), i3 = vtx(code, 1, 3);
     const zd A0 = sel4(i0, g0, g1, g2, g3), A1 = sel4(i1, g0, g1, g2, g3);

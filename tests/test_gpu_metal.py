@@ -1,21 +1,18 @@
 """Metals on the GPU (partially occupied bands): tetrahedra cut by E_F
 (k_tetra_cut, ttct in ttrhdrn_chi0) and the q = 0 intraband Drude term
 (k_drude, chiqw.F:1631-1733), against the oracle on the same synthetic
-metals.  The weights of a pair whose bands cross E_F within FP64 closed-form
-rounding; at q != 0 chi and eps^-1 from the FP64 path within 1e-6 relative
-Frobenius, as for insulators; the split-BF16 path within 1e-5 of FP64.
+metals.  The weights of a pair whose bands cross E_F within 1e-9 of their
+scale at every frequency; chi and eps^-1 from the FP64 path within 1e-6
+relative Frobenius at q != 0 and q = 0 (per frequency at q = 0, where the
+Drude head enters), as for insulators; the split-BF16 path within 1e-5 of
+FP64.
 
-q = 0 of a metal: pairs of two partially occupied bands have small gaps, so
-their G = 0 entries -i<v|p|o>/(e_v - e_o) are large and dominate the chi head,
-and their weights come from small Fermi-surface pieces whose closed forms
-cancel at large omega.  There the reference itself carries few digits: on
-metal_tiny the compiled reference ttrhdrn_chi0 and the oracle disagree on
-single weights by up to 2.6e-3 above 5 eV
-(tests/test_oracle_vs_ref.py::test_fermi_surface_weights_on_a_metal), and on
-the sum over dsgn = +-1 of the pair of the two crossing bands by 1e-6 at
-9 eV, 2e-2 at 49 eV and 0.17 at 98 eV (the GPU is closer to the oracle).  The bars at q = 0 are therefore
-eps^-1 within 1e-6 per frequency below 20 eV, within 1e-3 over all
-frequencies, and chi within 1e-5; eps^-1 carries the Drude head.
+At large omega the closed forms of small Fermi-surface pieces cancel (the
+compiled reference carries only ~5e-5 of the scale there: its own rounding,
+measured against its formulas in __float128 by
+tests/test_oracle_vs_ref.py::test_fermi_surface_weights_on_a_metal).  The
+oracle and the GPU kernels both evaluate those integrals by their convergent
+series where the closed forms would cancel, so they agree at every frequency.
 """
 import numpy as np
 import pytest
@@ -58,8 +55,7 @@ def test_cut_weights_match_oracle(rough):
         err = np.abs(d[:, :ntk] - want)
         print(f"rough={rough}: weights max err {err.max() / scale:.2e} "
               f"(low band {err[low].max() / scale:.2e})", flush=True)
-        np.testing.assert_allclose(d[low, :ntk], want[low], rtol=0, atol=1e-8 * scale)
-        np.testing.assert_allclose(d[:, :ntk], want, rtol=0, atol=1e-6 * scale)
+        np.testing.assert_allclose(d[:, :ntk], want, rtol=0, atol=1e-9 * scale)
     finally:
         g.close()
 
@@ -86,11 +82,10 @@ def test_metal_chi_and_epsinv(name, q, rough):
             assert ec <= 1e-6 and ee <= 1e-6
         else:
             assert len(o["intra"]) > 0 and np.all(o["wd"] > 0)
-            low = o["em"].real < 20.0 / 27.21151
             per = [relf(r["epsinv"][:, ie], o["epsinv"][:, ie]) for ie in range(len(o["em"]))]
             print("  eps^-1 per frequency:", np.array2string(np.array(per), precision=1), flush=True)
-            assert max(np.array(per)[low]) <= 1e-6
-            assert ec <= 1e-5 and ee <= 1e-3
+            assert max(per) <= 1e-6
+            assert ec <= 1e-6 and ee <= 1e-6
         if name == "metal_c1":
             m = g.run_q(q, "bf16x3", want_chi=True)
             em, eme = relf(m["chi"], r["chi"]), relf(m["epsinv"], r["epsinv"])

@@ -81,7 +81,7 @@ def test_linear_band_tetrahedron_against_quadrature():
     em = np.array([0.0, 0.2, 0.35, 0.9], np.complex128)
     fk = np.ones(4, np.complex128)
     xo = np.zeros((4, len(em)), np.complex128)
-    oracle.lib().ro_ttrhdrn_calc(1e-3, 1e-3, ptr(fk), ptr(g), len(em), ptr(em), ptr(xo))
+    oracle.lib().ro_ttrhdrn_calc(1e-3, 1e-3, ptr(fk), ptr(g), len(em), ptr(em), ptr(xo), 1)
     # quadrature over the unit simplex (volume 1/6): weight of vertex v is its
     # barycentric coordinate; result scaled by 6 * (1/24 per-vertex normalisation)
     n = 2000000
@@ -136,3 +136,34 @@ def test_chi_of_plane_wave_states_closed_form():
     assert np.linalg.norm(off) > 0.3 * np.linalg.norm(chi)
     assert np.linalg.norm(np.conj(np.swapaxes(off, 1, 2)) - off) > 0.1 * np.linalg.norm(off)
     assert e_chi <= 1e-6 and e_eps <= 1e-6, (e_chi, e_eps)
+
+
+@pytest.mark.parametrize("cls", ["normal", "pair", "triple", "two_pairs", "all_equal"])
+def test_tetrahedron_series_against_quad_closed_forms(cls):
+    """The series evaluation of calc's vertex weights (oracle/tet_forms.h)
+    against the reference's closed forms of the same class evaluated in
+    __float128, on one tetrahedron per class at frequencies that put
+    r = max|g~ - c| / |e + c| between 1e-4 and 0.5.  There the double closed
+    forms lose up to ~1e16 ulps (a relative error of order one at r = 1e-4)
+    and the quad ones keep ~1e-18: every weight within 1e-14 of the largest."""
+    rng = np.random.default_rng({"normal": 1, "pair": 2, "triple": 3, "two_pairs": 4,
+                                 "all_equal": 5}[cls])
+    delt = 0.004
+    base = {"normal": [0.0, 0.031, 0.074, 0.12], "pair": [0.0, 2e-4, 0.05, 0.11],
+            "triple": [0.0, 3e-4, 5e-4, 0.09], "two_pairs": [0.0, 4e-4, 0.08, 0.0806],
+            "all_equal": [0.0, 1e-4, 3e-4, 6e-4]}[cls]
+    g = -(0.3 + np.array(base) + rng.uniform(0, 1e-5, 4)) + 1j * delt
+    fk = np.ones(4, np.complex128)
+    # frequencies e with max_j |g_j - c| / |e + c| = r (c the mean vertex value)
+    c = g.real.mean()
+    dm = np.abs(g.real - c).max() + 1e-300
+    r = np.geomspace(1e-4, 0.45, 24)
+    em = (-c + dm / r).astype(np.complex128)
+    x = np.zeros((4, len(em)), np.complex128)
+    xq = np.zeros_like(x)
+    assert oracle.lib().ro_ttrhdrn_calc(1e-3, 1e-3, ptr(fk), ptr(g), len(em), ptr(em), ptr(x), 1) == 0
+    assert oracle.lib().ro_ttrhdrn_calc_quad(1e-3, 1e-3, ptr(fk), ptr(g), len(em), ptr(em),
+                                             ptr(xq)) == 0
+    err = np.abs(x - xq).max(axis=0) / np.abs(xq).max(axis=0)
+    print(f"{cls}: series vs quad closed forms, worst {err.max():.1e} (relative per frequency)")
+    assert err.max() <= 1e-14
