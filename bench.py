@@ -228,6 +228,28 @@ def main() -> int:
         # three bf16 products per real product: executed MFMA work against the same peak
         rf["executed_frac"] = 3.0 * achieved / PEAK[main_prec]
     out["roofline"] = rf
+    # the other stages of the q (device times from events around each stage):
+    # pair densities against HBM (per (pair, k): the two complex64 band grids
+    # of N_r points read, M(G) written), the FP64 Dyson step against the FP64
+    # peak (8 NG^3 per frequency), tetrahedron evaluations per second
+    nr = int(np.prod(info["nw"]))
+    pd_bytes = 8.0 * (2 * nr + ng) * K
+    dy_flops = 8.0 * float(ng) ** 3 * nen
+    n_eval = npair * 6 * ntk * 2 * nen
+    out["rooflines"] = {
+        "k_pair_densities_pruned": {"bound": "hbm", "achieved": pd_bytes / st["pairdens"] / 1e9,
+                                    "peak": 8000.0, "unit": "GB/s",
+                                    "frac": pd_bytes / st["pairdens"] / 8.0e12,
+                                    "algorithmic_bytes_per_launch": pd_bytes,
+                                    "avg_launch_ms": st["pairdens"] * 1e3,
+                                    "traffic": traffic_per_launch(args.workload,
+                                                                  "k_pair_densities_pruned")},
+        "dyson": {"bound": "fp64", "achieved": dy_flops / st["dyson"] / 1e12, "peak": 78.6,
+                  "unit": "TFLOP/s", "frac": dy_flops / st["dyson"] / 78.6e12,
+                  "algorithmic_flops": dy_flops, "stage_ms": st["dyson"] * 1e3},
+        "tetrahedron": {"evaluations_per_s": n_eval / st["weights"], "stage_ms": st["weights"] * 1e3,
+                        "unit": "(pair, tetrahedron, dsgn, w) per s"},
+    }
     if "fp64" in results:
         s64 = results["fp64"]["stages"]
         a64 = flops / (s64["gemm_kernel"] / max(1, s64["gemm_launches"]))
@@ -289,7 +311,7 @@ def c3_bench(args) -> int:
                 c.set_window(cases[0][1], cases[0][2])
                 c.run_q(cases[0][0], prec, want_eps=False)
             walls, dev, ngs, flops = [], [], [], 0.0
-            keep = []
+            stage_sum = {}
             for step in range(args.steps):
                 t0 = time.perf_counter()
                 devs = 0.0
@@ -297,6 +319,8 @@ def c3_bench(args) -> int:
                     c.set_window(ecut, nvir)
                     r = c.run_q(q, prec, want_eps=False)
                     devs += r["timings"]["device_total"]
+                    for k_ in ("pairdens", "weights", "contraction", "dyson"):
+                        stage_sum[k_] = stage_sum.get(k_, 0.0) + r["timings"][k_] / args.steps
                     if step == 0:
                         ng = int(len(r["lg"]))
                         ngs.append(ng)
@@ -305,7 +329,7 @@ def c3_bench(args) -> int:
                 dev.append(devs)
                 log(f"{prec} step {step}: {walls[-1]:.3f} s wall, {devs:.3f} s device for {len(cases)} q")
             res[prec] = dict(wall=float(np.mean(walls)), device=float(np.mean(dev)), ngs=ngs,
-                             flops=flops)
+                             flops=flops, stages=stage_sum)
         # errors of the mixed path against FP64, q by q (one pair of results held at a time)
         rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
         m = args.mixed
@@ -319,6 +343,14 @@ def c3_bench(args) -> int:
     finally:
         c.close()
     nq = len(cases)
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on a sample of the pairs of q 1 (its cutoff and window)
+        q1, ec1, nv1 = cases[1]
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(w, dict(wl, q=q1, ecut_eps=ec1), info["n_occ"] * nv1, args.cpu_pairs,
+                           threads)
+        cpu["sample"] += f"; q {tuple(round(x, 4) for x in q1)}, {nv1} empty bands"
     out = {"metric": "chi(q,w) s per q-point over a batch of uneven q (3^3 mesh)",
            "value": graph[m]["wall"] / nq, "unit": "s/q-point", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "higher_is_better": False, "dtype": m,
@@ -335,6 +367,9 @@ def c3_bench(args) -> int:
                         for p, ts in (("fp64", [t for t in tune if t["precision"] == 0]),
                                       (m, [t for t in tune if t["precision"] != 0]))},
            "chi_rel_err_vs_fp64_max": err_chi, "epsinv_rel_err_vs_fp64_max": err_eps,
+           "eager_stage_s_per_q": {p: {k_: v_ / nq for k_, v_ in res[p]["stages"].items()}
+                                   for p in res},
+           "cpu_baseline": cpu,
            "contraction_tflop_total": res[m]["flops"] / 1e12,
            "data": "synthetic (seeded Gaussian plane-wave coefficients, Gram-Schmidt per k)",
            "config": {"workload": "c3_batch: a=7.0 bohr sc, 3^3 k (NTK 27), 8 occ + 200 empty, "
@@ -368,15 +403,23 @@ def closure_bench(args) -> int:
 
 def calc_int_bench(args) -> int:
     """calc_w3d and calc_j3d at the calc_int size (--workload c4_calc_int; not
-    the default line): 6^3 mesh (NTK = NTQ = 216), 5 d-like Wannier functions,
-    chi cutoff 22.2 Ry (NG ~ 600), 32 frequencies, eps^-1 of every irreducible q
-    resident in HBM.  eps^-1 is synthetic (identity plus a seeded rank-one
-    term per frequency): the kernels do the same work for any values.  A step
-    is one rp_w3d_funcs + rp_w3d_vw (V and W over all q), and one rp_j3d_run for
-    R = 0 (X and J of one lattice vector)."""
+    the default line): 6^3 mesh (NTK = NTQ = 216, 112 irreducible q), 5 d-like
+    Wannier functions, chi cutoff 22.2 Ry (NG ~ 600), 32 frequencies.  eps^-1
+    of every irreducible q comes from this build's chi on the same system
+    (split-BF16 contraction, FP64 Dyson; at Gamma the mean of the three
+    directions, as calc_int's reader forms it), resident in HBM as complex64.
+    A step is one rp_w3d_funcs + rp_w3d_vw (V and W over all q), and one
+    rp_j3d_run for R = 0 (X and J of one lattice vector).
+
+    Also reported: device time per stage (events on the library's stream), the
+    HBM rooflines of the W and J contractions (bytes per (q, w) = 8 NG_eps^2
+    + 16 NG_psi NV, NV the vectors: NWF for W, NWF^2 pair vectors for J), V
+    and W against the oracle over every q in meV (calc_intW.F:942-985),
+    X_ii(0) - V_ii(0) and J_ii(0, w) - W_ii(0, w) in meV, and the oracle's
+    calc_w3d timed on the host as the CPU baseline."""
     from respackdev_amd import gen
-    from respackdev_amd.chiqw import from_system
-    from respackdev_amd.w3d import W3d
+    from respackdev_amd.chiqw import Chiqw, from_system
+    from respackdev_amd.w3d import AU, W3d
 
     wl = gen.WORKLOADS["c4_calc_int"]
     t0 = time.perf_counter()
@@ -387,43 +430,93 @@ def calc_int_bench(args) -> int:
     W = np.zeros((len(wan), wl["nwf"], ntg), np.complex128)
     for k, c in enumerate(wan):
         W[k, :, :c.shape[1]] = c
-    from respackdev_amd import _lib
-    em = np.zeros(wl["nen"], np.complex128)  # the reference's gen_grid rule, 0..50 eV
-    _lib.check(_lib.lib().rp_gen_grid(wl["nen"], 50.0 / 27.21151, _lib.ptr(em)))
     sqi = np.array([[gen._fold(x) for x in k] for k in w.ski])
+    # eps^-1 of every irreducible q from chi (untimed)
+    c = Chiqw(0)
+    try:
+        cinfo = c.open(w, ecut_for_eps=wl["ecut_eps"], num_freq_grid=wl["nen"])
+        em = np.asarray(cinfo["em"], np.complex128)
+        eps = []
+        for q in sqi:
+            e = c.run_q(q, "bf16x3")["epsinv"]
+            eps.append((e.mean(axis=0) if e.shape[0] > 1 else e[0]).astype(np.complex64))
+    finally:
+        c.close()
+    log(f"eps^-1 of {len(sqi)} irreducible q from chi in {time.perf_counter() - t0:.1f} s")
     g = W3d(0)
     try:
         info = g.open(w, wl["ecut_eps"], em)
         g.set_wannier(W)
         ng = g.set_q(sqi)
-        rng = np.random.default_rng(wl["seed"])
-        for iq, n in enumerate(ng):
-            u = rng.standard_normal((len(em), n)) + 1j * rng.standard_normal((len(em), n))
-            e = 0.01 / n * u[:, :, None] * np.conj(u[:, None, :])
-            e[:, np.arange(n), np.arange(n)] += 1.0
+        assert [e.shape[-1] for e in eps] == list(ng), "chi sphere differs from calc_int's LG0"
+        for iq, e in enumerate(eps):
             g.set_epsinv(iq, e)
         info = g.info()
-        log(f"calc_int set up in {time.perf_counter() - t0:.1f} s: NTK {info['ntk']}, NTQ "
-            f"{info['ntq']}, {len(ng)} irreducible q, NG {ng.min()}-{ng.max()}, NWF {info['nwf']}")
+        log(f"calc_int set up: NTK {info['ntk']}, NTQ {info['ntq']}, {len(ng)} irreducible q, "
+            f"NG {ng.min()}-{ng.max()}, NWF {info['nwf']}")
         for i in range(args.warmup):
             g.funcs(); g.vw(); g.j3d([(0, 0, 0)])
-        tf, tv, tj = [], [], []
+        tf, tv, tj, st = [], [], [], []
         for i in range(args.steps):
             t0 = time.perf_counter(); g.funcs(); tf.append(time.perf_counter() - t0)
-            t0 = time.perf_counter(); g.vw(); tv.append(time.perf_counter() - t0)
-            t0 = time.perf_counter(); g.j3d([(0, 0, 0)]); tj.append(time.perf_counter() - t0)
-            log(f"step {i}: funcs {tf[-1]:.3f} s, vw {tv[-1]:.3f} s, j3d(R=0) {tj[-1]:.3f} s")
+            sw = g.stage_times()
+            t0 = time.perf_counter(); v, wm, _, _ = g.vw(); tv.append(time.perf_counter() - t0)
+            t0 = time.perf_counter(); x, j, _, _ = g.j3d([(0, 0, 0)]); tj.append(time.perf_counter() - t0)
+            sj = g.stage_times()
+            st.append({**{k: sw[k] for k in sw if k.startswith("w_")},
+                       **{k: sj[k] for k in sj if k.startswith("j_")}})
+            log(f"step {i}: funcs {tf[-1]:.4f} s, vw {tv[-1]:.4f} s, j3d(R=0) {tj[-1]:.4f} s; "
+                f"stages {st[-1]}")
     finally:
         g.close()
+    stage = {k: float(np.mean([d[k] for d in st])) for k in st[0]}
+    nwf, nen, ntq = info["nwf"], info["nen"], info["ntq"]
     out = {"metric": "calc_w3d s per run (V and W over all q) and calc_j3d s per lattice vector",
            "value": float(np.mean(tf) + np.mean(tv)), "unit": "s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "higher_is_better": False,
            "w3d_funcs_s": float(np.mean(tf)), "w3d_vw_s": float(np.mean(tv)),
-           "j3d_per_r_s": float(np.mean(tj)), "dtype": "fp64 (eps^-1 complex64 as the reference)",
-           "data": "synthetic Wannier functions and eps^-1",
-           "config": {"workload": "c4_calc_int", "ntk": info["ntk"], "ntq": info["ntq"],
-                      "nwf": info["nwf"], "nen": info["nen"], "nq_irr": int(len(ng)),
-                      "ng_eps": [int(ng.min()), int(ng.max())]}}
+           "j3d_per_r_s": float(np.mean(tj)), "stage_s": stage,
+           "dtype": "fp64 (eps^-1 complex64 as the reference)",
+           "data": "synthetic Wannier functions; eps^-1 from this build's chi of the same system",
+           "config": {"workload": "c4_calc_int: a=7.0 bohr sc, 6^3 k (NTK = NTQ 216), 5 Wannier "
+                                  "functions, NG_eps ~600, nen 32",
+                      "ntk": info["ntk"], "ntq": ntq, "nwf": nwf, "nen": nen,
+                      "nq_irr": int(len(ng)), "ng_eps": [int(ng.min()), int(ng.max())]}}
+    # HBM rooflines of the two contractions: eps^-1 of each full-mesh q read
+    # once per frequency (NG_eps of a full-mesh q is its irreducible one's),
+    # plus the vectors
+    from oracle import pipeline_w3d
+    cset = pipeline_w3d.setup_w3d(w, W, sqi, lambda iq: eps[iq], em, wl["ecut_eps"])
+    nge = np.array([cset["nge"][iq] for iq in range(ntq)], np.float64)
+    ngp = np.array([cset["ngp"][iq] for iq in range(ntq)], np.float64)
+    for name, nv, key in (("k_w_contract", nwf, "w_contract"), ("k_j_contract", nwf * nwf, "j_contract")):
+        byt = float(np.sum(nen * (8.0 * nge ** 2 + 16.0 * ngp * nv)))
+        out.setdefault("rooflines", {})[name] = {
+            "bound": "hbm", "achieved": byt / stage[key] / 1e9, "peak": 8000.0, "unit": "GB/s",
+            "frac": byt / stage[key] / 8.0e12, "algorithmic_bytes_per_launch": byt,
+            "avg_launch_ms": stage[key] * 1e3,
+            "traffic": traffic_per_launch("c4_calc_int", name)}
+    # parity: V and W over every q against the oracle (its calc_w3d on the
+    # host is also the CPU baseline), and the exchange identities at R = 0
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        log(f"oracle calc_w3d over {ntq} q on the host ({threads} threads)")
+        t0 = time.perf_counter()
+        o = pipeline_w3d.run_w3d(w, W, sqi, lambda iq: eps[iq], em, wl["ecut_eps"])
+        tcpu = time.perf_counter() - t0
+        out["v_err_meV"] = float(np.abs(v - o["vmat"]).max() * AU * 1e3)
+        out["w_err_meV"] = float(np.abs(wm - o["wmat"]).max() * AU * 1e3)
+        out["v_rel_err"] = float(np.linalg.norm(v - o["vmat"]) / np.linalg.norm(o["vmat"]))
+        out["w_rel_err"] = float(np.linalg.norm(wm - o["wmat"]) / np.linalg.norm(o["wmat"]))
+        out["cpu_baseline"] = {"value": tcpu, "unit": "s per calc_w3d run", "cores": threads,
+                               "kind": "port", "sample": f"the oracle's calc_w3d over all {ntq} q "
+                               "(pair densities in threads over k, funcw in threads over w)"}
+    na = (v.shape[0] - 1) // 2
+    out["x_minus_v_diag_meV"] = float(np.abs(np.diagonal(x[0]) -
+                                             np.diagonal(v[na, na, na])).max() * AU * 1e3)
+    out["j_minus_w_diag_meV"] = float(np.abs(np.diagonal(j[:, 0], axis1=-2, axis2=-1) -
+                                             np.diagonal(wm[:, na, na, na], axis1=-2,
+                                                         axis2=-1)).max() * AU * 1e3)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -39,7 +39,7 @@ module respack_gpu
   public :: rp_mlp_open, rp_mlp_nparams, rp_mlp_get_params, rp_mlp_set_params, rp_mlp_forward
   public :: rp_mlp_gradient, rp_mlp_train, rp_chi_set
   public :: rp_w3d_open, rp_w3d_info, rp_w3d_kmesh, rp_w3d_set_wannier, rp_w3d_set_q
-  public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run
+  public :: rp_w3d_set_epsinv, rp_w3d_funcs, rp_w3d_vw, rp_j3d_run, rp_w3d_stage_times
 
   interface
     integer(c_int) function rp_init(device, h) bind(C, name="rp_init")
@@ -292,6 +292,12 @@ module respack_gpu
       integer(c_int32_t) :: rvec(3, *)
       complex(c_double_complex) :: xmat(*), jmat(*)
       real(c_double) :: heads(*)
+    end function
+    ! device seconds of the last rp_w3d_funcs and rp_j3d_run (see respack_gpu.h)
+    integer(c_int) function rp_w3d_stage_times(h, t) bind(C, name="rp_w3d_stage_times")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: h
+      real(c_double) :: t(8)
     end function
     integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
       import :: c_size_t, c_ptr

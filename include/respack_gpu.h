@@ -240,6 +240,12 @@ int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads);
  * rp_w3d_vw (added on i = j at R = 0 only). */
 int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, double* jmat,
                double* heads);
+/* Device seconds (events on the handle's stream) of the last rp_w3d_funcs:
+ * t[0] rho, t[1] func, t[2] the W contraction, t[3] total; and of the last
+ * rp_j3d_run: t[4] rho of every R, t[5] func and sums, t[6] the J
+ * contraction (all R), t[7] total (8 doubles).  No counterpart in the
+ * reference (instrumentation). */
+int rp_w3d_stage_times(rp_handle h, double* t);
 
 /* ------------------------------------------------------------------------
  * Band-tail closure network.  No counterpart in the reference: a small MLP

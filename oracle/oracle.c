@@ -827,6 +827,9 @@ void ro_w3d_func(int nwf, int ng_psi, int ldr, const ro_z *rho, ro_z *func) {
 void ro_w3d_funcw(int nwf, int ng_eps, int ng_psi, int ldr, const ro_z *rho, int ntgq,
                   int ne, const float _Complex *epsmk, ro_z *funcw) { /* calc_intW.F:848-912 */
   const size_t plane = (size_t)ntgq * ntgq;
+  /* threads over the frequencies, as the reference's OMP DO: every element is
+   * summed by one thread in the reference's order */
+#pragma omp parallel for schedule(dynamic)
   for (int ie = 0; ie < ne; ++ie)
     for (int iw = 0; iw < nwf; ++iw)
       for (int jw = 0; jw < nwf; ++jw) {

@@ -28,7 +28,7 @@ EXPORTS = [
     "rp_mlp_open", "rp_mlp_nparams", "rp_mlp_get_params", "rp_mlp_set_params", "rp_mlp_forward",
     "rp_mlp_gradient", "rp_mlp_train", "rp_chi_set",
     "rp_w3d_open", "rp_w3d_info", "rp_w3d_kmesh", "rp_w3d_set_wannier", "rp_w3d_set_q",
-    "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run",
+    "rp_w3d_set_epsinv", "rp_w3d_funcs", "rp_w3d_vw", "rp_j3d_run", "rp_w3d_stage_times",
     "rp_chiqw_batch_plan", "rp_chiqw_batch_run", "rp_chiqw_batch_result", "rp_chiqw_batch_free",
     "rp_chiqw_autotune", "rp_chiqw_tune_count", "rp_chiqw_tune_info", "rp_chiqw_set_span",
 ]
@@ -101,6 +101,7 @@ def lib():
         L.rp_w3d_funcs.argtypes = [P, P, P]
         L.rp_w3d_vw.argtypes = [P, P, P, P]
         L.rp_j3d_run.argtypes = [P, I32, P, P, P, P]
+        L.rp_w3d_stage_times.argtypes = [P, P]
         for name in EXPORTS:
             getattr(L, name).restype = getattr(L, name).restype or I32
         _lib = L

@@ -698,6 +698,13 @@ static std::vector<ChiLaunch> chi_candidates(const rp_ctx* h, int prec) {
   return out;
 }
 
+// seconds between two recorded (and completed) events
+static double event_seconds(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  RP_HIP(hipEventElapsedTime(&ms, a, b));
+  return 1e-3 * ms;
+}
+
 static double time_launch(rp_ctx* h, int prec, const ChiLaunch& c) {
   issue_accumulate(h, prec, c, false);  // warm-up (and buffer growth)
   double best = 1e300;
@@ -870,25 +877,30 @@ int rp_chiqw_q(rp_handle h, const double* q, int32_t precision, double* epsinv_o
     if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
     RP_HIP(hipEventRecord(h->ev[3], s));
     if (rp_dyson(h, epsinv_out)) throw Error(g_err);
+    RP_HIP(hipEventRecord(h->ev[4], s));
     // an ill-conditioned eps amplifies the FP32 accumulation error of the
     // split-BF16 contraction: run it again with the span its eps^-1 calls for
-    if (const int sp = adapt_span(h, precision, h->q_span)) {
-      h->q_span = sp;
-      RP_HIP(hipEventRecord(h->ev[2], s));
-      if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
-      RP_HIP(hipEventRecord(h->ev[3], s));
-      if (rp_dyson(h, epsinv_out)) throw Error(g_err);
-    }
-    RP_HIP(hipEventRecord(h->ev[4], s));
+    // (its contraction and Dyson times are added to those stages)
+    const int sp = adapt_span(h, precision, h->q_span);
     RP_HIP(hipEventSynchronize(h->ev[4]));
+    double st[5];
+    for (int i = 0; i < 4; ++i) st[i] = event_seconds(h->ev[i], h->ev[i + 1]);
+    st[4] = event_seconds(h->ev[0], h->ev[4]);
+    if (sp) {
+      h->q_span = sp;
+      RP_HIP(hipEventRecord(h->ev[1], s));
+      if (rp_chi_accumulate(h, precision, chi_out)) throw Error(g_err);
+      RP_HIP(hipEventRecord(h->ev[2], s));
+      if (rp_dyson(h, epsinv_out)) throw Error(g_err);
+      RP_HIP(hipEventRecord(h->ev[3], s));
+      RP_HIP(hipEventSynchronize(h->ev[3]));
+      const double ta = event_seconds(h->ev[1], h->ev[2]), td = event_seconds(h->ev[2], h->ev[3]);
+      st[2] += ta;
+      st[3] += td;
+      st[4] += ta + td;
+    }
     if (timings) {
-      float ms;
-      for (int i = 0; i < 4; ++i) {
-        RP_HIP(hipEventElapsedTime(&ms, h->ev[i], h->ev[i + 1]));
-        timings[i] = ms * 1e-3;
-      }
-      RP_HIP(hipEventElapsedTime(&ms, h->ev[0], h->ev[4]));
-      timings[4] = ms * 1e-3;
+      for (int i = 0; i < 5; ++i) timings[i] = st[i];
       timings[5] = h->prep_seconds;
       timings[6] = h->gemm_seconds;
       timings[7] = h->gemm_launches;

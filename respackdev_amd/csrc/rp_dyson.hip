@@ -747,32 +747,28 @@ __global__ void __launch_bounds__(256) k_gj_colswap(double2* __restrict__ a, int
 }
 
 // kappa[b] = max_j sum_i |a_b(i, j)| (the 1-norm of each matrix of the batch).
-// grid: nbatch blocks of 256.
+// grid: (ceil(n / 4), nbatch) blocks of 256: one wave per column, its lanes
+// reading consecutive rows; the column sums meet in kappa[b] through a 64-bit
+// atomic maximum on the bit patterns (non-negative doubles order as their
+// bits).  kappa must start at zero.
 __global__ void __launch_bounds__(256) k_norm1(const double2* __restrict__ a, int n,
-                                               double* __restrict__ kappa) {
-  const int b = blockIdx.x;
-  const double2* m = a + (size_t)b * n * n;
-  double best = 0.0;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double2 z = m[i + (size_t)n * j];
-      s += sqrt(z.x * z.x + z.y * z.y);
-    }
-    best = fmax(best, s);
+                                               unsigned long long* __restrict__ kappa) {
+  const int b = blockIdx.y, j = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (j >= n) return;  // uniform per wave
+  const double2* col = a + ((size_t)b * n + j) * n;
+  double s = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double2 z = col[i];
+    s += sqrt(z.x * z.x + z.y * z.y);
   }
-  __shared__ double red[256];
-  red[threadIdx.x] = best;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) kappa[b] = red[0];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (lane == 0) atomicMax(kappa + b, __double_as_longlong(s));
 }
 
 void launch_norm1(const double2* a, int n, int nbatch, double* kappa, hipStream_t s) {
-  hipLaunchKernelGGL(k_norm1, dim3(nbatch), dim3(256), 0, s, a, n, kappa);
+  launch_zero(kappa, sizeof(double) * (size_t)nbatch, s);
+  hipLaunchKernelGGL(k_norm1, dim3((n + 3) / 4, nbatch), dim3(256), 0, s, a, n,
+                     (unsigned long long*)kappa);
   RP_HIP(hipGetLastError());
 }
 

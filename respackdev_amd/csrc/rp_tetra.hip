@@ -333,7 +333,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int nnormal = count[0], nother = count[1];
   // ---- pass 2: non-degenerate tetrahedra (calc, case(0); ttrhdrn_sub2normal x3 and
   // ttrhdrn_sub1normal, m_tetrahedron_20170325.F:215-229,431-451)
-  for (int j = threadIdx.x; j < nnormal; j += blockDim.x) {
+  // Work units (item, chunk of frequencies): with few tetrahedra (small k
+  // meshes) each item's frequencies are split so that about twice the
+  // block's threads have work; the per-item factors are formed per unit.
+  const int fsplit = max(1, min(nfb, (2 * (int)blockDim.x + max(nnormal, 1) - 1) / max(nnormal, 1)));
+  const int fchunk = (nfb + fsplit - 1) / fsplit;
+  for (int jf = threadIdx.x; jf < nnormal * fsplit; jf += blockDim.x) {
+    const int j = jf / fsplit, fa = (jf % fsplit) * fchunk, fb = min(nfb, fa + fchunk);
     const int2 ent = list[j];
     const Item m = load_item(ent.x, ntet, imt, e1, e2, delt);
     const int code = ent.y;
@@ -377,7 +383,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     // K31, J1..J3) change sign, those of four do not (exactly, in IEEE
     // arithmetic).  Their corner terms are summed before the LDS atomics.
 #pragma unroll 1
-    for (int f = 0; f < nfb; ++f) {
+    for (int f = fa; f < fb; ++f) {
       const double2 ev = em[f0 + f];
       const zd e = mkz(ev.x, ev.y);
       zd T0 = mkz(0.0, 0.0), T1 = mkz(0.0, 0.0), T2 = mkz(0.0, 0.0), T3 = mkz(0.0, 0.0);

@@ -65,7 +65,29 @@ struct W3d {
   // summed pair-density grids P[q][p][r] of a chunk of q (launch_pair_rsum)
   DevBuf<double2> d_P;
   DevBuf<double2> d_rho, d_pf1, d_pf2, d_func, d_shell, d_funcw, d_cheadw, d_vmat, d_wmat;
+  // device seconds of the last rp_w3d_funcs (rho, func, W contraction, total)
+  // and rp_j3d_run (rho of every R, func and sums, J contraction, total),
+  // from events on the handle's stream; ev grows to 2 per J launch
+  double stage_s[8] = {};
+  std::vector<hipEvent_t> ev;
+  void events(size_t n) {
+    while (ev.size() < n) {
+      hipEvent_t e;
+      RP_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+  }
+  ~W3d() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
 };
+
+// seconds between events a and b (recorded, the stream synchronised)
+double event_s(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.0f;
+  RP_HIP(hipEventElapsedTime(&ms, a, b));
+  return 1e-3 * ms;
+}
 
 }  // namespace rp
 
@@ -574,13 +596,18 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     w.d_rho.reserve((size_t)(w.rho_off.back() + (long long)w.ngpsi.back() * w3d_max_wannier()));
     // rho for every q (calc_intW.F:647-767): the diagonal Wannier pairs
     ensure_geometry(w, s);
+    w.events(4);
+    RP_HIP(hipEventRecord(w.ev[0], s));
     rho_all_q(w, false, nullptr, w.d_rho_off.p, w.d_rho.p, s);
+    RP_HIP(hipEventRecord(w.ev[1], s));
     w.d_func.reserve((size_t)w.ntq * nwf * nwf);
     w.d_shell.reserve((size_t)w.ntq * nwf * nwf);
     w.d_funcw.reserve((size_t)w.ntq * nen * nwf * nwf);
     launch_w_func(w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.ntq, nwf, w.d_func.p, w.d_shell.p, s);
+    RP_HIP(hipEventRecord(w.ev[2], s));
     launch_w_contract(w.d_eps.p, w.d_eps_off.p, w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.d_minv.p,
                       w.d_pf1.p, w.d_pf2.p, w.d_map_off.p, w.d_shell.p, w.ntq, nwf, nen, w.d_funcw.p, s);
+    RP_HIP(hipEventRecord(w.ev[3], s));
     if (func_out)
       RP_HIP(hipMemcpyAsync(func_out, w.d_func.p, sizeof(double2) * w.ntq * nwf * nwf,
                             hipMemcpyDeviceToHost, s));
@@ -588,6 +615,10 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
       RP_HIP(hipMemcpyAsync(funcw_out, w.d_funcw.p, sizeof(double2) * w.ntq * nen * nwf * nwf,
                             hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
+    w.stage_s[0] = event_s(w.ev[0], w.ev[1]);
+    w.stage_s[1] = event_s(w.ev[1], w.ev[2]);
+    w.stage_s[2] = event_s(w.ev[2], w.ev[3]);
+    w.stage_s[3] = event_s(w.ev[0], w.ev[3]);
     w.have_funcs = true;
   });
 }
@@ -625,6 +656,14 @@ int rp_w3d_vw(rp_handle h, double* vmat, double* wmat, double* heads) {
   });
 }
 
+int rp_w3d_stage_times(rp_handle h, double* t) {
+  return guarded([&] {
+    W3d& w = w3d_of(h, "rp_w3d_stage_times");
+    RP_CHECK(t, "rp_w3d_stage_times: null output");
+    for (int i = 0; i < 8; ++i) t[i] = w.stage_s[i];
+  });
+}
+
 int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, double* jmat,
                double* heads) {
   return guarded([&] {
@@ -658,8 +697,11 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     DevBuf<double2> d_jrho, d_jfunc, d_jshell, d_jfuncw, d_x, d_j, d_cw, d_xg;
     d_jrho.reserve((size_t)tot * nr);
     ensure_geometry(w, s);
+    w.events(2 + 2 * (size_t)nr);
+    RP_HIP(hipEventRecord(w.ev[0], s));
     for (int r = 0; r < nr; ++r)
       rho_all_q(w, true, d_pfk.p + (size_t)r * ntk, d_joff.p, d_jrho.p + (size_t)r * tot, s);
+    RP_HIP(hipEventRecord(w.ev[1], s));
     d_jfunc.reserve((size_t)w.ntq * npair);
     d_jshell.reserve((size_t)w.ntq * npair);
     d_jfuncw.reserve((size_t)w.ntq * nen * npair);
@@ -670,9 +712,11 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     for (int r = 0; r < nr; ++r) {
       const double2* rho = d_jrho.p + (size_t)r * tot;
       launch_j_func(rho, d_joff.p, w.d_qinfo.p, w.ntq, npair, d_jfunc.p, d_jshell.p, s);
+      RP_HIP(hipEventRecord(w.ev[2 + 2 * r], s));
       launch_j_contract(w.d_eps.p, w.d_eps_off.p, rho, d_joff.p, w.d_qinfo.p, w.d_minv.p, w.d_pf1.p,
                         w.d_pf2.p, w.d_map_off.p, d_xg.p, d_jshell.p, w.ntq, npair, nen, d_jfuncw.p,
                         s);
+      RP_HIP(hipEventRecord(w.ev[3 + 2 * r], s));
       const int r0 = rvec[3 * r] == 0 && rvec[3 * r + 1] == 0 && rvec[3 * r + 2] == 0;
       launch_j_sum(d_jfunc.p, d_jfuncw.p, w.ntq, nwf, nen, r0, w.volume, w.chead, d_cw.p,
                    d_x.p + (size_t)r * npair, d_j.p + (size_t)r * npair, (size_t)nr * npair, s);
@@ -682,6 +726,12 @@ int rp_j3d_run(rp_handle h, int32_t nr, const int32_t* rvec, double* xmat, doubl
     if (jmat)
       RP_HIP(hipMemcpyAsync(jmat, d_j.p, sizeof(double2) * nr * nen * npair, hipMemcpyDeviceToHost, s));
     RP_HIP(hipStreamSynchronize(s));
+    double tj = 0.0;
+    for (int r = 0; r < nr; ++r) tj += event_s(w.ev[2 + 2 * r], w.ev[3 + 2 * r]);
+    w.stage_s[4] = event_s(w.ev[0], w.ev[1]);
+    w.stage_s[6] = tj;
+    w.stage_s[7] = event_s(w.ev[0], w.ev[3 + 2 * (nr - 1)]);
+    w.stage_s[5] = w.stage_s[7] - w.stage_s[4] - tj;
     if (heads) {
       heads[0] = w.chead;
       heads[1] = 0.0;

@@ -131,6 +131,14 @@ class W3d:
         check(_lib.lib().rp_w3d_vw(self._h, ptr(v), ptr(wm), ptr(heads)))
         return v, wm, heads[0], heads[2::2] + 1j * heads[3::2]
 
+    def stage_times(self) -> dict:
+        """Device seconds of the last funcs() (rho, func, W contraction, total)
+        and j3d() (rho of every R, func and sums, J contraction, total)."""
+        t = np.zeros(8)
+        check(_lib.lib().rp_w3d_stage_times(self._h, ptr(t)))
+        return {"w_rho": t[0], "w_func": t[1], "w_contract": t[2], "w_total": t[3],
+                "j_rho": t[4], "j_func_sum": t[5], "j_contract": t[6], "j_total": t[7]}
+
     def j3d(self, rlist):
         """calc_j3d exchange X_ij(R), J_ij(R, w) for the lattice vectors rlist.
         Returns X [r][j][i], J [ie][r][j][i], chead, cheadw."""
