@@ -62,6 +62,15 @@ __global__ void k_eps_build(const double2* __restrict__ chi, const double* __res
 
 __device__ inline double cabs1(double2 z) { return fabs(z.x) + fabs(z.y); }
 
+// acc + a b and acc - a b for complex FP64 as four fused multiply-adds each
+// (a product's two terms, each folded into the accumulator).
+__device__ __forceinline__ double2 cfma(double2 acc, double2 a, double2 b) {
+  return make_double2(fma(a.x, b.x, fma(-a.y, b.y, acc.x)), fma(a.x, b.y, fma(a.y, b.x, acc.y)));
+}
+__device__ __forceinline__ double2 cfms(double2 acc, double2 a, double2 b) {
+  return make_double2(fma(-a.x, b.x, fma(a.y, b.y, acc.x)), fma(-a.x, b.y, fma(-a.y, b.x, acc.y)));
+}
+
 // Largest panel width; panel rows moved by the pivoting of one panel: at most
 // the nb panel rows plus nb others.
 constexpr int kPanelMax = 16;
@@ -151,12 +160,11 @@ __global__ void __launch_bounds__(1024) k_gj_panel(double2* __restrict__ a, int 
       if (i == c) {
         for (int q = 0; q < nb; ++q) P[(size_t)q * n + i] = prow[q];
       } else {
-        const zd t = mkz(col[i].x, col[i].y);
+        const double2 t = col[i];
         for (int q = 0; q < nb; ++q) {
           double2* e = P + (size_t)q * n + i;
-          const zd v = q == l ? mkz(0.0, 0.0) : mkz(e->x, e->y);
-          const zd u = v - t * mkz(prow[q].x, prow[q].y);
-          *e = make_double2(u.x, u.y);
+          const double2 v = q == l ? make_double2(0.0, 0.0) : *e;
+          *e = cfms(v, t, prow[q]);
         }
       }
     }
@@ -183,15 +191,21 @@ __global__ void __launch_bounds__(1024) k_gj_panel(double2* __restrict__ a, int 
 }
 
 // The same panel step for full panels (nb = 16) of n <= 768 rows, with the
-// panel in registers: thread i holds row i of the 16 panel columns.  The 16
-// column steps are expanded at compile time (gj_reg_step<L>), so every register
-// index is a constant.  The pivot search reads the threads' own rows, and the
-// pivot row and row c are exchanged through LDS slots: three barriers per
-// column.  The arithmetic is k_gj_panel's, operation for operation.
+// panel in registers: thread i starts with row i of the 16 panel columns.  The
+// 16 column steps are expanded at compile time (gj_reg_step<L>), so every
+// register index is a constant.  Rows are never exchanged between threads:
+// each thread carries the matrix row it holds (lr), the pivot search takes the
+// largest |Re|+|Im| over rows lr >= c with the smallest row on ties (izamax),
+// and the interchange of rows c and p only swaps the two threads' row numbers.
+// The pivot row goes to the others through LDS: two barriers per column.  The
+// elimination multiplies by 1/piv once per row, r - (r_L/piv) rowp, where
+// k_gj_panel scales the pivot row first, r - r_L (rowp/piv): the same
+// arithmetic up to the order of one product.  The rows go back to the matrix at
+// their row numbers (gj_reg_store).
 struct GjRegShared {
   double wv[16];
   int wi[16];
-  double2 rowp[kPanelMax], rowc[kPanelMax], prow[kPanelMax];
+  double2 rowp[kPanelMax];
   int orig[768];
   int origb[768];  // the rows' sources since the last reset (one half of a 32-column panel)
   int nmove;
@@ -215,17 +229,18 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
   return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
-// Returns true on a zero pivot (the same for every thread of the block: the
-// pivot is read from LDS after a barrier).
+// A zero pivot sets the matrix's status bit and sh.singular and the steps go
+// on (on values that are then discarded): every thread runs the same steps and
+// meets the same barriers, and no branch keeps an older copy of the registers.
 template <int L>
-__device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared& sh, int i, int n,
-                                            int c0, bool mine, int b, int* __restrict__ ipiv,
-                                            int* __restrict__ status) {
+__device__ __forceinline__ void gj_reg_step(double2 (&r)[kPanelMax], int& lr, GjRegShared& sh,
+                                            int i, int n, int c0, bool mine, int b,
+                                            int* __restrict__ ipiv, int* __restrict__ status) {
   constexpr int NB = kPanelMax;
   const int lane = i & 63, wave = i >> 6, nwave = (int)(blockDim.x >> 6);
   const int c = c0 + L;
-  // pivot: largest |Re|+|Im| in rows c..n-1, first index on ties (izamax)
-  const double own = (mine && i >= c) ? fabs(r[L].x) + fabs(r[L].y) : -1.0;
+  // pivot: largest |Re|+|Im| in rows c..n-1, first row on ties (izamax)
+  const double own = (mine && lr >= c) ? fabs(r[L].x) + fabs(r[L].y) : -1.0;
   // wave maximum: butterfly within rows of 16 lanes (DPP), then the four rows
   double best = own;
   best = fmax(best, dpp_d<0xB1>(best));   // quad_perm [1,0,3,2]
@@ -234,13 +249,18 @@ __device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared
   best = fmax(best, dpp_d<0x128>(best));  // row_ror:8
   best = fmax(fmax(readlane_d(best, 0), readlane_d(best, 16)),
               fmax(readlane_d(best, 32), readlane_d(best, 48)));
-  // first row holding it (rows increase with the lane); none: no candidate here
-  const unsigned long long hit = __ballot(own == best);
-  int bi = (best >= 0.0 && hit) ? wave * 64 + (int)__builtin_ctzll(hit) : n;
-  if (lane == 0) { sh.wv[wave] = best; sh.wi[wave] = bi; }
+  // the smallest row holding it (n: no candidate here)
+  int key = (best >= 0.0 && own == best) ? lr : n;
+  key = min(key, __builtin_amdgcn_update_dpp(key, key, 0xB1, 0xf, 0xf, false));
+  key = min(key, __builtin_amdgcn_update_dpp(key, key, 0x4E, 0xf, 0xf, false));
+  key = min(key, __builtin_amdgcn_update_dpp(key, key, 0x124, 0xf, 0xf, false));
+  key = min(key, __builtin_amdgcn_update_dpp(key, key, 0x128, 0xf, 0xf, false));
+  key = min(min(__builtin_amdgcn_readlane(key, 0), __builtin_amdgcn_readlane(key, 16)),
+            min(__builtin_amdgcn_readlane(key, 32), __builtin_amdgcn_readlane(key, 48)));
+  if (lane == 0) { sh.wv[wave] = best; sh.wi[wave] = key; }
   __syncthreads();
   best = sh.wv[0];
-  bi = sh.wi[0];
+  int bi = sh.wi[0];
   for (int w = 1; w < nwave; ++w)
     if (sh.wv[w] > best || (sh.wv[w] == best && sh.wi[w] < bi)) { best = sh.wv[w]; bi = sh.wi[w]; }
   const int p = bi < n ? bi : c;  // no candidate compared (NaN column): keep row c
@@ -253,55 +273,49 @@ __device__ __forceinline__ bool gj_reg_step(double2 (&r)[kPanelMax], GjRegShared
     sh.origb[c] = sh.origb[p];
     sh.origb[p] = tb;
   }
-  if (i == p) {
+  const bool is_p = mine && lr == p;
+  if (is_p) {
 #pragma unroll
     for (int q = 0; q < NB; ++q) sh.rowp[q] = r[q];
   }
-  if (i == c) {
-#pragma unroll
-    for (int q = 0; q < NB; ++q) sh.rowc[q] = r[q];
-  }
   __syncthreads();
   const double2 piv = sh.rowp[L];
-  if (piv.x == 0.0 && piv.y == 0.0) {  // uniform: every thread reads the same pivot
-    if (i == 0) { atomicOr(status, 4); sh.singular = 1; }
-    return true;
+  if (i == 0 && piv.x == 0.0 && piv.y == 0.0) {
+    atomicOr(status, 4);
+    sh.singular = 1;
   }
-  // rows c and p exchanged (the pivot row moves to c)
-  if (i == p && p != c) {
+  const zd pz = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
+  const double2 pinv = make_double2(pz.x, pz.y);
+  if (is_p) {
+    // the pivot row becomes row c, scaled (its column L becomes 1/piv)
 #pragma unroll
-    for (int q = 0; q < NB; ++q) r[q] = sh.rowc[q];
-  }
-  // pivot row c scaled (its column L becomes 1/piv)
-  if (i < NB) {
-    const zd pinv = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
-    const zd u = i == L ? pinv : mkz(sh.rowp[i].x, sh.rowp[i].y) * pinv;
-    sh.prow[i] = make_double2(u.x, u.y);
-  }
-  __syncthreads();
-  if (i == c) {
-#pragma unroll
-    for (int q = 0; q < NB; ++q) r[q] = sh.prow[q];
+    for (int q = 0; q < NB; ++q)
+      r[q] = q == L ? pinv : cfma(make_double2(0.0, 0.0), r[q], pinv);
+    lr = c;
   } else if (mine) {
-    const zd t = mkz(r[L].x, r[L].y);
+    if (lr == c) lr = p;  // row c's content is row p's now
+    const double2 s = cfma(make_double2(0.0, 0.0), r[L], pinv);
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      const zd v = q == L ? mkz(0.0, 0.0) : mkz(r[q].x, r[q].y);
-      const zd u = v - t * mkz(sh.prow[q].x, sh.prow[q].y);
-      r[q] = make_double2(u.x, u.y);
-    }
+    for (int q = 0; q < NB; ++q)
+      r[q] = q == L ? make_double2(-s.x, -s.y) : cfms(r[q], s, sh.rowp[q]);
   }
-  return false;
 }
 
 template <int... L>
 __device__ __forceinline__ void gj_reg_steps(std::integer_sequence<int, L...>, double2 (&r)[kPanelMax],
-                                             GjRegShared& sh, int i, int n, int c0, bool mine, int b,
-                                             int* __restrict__ ipiv, int* __restrict__ status) {
-  // in order; a zero pivot ends the sequence for every thread alike (the flag
-  // is each thread's own, set from the same LDS value)
-  bool dead = false;
-  ((dead ? void() : void(dead = gj_reg_step<L>(r, sh, i, n, c0, mine, b, ipiv, status))), ...);
+                                             int& lr, GjRegShared& sh, int i, int n, int c0,
+                                             bool mine, int b, int* __restrict__ ipiv,
+                                             int* __restrict__ status) {
+  (gj_reg_step<L>(r, lr, sh, i, n, c0, mine, b, ipiv, status), ...);
+}
+
+// Columns [c0, c0+16) of the rows the threads hold, back to the matrix at
+// their row numbers.
+__device__ __forceinline__ void gj_reg_store(const double2 (&r)[kPanelMax], int lr, bool mine,
+                                             double2* __restrict__ m, int n, int c0) {
+  if (mine)
+#pragma unroll
+    for (int q = 0; q < kPanelMax; ++q) m[lr + (size_t)n * (c0 + q)] = r[q];
 }
 
 // Factors the panel [c0, c0+16) of matrix m held in r (row i of the panel in
@@ -314,12 +328,11 @@ __device__ __forceinline__ void gj_reg_factor(double2 (&r)[kPanelMax], GjRegShar
   if (mine) sh.orig[i] = i;
   if (i == 0) { sh.nmove = 0; sh.singular = 0; }
   __syncthreads();
-  gj_reg_steps(std::make_integer_sequence<int, NB>{}, r, sh, i, n, c0, mine, b, ipiv, status);
+  int lr = i;
+  gj_reg_steps(std::make_integer_sequence<int, NB>{}, r, lr, sh, i, n, c0, mine, b, ipiv, status);
   __syncthreads();
   const bool singular = sh.singular != 0;
-  if (!singular && mine)
-#pragma unroll
-    for (int q = 0; q < NB; ++q) m[i + (size_t)n * (c0 + q)] = r[q];
+  if (!singular) gj_reg_store(r, lr, mine, m, n, c0);
   // move list: the nb panel rows first (in order), then other rows whose
   // content changed
   if (i < NB) {
@@ -439,12 +452,8 @@ __global__ void __launch_bounds__(768) k_gj_step(double2* __restrict__ a, int n,
   if (mine) {
     for (int q = 0; q < NB; ++q) {
       const double2 wq = m[i + (size_t)n * (ck + q)];
-      const zd w = mkz(wq.x, wq.y);
 #pragma unroll
-      for (int t = 0; t < NB; ++t) {
-        const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.vals[q][t].x, sh.vals[q][t].y);
-        r[t] = make_double2(u.x, u.y);
-      }
+      for (int t = 0; t < NB; ++t) r[t] = cfma(r[t], wq, sh.vals[q][t]);
     }
   }
   if (!panel) {
@@ -511,12 +520,8 @@ __device__ __forceinline__ void gj32_apply(double2 (&r)[kPanelMax], const Gj32Sh
   if (!mine) return;
   for (int q = 0; q < W; ++q) {
     const double2 wq = m[i + (size_t)n * (ck + q)];
-    const zd w = mkz(wq.x, wq.y);
 #pragma unroll
-    for (int t = 0; t < kPanelMax; ++t) {
-      const zd u = mkz(r[t].x, r[t].y) + w * mkz(sh.vals[q][t].x, sh.vals[q][t].y);
-      r[t] = make_double2(u.x, u.y);
-    }
+    for (int t = 0; t < kPanelMax; ++t) r[t] = cfma(r[t], wq, sh.vals[q][t]);
   }
 }
 
@@ -550,32 +555,30 @@ __device__ __forceinline__ void gj32_chain(double2 (&r)[kPanelMax], Gj32Shared& 
   if (i == 0) sh.p.singular = 0;
   __syncthreads();
   // half A
-  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, sh.p, i, n, c, mine, b, ipiv,
+  int lr = i;
+  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, lr, sh.p, i, n, c, mine, b, ipiv,
                status);
   __syncthreads();
   if (sh.p.singular) {
     if (i == 0) mv[0] = -1;
     return;  // uniform
   }
-  if (mine)
-#pragma unroll
-    for (int q = 0; q < kPanelMax; ++q) m[i + (size_t)n * (c + q)] = r[q];
+  gj_reg_store(r, lr, mine, m, n, c);
   // half B after A's transform (A's moves: origb since the reset)
   gj32_list(sh, sh.p.origb, n, c, kPanelMax, i, mine);
   gj32_stage(sh, m, n, c + kPanelMax, kPanelMax, sh.nm);
   gj32_apply<kPanelMax>(r, sh, m, n, c, c + kPanelMax, kPanelMax, sh.nm, i, mine);
   if (mine) sh.p.origb[i] = i;
   __syncthreads();
-  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, sh.p, i, n, c + kPanelMax, mine, b,
-               ipiv, status);
+  lr = i;
+  gj_reg_steps(std::make_integer_sequence<int, kPanelMax>{}, r, lr, sh.p, i, n, c + kPanelMax, mine,
+               b, ipiv, status);
   __syncthreads();
   if (sh.p.singular) {
     if (i == 0) mv[0] = -1;
     return;  // uniform
   }
-  if (mine)
-#pragma unroll
-    for (int q = 0; q < kPanelMax; ++q) m[i + (size_t)n * (c + kPanelMax + q)] = r[q];
+  gj_reg_store(r, lr, mine, m, n, c + kPanelMax);
   // half A after B's transform (B's moves: origb since its reset)
   gj32_list(sh, sh.p.origb, n, c + kPanelMax, kPanelMax, i, mine);
   gj32_stage(sh, m, n, c, kPanelMax, sh.nm);
@@ -707,10 +710,10 @@ __global__ void __launch_bounds__(256) k_gj_update(double2* __restrict__ a, int 
   __syncthreads();
   const double2* W = m + (size_t)n * c0;
   for (int i = tid; i < n; i += blockDim.x) {
-    zd w[kPanelMax];
+    double2 w[kPanelMax];
 #pragma unroll
     for (int q = 0; q < kPanelMax; ++q)
-      if (q < nb) w[q] = mkz(W[i + (size_t)n * q].x, W[i + (size_t)n * q].y);
+      if (q < nb) w[q] = W[i + (size_t)n * q];
     const bool inpanel = i >= c0 && i < c0 + nb;
     int slot = -1;
     if (!inpanel)
@@ -719,12 +722,11 @@ __global__ void __launch_bounds__(256) k_gj_update(double2* __restrict__ a, int 
     for (int t = 0; t < ntj; ++t) {
       const int jj = jt0 + t, j = jj < c0 ? jj : jj + nb;
       double2* out = m + i + (size_t)n * j;
-      zd acc = inpanel ? mkz(0.0, 0.0)
-                       : (slot >= 0 ? mkz(vals[slot][t].x, vals[slot][t].y) : mkz(out->x, out->y));
+      double2 acc = inpanel ? make_double2(0.0, 0.0) : (slot >= 0 ? vals[slot][t] : *out);
 #pragma unroll
       for (int q = 0; q < kPanelMax; ++q)
-        if (q < nb) acc = acc + w[q] * mkz(vals[q][t].x, vals[q][t].y);
-      *out = make_double2(acc.x, acc.y);
+        if (q < nb) acc = cfma(acc, w[q], vals[q][t]);
+      *out = acc;
     }
   }
 }
