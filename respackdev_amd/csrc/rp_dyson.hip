@@ -206,6 +206,7 @@ struct GjRegShared {
   double wv[16];
   int wi[16];
   double2 rowp[kPanelMax];
+  double2 pinv;  // 1/piv, formed by the pivot row's thread
   int orig[768];
   int origb[768];  // the rows' sources since the last reset (one half of a 32-column panel)
   int nmove;
@@ -277,15 +278,15 @@ __device__ __forceinline__ void gj_reg_step(double2 (&r)[kPanelMax], int& lr, Gj
   if (is_p) {
 #pragma unroll
     for (int q = 0; q < NB; ++q) sh.rowp[q] = r[q];
+    const zd pz = mkz(1.0, 0.0) / mkz(r[L].x, r[L].y);
+    sh.pinv = make_double2(pz.x, pz.y);
+    if (r[L].x == 0.0 && r[L].y == 0.0) {
+      atomicOr(status, 4);
+      sh.singular = 1;
+    }
   }
   __syncthreads();
-  const double2 piv = sh.rowp[L];
-  if (i == 0 && piv.x == 0.0 && piv.y == 0.0) {
-    atomicOr(status, 4);
-    sh.singular = 1;
-  }
-  const zd pz = mkz(1.0, 0.0) / mkz(piv.x, piv.y);
-  const double2 pinv = make_double2(pz.x, pz.y);
+  const double2 pinv = sh.pinv;
   if (is_p) {
     // the pivot row becomes row c, scaled (its column L becomes 1/piv)
 #pragma unroll
@@ -482,29 +483,32 @@ constexpr int kSuper = 32;
 constexpr int kMoveMax32 = 2 * kSuper;          // panel rows and rows they displaced
 constexpr int kMoveInts32 = 1 + 2 * kMoveMax32;  // {count, dst[64], src[64]}
 
+constexpr int kUpdCols = 32;  // columns per update workgroup of a 32-column step
+
 struct Gj32Shared {
   GjRegShared p;
-  double2 vals[kMoveMax32][kPanelMax];  // old values of the moved rows in 16 columns
+  double2 vals[kMoveMax32][kUpdCols];  // old values of the moved rows in up to 32 columns
   int dst[kMoveMax32], src[kMoveMax32];
   int nm;
 };
 
 // Loads into sh.vals the old values of the nm moved rows (sources sh.src) in
-// columns [cj, cj + ntj) and ends with a barrier.
+// columns [cj, cj + ntj) of a CW-column group and ends with a barrier.
+template <int CW>
 __device__ __forceinline__ void gj32_stage(Gj32Shared& sh, const double2* __restrict__ m, int n,
                                            int cj, int ntj, int nm) {
-  for (int x = threadIdx.x; x < nm * kPanelMax; x += blockDim.x) {
-    const int e = x / kPanelMax, t = x % kPanelMax;
+  for (int x = threadIdx.x; x < nm * CW; x += blockDim.x) {
+    const int e = x / CW, t = x % CW;
     sh.vals[e][t] = t < ntj ? m[sh.src[e] + (size_t)n * (cj + t)] : make_double2(0.0, 0.0);
   }
   __syncthreads();
 }
 
-// Row i of columns [cj, cj + ntj) after the rank-W transform of the panel
-// [ck, ck + W): the staged move list (sh.dst/sh.vals, its first W entries the
-// panel rows in order) and the panel columns W(i, q) read from m.
-template <int W>
-__device__ __forceinline__ void gj32_apply(double2 (&r)[kPanelMax], const Gj32Shared& sh,
+// Row i of columns [cj, cj + ntj) (at most CW) after the rank-W transform of
+// the panel [ck, ck + W): the staged move list (sh.dst/sh.vals, its first W
+// entries the panel rows in order) and the panel columns W(i, q) read from m.
+template <int W, int CW>
+__device__ __forceinline__ void gj32_apply(double2 (&r)[CW], const Gj32Shared& sh,
                                            const double2* __restrict__ m, int n, int ck, int cj,
                                            int ntj, int nm, int i, bool mine) {
   const bool inpanel = i >= ck && i < ck + W;
@@ -513,7 +517,7 @@ __device__ __forceinline__ void gj32_apply(double2 (&r)[kPanelMax], const Gj32Sh
     for (int e = W; e < nm; ++e)
       if (sh.dst[e] == i) slot = e;
 #pragma unroll
-  for (int t = 0; t < kPanelMax; ++t)
+  for (int t = 0; t < CW; ++t)
     r[t] = (!mine || inpanel || t >= ntj) ? make_double2(0.0, 0.0)
            : slot >= 0                    ? sh.vals[slot][t]
                                           : m[i + (size_t)n * (cj + t)];
@@ -521,7 +525,7 @@ __device__ __forceinline__ void gj32_apply(double2 (&r)[kPanelMax], const Gj32Sh
   for (int q = 0; q < W; ++q) {
     const double2 wq = m[i + (size_t)n * (ck + q)];
 #pragma unroll
-    for (int t = 0; t < kPanelMax; ++t) r[t] = cfma(r[t], wq, sh.vals[q][t]);
+    for (int t = 0; t < CW; ++t) r[t] = cfma(r[t], wq, sh.vals[q][t]);
   }
 }
 
@@ -566,8 +570,8 @@ __device__ __forceinline__ void gj32_chain(double2 (&r)[kPanelMax], Gj32Shared& 
   gj_reg_store(r, lr, mine, m, n, c);
   // half B after A's transform (A's moves: origb since the reset)
   gj32_list(sh, sh.p.origb, n, c, kPanelMax, i, mine);
-  gj32_stage(sh, m, n, c + kPanelMax, kPanelMax, sh.nm);
-  gj32_apply<kPanelMax>(r, sh, m, n, c, c + kPanelMax, kPanelMax, sh.nm, i, mine);
+  gj32_stage<kPanelMax>(sh, m, n, c + kPanelMax, kPanelMax, sh.nm);
+  gj32_apply<kPanelMax, kPanelMax>(r, sh, m, n, c, c + kPanelMax, kPanelMax, sh.nm, i, mine);
   if (mine) sh.p.origb[i] = i;
   __syncthreads();
   lr = i;
@@ -581,8 +585,8 @@ __device__ __forceinline__ void gj32_chain(double2 (&r)[kPanelMax], Gj32Shared& 
   gj_reg_store(r, lr, mine, m, n, c + kPanelMax);
   // half A after B's transform (B's moves: origb since its reset)
   gj32_list(sh, sh.p.origb, n, c + kPanelMax, kPanelMax, i, mine);
-  gj32_stage(sh, m, n, c, kPanelMax, sh.nm);
-  gj32_apply<kPanelMax>(r, sh, m, n, c + kPanelMax, c, kPanelMax, sh.nm, i, mine);
+  gj32_stage<kPanelMax>(sh, m, n, c, kPanelMax, sh.nm);
+  gj32_apply<kPanelMax, kPanelMax>(r, sh, m, n, c + kPanelMax, c, kPanelMax, sh.nm, i, mine);
   __syncthreads();  // every staged read of A is done before A is overwritten
   if (mine)
 #pragma unroll
@@ -613,11 +617,12 @@ __global__ void __launch_bounds__(768) k_gj_first32(double2* __restrict__ a, int
 // move list moves_k) on every other column and, when next is set, the
 // factoring of panel k+1.  Blocks [0, nbatch) (next set) are panel blocks:
 // they update the 32 columns of panel k+1 (half B to HBM, half A kept in
-// registers) and factor it (gj32_chain).  Every other block updates 16 other
-// columns of one matrix: those outside [ck, ck + 32 (1 + next)), ngroups per
-// matrix.  The two kinds touch disjoint columns and read panel k's columns
-// only.  With nbatch a multiple of 8 the update blocks of matrix b run on XCD
-// b mod 8.
+// registers) and factor it (gj32_chain).  Every other block updates 32 other
+// columns of one matrix (fewer in the last group): those outside
+// [ck, ck + 32 (1 + next)), ngroups per matrix, so a block reads the panel's
+// 32 columns once for 32 of its own.  The two kinds touch disjoint columns and
+// read panel k's columns only.  With nbatch a multiple of 8 the update blocks
+// of matrix b run on XCD b mod 8.
 __global__ void __launch_bounds__(768) k_gj_step32(double2* __restrict__ a, int n, int ck, int next,
                                                    int nbatch, int ngroups, int* __restrict__ ipiv,
                                                    const int* __restrict__ moves_k,
@@ -642,9 +647,9 @@ __global__ void __launch_bounds__(768) k_gj_step32(double2* __restrict__ a, int 
       t = u % ngroups;
     }
     const int skip = kSuper * (1 + next);  // columns [ck, ck + skip) are not in the groups
-    const int t0 = t * kPanelMax;          // groups never straddle ck (multiples of 16)
+    const int t0 = t * kUpdCols;           // groups never straddle ck (multiples of 32)
     cj = t0 < ck ? t0 : t0 + skip;
-    ntj = min(kPanelMax, n - skip - t0);
+    ntj = min(kUpdCols, n - skip - t0);
   }
   double2* m = a + (size_t)b * n * n;
   const int* mv = moves_k + (size_t)b * kMoveInts32;
@@ -658,26 +663,27 @@ __global__ void __launch_bounds__(768) k_gj_step32(double2* __restrict__ a, int 
     sh.src[i] = mv[1 + kMoveMax32 + i];
   }
   __syncthreads();
-  double2 r[kPanelMax];
   if (!panel) {
-    gj32_stage(sh, m, n, cj, ntj, nm);
-    gj32_apply<kSuper>(r, sh, m, n, ck, cj, ntj, nm, i, mine);
+    double2 ru[kUpdCols];
+    gj32_stage<kUpdCols>(sh, m, n, cj, ntj, nm);
+    gj32_apply<kSuper, kUpdCols>(ru, sh, m, n, ck, cj, ntj, nm, i, mine);
     if (mine)
 #pragma unroll
-      for (int t = 0; t < kPanelMax; ++t)
-        if (t < ntj) m[i + (size_t)n * (cj + t)] = r[t];
+      for (int t = 0; t < kUpdCols; ++t)
+        if (t < ntj) m[i + (size_t)n * (cj + t)] = ru[t];
     return;  // uniform per block
   }
+  double2 r[kPanelMax];
   // panel k+1: half B brought up to date into HBM, then half A into registers
   const int c = ck + kSuper;
-  gj32_stage(sh, m, n, c + kPanelMax, kPanelMax, nm);
-  gj32_apply<kSuper>(r, sh, m, n, ck, c + kPanelMax, kPanelMax, nm, i, mine);
+  gj32_stage<kPanelMax>(sh, m, n, c + kPanelMax, kPanelMax, nm);
+  gj32_apply<kSuper, kPanelMax>(r, sh, m, n, ck, c + kPanelMax, kPanelMax, nm, i, mine);
   __syncthreads();  // staged reads of B done before B is overwritten
   if (mine)
 #pragma unroll
     for (int t = 0; t < kPanelMax; ++t) m[i + (size_t)n * (c + kPanelMax + t)] = r[t];
-  gj32_stage(sh, m, n, c, kPanelMax, nm);
-  gj32_apply<kSuper>(r, sh, m, n, ck, c, kPanelMax, nm, i, mine);
+  gj32_stage<kPanelMax>(sh, m, n, c, kPanelMax, nm);
+  gj32_apply<kSuper, kPanelMax>(r, sh, m, n, ck, c, kPanelMax, nm, i, mine);
   __syncthreads();  // every thread's B row is written before the chain stages B rows
   gj32_chain(r, sh, m, n, c, b, ipiv, moves_next + (size_t)b * kMoveInts32, status);
 }
@@ -800,7 +806,7 @@ void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, in
     for (int k = 0; k < nfull; ++k) {
       const int next = k + 1 < nfull ? 1 : 0;
       const int nrest = n - kSuper * (1 + next);
-      const int ngroups = (nrest + kPanelMax - 1) / kPanelMax;
+      const int ngroups = (nrest + kUpdCols - 1) / kUpdCols;
       const int nblk = (next ? nbatch : 0) + nbatch * ngroups;
       if (nblk > 0)
         hipLaunchKernelGGL(k_gj_step32, dim3(nblk), dim3(nt), 0, s, a, n, k * kSuper, next, nbatch,
