@@ -182,14 +182,62 @@ __constant__ double kMomentInv[kSeriesMax] = {
     RP_MI(56), RP_MI(57), RP_MI(58), RP_MI(59), RP_MI(60), RP_MI(61), RP_MI(62), RP_MI(63),
 #undef RP_MI
 };
+// The sweep over non-degenerate tetrahedra takes the series only below
+// kFastR (at most 37 terms; the closed forms lose at most ~24 r^-3 = 560 ulps
+// above it) and counts terms from r^2 in single precision: r^n < 2^-54 for
+// n > 108 / -log2(r^2), with a quarter term of margin for the rounding.
+constexpr double kFastR = 0.35;
+constexpr double kFastInvR2 = 1.0 / (kFastR * kFastR);
+__device__ __forceinline__ int fast_terms(double r2) {
+  const float lg = __log2f((float)r2);
+  return r2 > 0.0 ? min(kSeriesMax - 1, (int)ceilf(-108.0f / lg + 0.25f)) : 0;
+}
+// Both items of a tetrahedron by one series: the nodes of dsgn = -1 are the
+// negated nodes of dsgn = +1, so d_j and the mean change sign and P_{v,n}
+// takes (-1)^n; one set of recurrences serves both, and the two items' terms
+// (the second taken with its factor dsgn = -1) sum to
+//   T_v = W_v(+) - W_v(-) = sum c_n P_{v,n} ((1/s+)(-1/s+)^n - (1/s-)^(n+1))
+//       = -sum c_n P_{v,n} (u^(n+1) + v^(n+1)),  u = -1/s+, v = 1/s-,
+// with s+- = e +- c and |s+-|^2 = ap, am (u, v formed as conj(s)/|s|^2).
+// r^2 = dm^2 / min(ap, am) sets the terms.
+__device__ __forceinline__ void joint_series(double d0, double d1, double d2, double d3, double dm,
+                                             zd sp, zd sm, double ap, double am, zd& T0, zd& T1,
+                                             zd& T2, zd& T3) {
+  const int nmax = fast_terms(dm * dm / fmin(ap, am));
+  const double q = 1.0 / (ap * am), ip = am * q, im = ap * q;
+  const zd u = mkz(-sp.x * ip, sp.y * ip), v = mkz(sm.x * im, -sm.y * im);
+  zd U = u, V = v;
+  T0 = T1 = T2 = T3 = mkz(0.0, 0.0);
+  double h1 = 1.0, h2 = 1.0, h3 = 1.0, h = 1.0;
+  double p0 = 1.0, p1 = 1.0, p2 = 1.0, p3 = 1.0;
+  for (int n = 0; n <= nmax; ++n) {
+    const double cf = -kMomentInv[n];
+    const zd W = (U + V) * cf;
+    T0 = T0 + W * p0;
+    T1 = T1 + W * p1;
+    T2 = T2 + W * p2;
+    T3 = T3 + W * p3;
+    U = U * u;
+    V = V * v;
+    h1 *= d0;
+    h2 = h1 + d1 * h2;
+    h3 = h2 + d2 * h3;
+    h = h3 + d3 * h;
+    p0 = h + d0 * p0;
+    p1 = h + d1 * p1;
+    p2 = h + d2 * p2;
+    p3 = h + d3 * p3;
+  }
+}
+
 __device__ __forceinline__ bool series_w(zd g0, zd g1, zd g2, zd g3, zd e, zd& w0, zd& w1,
-                                         zd& w2, zd& w3) {
+                                         zd& w2, zd& w3, double rmax = kSeriesR) {
   const zd c = (g0 + g1 + g2 + g3) * 0.25;
   const zd s = e + c;
   const double d0 = g0.x - c.x, d1 = g1.x - c.x, d2 = g2.x - c.x, d3 = g3.x - c.x;
   const double dm = fmax(fmax(fabs(d0), fabs(d1)), fmax(fabs(d2), fabs(d3)));
   const double as = absz(s);
-  if (!(dm < kSeriesR * as)) return false;
+  if (!(dm < rmax * as)) return false;
   const int nmax = series_terms(dm / as);
   const zd t = -1.0 / s;
   zd tn = 1.0 / s;
@@ -270,7 +318,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const int* __restrict__ index_kpt, const int* __restrict__ pocc,
     const int* __restrict__ pvir, const int* __restrict__ ikq, const double* __restrict__ eband,
     const double2* __restrict__ pband, double ef, double delt, double dmna, double dmnr, int nen,
-    int nf, const double2* __restrict__ em, size_t ldd, int cuts_elsewhere,
+    int nf, const double2* __restrict__ em, size_t ldd, int cuts_elsewhere, int qcap,
     int* __restrict__ status, double2* __restrict__ dout) {
   extern __shared__ double2 lds[];
   const int nk3 = n1 * n2 * n3;
@@ -280,7 +328,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   double* e1 = (double*)(lv + (size_t)nk3 * nf * 2);  // [nk3]
   double* e2 = e1 + nk3;                          // [nk3]
   int2* list = (int2*)(e2 + nk3);                 // [ntet] {tetrahedron, class}
-  __shared__ int count[2];
+  int* queue = (int*)(list + ntet);               // [qcap] (item, frequency) for the closed forms
+  __shared__ int count[3];
   const int p = blockIdx.x;
   const int f0 = blockIdx.y * nf, nfb = min(nf, nen - f0);
   const int jo = pocc[p], iv = pvir[p];
@@ -290,7 +339,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     e1[gi] = (double)(float)eband[(size_t)jo * ntk + k];
     e2[gi] = (double)(float)eband[(size_t)iv * ntk + ikq[k]];
   }
-  if (threadIdx.x < 2) count[threadIdx.x] = 0;
+  if (threadIdx.x < 3) count[threadIdx.x] = 0;
   __syncthreads();
   // ---- pass 1: cut test (ttct), keep test, class, per tetrahedron: its two
   // items (dsgn = +1 and -1) have vertex values g and -g, hence the same cut
@@ -333,13 +382,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const int nnormal = count[0], nother = count[1];
   // ---- pass 2: non-degenerate tetrahedra (calc, case(0); ttrhdrn_sub2normal x3 and
   // ttrhdrn_sub1normal, m_tetrahedron_20170325.F:215-229,431-451)
-  // Work units (item, chunk of frequencies): with few tetrahedra (small k
-  // meshes) each item's frequencies are split so that about twice the
-  // block's threads have work; the per-item factors are formed per unit.
-  const int fsplit = max(1, min(nfb, (2 * (int)blockDim.x + max(nnormal, 1) - 1) / max(nnormal, 1)));
-  const int fchunk = (nfb + fsplit - 1) / fsplit;
-  for (int jf = threadIdx.x; jf < nnormal * fsplit; jf += blockDim.x) {
-    const int j = jf / fsplit, fa = (jf % fsplit) * fchunk, fb = min(nfb, fa + fchunk);
+  // Closed forms of one (item j, frequency f): per sign the series where its
+  // ratio is below kFastR, else the closed forms from the vertex logarithms.
+  auto closed_unit = [&](int j, int f) {
     const int2 ent = list[j];
     const Item m = load_item(ent.x, ntet, imt, e1, e2, delt);
     const int code = ent.y;
@@ -352,10 +397,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const int k1 = sel4(i1, m.c[0], m.c[1], m.c[2], m.c[3]);
     const int k2 = sel4(i2, m.c[0], m.c[1], m.c[2], m.c[3]);
     const int k3 = sel4(i3, m.c[0], m.c[1], m.c[2], m.c[3]);
-    double2* x0 = x + (size_t)k0 * nf;
-    double2* x1 = x + (size_t)k1 * nf;
-    double2* x2 = x + (size_t)k2 * nf;
-    double2* x3 = x + (size_t)k3 * nf;
     const double2* l0 = lv + (size_t)k0 * nf * 2;
     const double2* l1 = lv + (size_t)k1 * nf * 2;
     const double2* l2 = lv + (size_t)k2 * nf * 2;
@@ -374,111 +415,121 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     const zd K31 = P2 * -s6, K32 = r12 * P1 * s6, K33 = r23 * P3 * s6, K34 = r02 * P0 * s6;
     // sub1normal(A0,A3,A1,A2)
     const zd J1 = P0 * 0.5, J2 = P1 * -0.5, J3 = P2 * -0.5;
-    // the series' nodes for dsgn = +1 (series_w's operations): mean sc, offsets sd
-    const zd sc = (A0 + A1 + A2 + A3) * 0.25;
-    const double sd0 = A0.x - sc.x, sd1 = A1.x - sc.x, sd2 = A2.x - sc.x, sd3 = A3.x - sc.x;
-    const double sdm = fmax(fmax(fabs(sd0), fabs(sd1)), fmax(fabs(sd2), fabs(sd3)));
     // Both items of the tetrahedron: dsgn = s takes the vertex values s*A, so
     // every r_ij takes the factor s: the factors of three of them (K11, K21,
     // K31, J1..J3) change sign, those of four do not (exactly, in IEEE
     // arithmetic).  Their corner terms are summed before the LDS atomics.
+    const double2 ev = em[f0 + f];
+    const zd e = mkz(ev.x, ev.y);
+    zd T0 = mkz(0.0, 0.0), T1 = mkz(0.0, 0.0), T2 = mkz(0.0, 0.0), T3 = mkz(0.0, 0.0);
+#pragma unroll
+    for (int sg = 0; sg < 2; ++sg) {
+      const double s = sg ? -1.0 : 1.0;
+      {
+        zd w0, w1, w2, w3;
+        if (series_w(A0 * s, A1 * s, A2 * s, A3 * s, e, w0, w1, w2, w3, kFastR)) {
+          T0 = T0 + w0 * s;
+          T1 = T1 + w1 * s;
+          T2 = T2 + w2 * s;
+          T3 = T3 + w3 * s;
+          continue;
+        }
+      }
+      const zd v0 = A0 * s + e, v1 = A1 * s + e, v2 = A2 * s + e, v3 = A3 * s + e;
+      // log(v_i), v_i = s A_i + e, from the vertex table (the same operations)
+      const double2 w0 = l0[2 * f + sg], w1 = l1[2 * f + sg], w2 = l2[2 * f + sg],
+                    w3 = l3[2 * f + sg];
+      const zd L0 = mkz(w0.x, w0.y), L1 = mkz(w1.x, w1.y), L2 = mkz(w2.x, w2.y),
+               L3 = mkz(w3.x, w3.y);
+      const zd D10 = logdiff(L1, L0), D20 = logdiff(L2, L0), D30 = logdiff(L3, L0);
+      const zd D21 = logdiff(L2, L1), D31 = logdiff(L3, L1), D32 = logdiff(L3, L2);
+      const zd q0 = v0 * v0, q1 = v1 * v1, q2 = v2 * v2, q3 = v3 * v3;
+      const zd u0 = q0 * v0, u1 = q1 * v1, u2 = q2 * v2, u3 = q3 * v3;
+      zd c1 = -q0 * (K11 * s);
+      c1 = c1 - u2 * D20 * K12;
+      c1 = c1 + u3 * D30 * K13;
+      c1 = c1 + u1 * D10 * K14;
+      zd c2 = -q1 * (K21 * s);
+      c2 = c2 - u2 * D21 * K22;
+      c2 = c2 + u3 * D31 * K23;
+      c2 = c2 - u0 * D10 * K24;
+      zd c3 = -q2 * (K31 * s);
+      c3 = c3 + u1 * D21 * K32;
+      c3 = c3 + u3 * D32 * K33;
+      c3 = c3 - u0 * D20 * K34;
+      const zd l1 = q3 * D30;
+      zd c4 = -l1 * (J1 * s);
+      c4 = c4 - (l1 - q1 * D10) * (J2 * s);
+      c4 = c4 + (l1 - q2 * D20) * (J3 * s);
+      T0 = T0 + c1 * s;
+      T1 = T1 + c2 * s;
+      T2 = T2 + c3 * s;
+      T3 = T3 + (c4 - c3 - c2 - c1) * s;
+    }
+    lds_add(x + (size_t)k0 * nf + f, T0);
+    lds_add(x + (size_t)k1 * nf + f, T1);
+    lds_add(x + (size_t)k2 * nf + f, T2);
+    lds_add(x + (size_t)k3 * nf + f, T3);
+  };
+  // The sweep: work units (item, chunk of frequencies); with few tetrahedra
+  // (small k meshes) each item's frequencies are split so that about twice
+  // the block's threads have work.  At a frequency where both signs' ratios
+  // are below kFastR (the usual case) the two series are summed here; the
+  // other (item, frequency) pairs are queued for closed_unit after the sweep
+  // (inline when the queue is full), so that the lanes of a wave do not wait
+  // on the closed forms of a few.
+  const int fsplit = max(1, min(nfb, (2 * (int)blockDim.x + max(nnormal, 1) - 1) / max(nnormal, 1)));
+  const int fchunk = (nfb + fsplit - 1) / fsplit;
+  for (int jf = threadIdx.x; jf < nnormal * fsplit; jf += blockDim.x) {
+    const int j = jf / fsplit, fa = (jf % fsplit) * fchunk, fb = min(nfb, fa + fchunk);
+    const int2 ent = list[j];
+    const Item m = load_item(ent.x, ntet, imt, e1, e2, delt);
+    const int code = ent.y;
+    const int i0 = vtx(code, 1, 0), i1 = vtx(code, 1, 1), i2 = vtx(code, 1, 2), i3 = vtx(code, 1, 3);
+    const zd A0 = sel4(i0, m.g[0], m.g[1], m.g[2], m.g[3]);
+    const zd A1 = sel4(i1, m.g[0], m.g[1], m.g[2], m.g[3]);
+    const zd A2 = sel4(i2, m.g[0], m.g[1], m.g[2], m.g[3]);
+    const zd A3 = sel4(i3, m.g[0], m.g[1], m.g[2], m.g[3]);
+    double2* x0 = x + (size_t)sel4(i0, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
+    double2* x1 = x + (size_t)sel4(i1, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
+    double2* x2 = x + (size_t)sel4(i2, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
+    double2* x3 = x + (size_t)sel4(i3, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
+    // the series' nodes for dsgn = +1 (series_w's operations): mean sc, offsets sd
+    const zd sc = (A0 + A1 + A2 + A3) * 0.25;
+    const double sd0 = A0.x - sc.x, sd1 = A1.x - sc.x, sd2 = A2.x - sc.x, sd3 = A3.x - sc.x;
+    const double sdm = fmax(fmax(fabs(sd0), fabs(sd1)), fmax(fabs(sd2), fabs(sd3)));
+    const double lim = sdm * sdm * kFastInvR2;  // r < kFastR  <=>  |s|^2 > lim
 #pragma unroll 1
     for (int f = fa; f < fb; ++f) {
       const double2 ev = em[f0 + f];
-      const zd e = mkz(ev.x, ev.y);
-      zd T0 = mkz(0.0, 0.0), T1 = mkz(0.0, 0.0), T2 = mkz(0.0, 0.0), T3 = mkz(0.0, 0.0);
-      // Both signs by the series (the usual case): the nodes of dsgn = -1 are
-      // -A, so d_j and the mean change sign and P_{v,n} takes (-1)^n: one set of
-      // recurrences serves both, W(+) = sum (1/s+)(-1/s+)^n c_n P_n and
-      // W(-) = sum (1/s-)(1/s-)^n c_n P_n with s+- = e +- c (the same values as
-      // two calls of series_w, the last terms aside).
-      const zd sp = e + sc, sm = e - sc;
-      const double rp = sdm / absz(sp), rm = sdm / absz(sm);
-      if (rp < kSeriesR && rm < kSeriesR) {
-        const int nmax = max(series_terms(rp), series_terms(rm));
-        const zd tp = -1.0 / sp, tm = 1.0 / sm;
-        zd np = 1.0 / sp, nm = 1.0 / sm;
-        double h1 = 1.0, h2 = 1.0, h3 = 1.0, h = 1.0;
-        double p0 = 1.0, p1 = 1.0, p2 = 1.0, p3 = 1.0;
-        for (int n = 0; n <= nmax; ++n) {
-          const double cf = kMomentInv[n];
-          const double c0 = cf * p0, c1 = cf * p1, c2 = cf * p2, c3 = cf * p3;
-          T0 = T0 + (np - nm) * c0;
-          T1 = T1 + (np - nm) * c1;
-          T2 = T2 + (np - nm) * c2;
-          T3 = T3 + (np - nm) * c3;
-          np = np * tp;
-          nm = nm * tm;
-          h1 *= sd0;
-          h2 = h1 + sd1 * h2;
-          h3 = h2 + sd2 * h3;
-          h = h3 + sd3 * h;
-          p0 = h + sd0 * p0;
-          p1 = h + sd1 * p1;
-          p2 = h + sd2 * p2;
-          p3 = h + sd3 * p3;
-        }
-        lds_add(x0 + f, T0);
-        lds_add(x1 + f, T1);
-        lds_add(x2 + f, T2);
-        lds_add(x3 + f, T3);
+      // both signs by the series (joint_series) where both ratios are below kFastR
+      const zd sp = mkz(ev.x + sc.x, ev.y + sc.y), sm = mkz(ev.x - sc.x, ev.y - sc.y);
+      const double ap = sp.x * sp.x + sp.y * sp.y, am = sm.x * sm.x + sm.y * sm.y;
+      if (!(lim < ap && lim < am)) {
+        const int slot = atomicAdd(&count[2], 1);
+        if (slot < qcap) queue[slot] = j * nf + f;
+        else closed_unit(j, f);
         continue;
       }
-#pragma unroll
-      for (int sg = 0; sg < 2; ++sg) {
-        const double s = sg ? -1.0 : 1.0;
-        {
-          zd w0, w1, w2, w3;
-          if (series_w(A0 * s, A1 * s, A2 * s, A3 * s, e, w0, w1, w2, w3)) {
-            T0 = T0 + w0 * s;
-            T1 = T1 + w1 * s;
-            T2 = T2 + w2 * s;
-            T3 = T3 + w3 * s;
-            continue;
-          }
-        }
-        const zd v0 = A0 * s + e, v1 = A1 * s + e, v2 = A2 * s + e, v3 = A3 * s + e;
-        // log(v_i), v_i = s A_i + e, from the vertex table (the same operations)
-        const double2 w0 = l0[2 * f + sg], w1 = l1[2 * f + sg], w2 = l2[2 * f + sg],
-                      w3 = l3[2 * f + sg];
-        const zd L0 = mkz(w0.x, w0.y), L1 = mkz(w1.x, w1.y), L2 = mkz(w2.x, w2.y),
-                 L3 = mkz(w3.x, w3.y);
-        const zd D10 = logdiff(L1, L0), D20 = logdiff(L2, L0), D30 = logdiff(L3, L0);
-        const zd D21 = logdiff(L2, L1), D31 = logdiff(L3, L1), D32 = logdiff(L3, L2);
-        const zd q0 = v0 * v0, q1 = v1 * v1, q2 = v2 * v2, q3 = v3 * v3;
-        const zd u0 = q0 * v0, u1 = q1 * v1, u2 = q2 * v2, u3 = q3 * v3;
-        zd c1 = -q0 * (K11 * s);
-        c1 = c1 - u2 * D20 * K12;
-        c1 = c1 + u3 * D30 * K13;
-        c1 = c1 + u1 * D10 * K14;
-        zd c2 = -q1 * (K21 * s);
-        c2 = c2 - u2 * D21 * K22;
-        c2 = c2 + u3 * D31 * K23;
-        c2 = c2 - u0 * D10 * K24;
-        zd c3 = -q2 * (K31 * s);
-        c3 = c3 + u1 * D21 * K32;
-        c3 = c3 + u3 * D32 * K33;
-        c3 = c3 - u0 * D20 * K34;
-        const zd l1 = q3 * D30;
-        zd c4 = -l1 * (J1 * s);
-        c4 = c4 - (l1 - q1 * D10) * (J2 * s);
-        c4 = c4 + (l1 - q2 * D20) * (J3 * s);
-        T0 = T0 + c1 * s;
-        T1 = T1 + c2 * s;
-        T2 = T2 + c3 * s;
-        T3 = T3 + (c4 - c3 - c2 - c1) * s;
-      }
+      zd T0, T1, T2, T3;
+      joint_series(sd0, sd1, sd2, sd3, sdm, sp, sm, ap, am, T0, T1, T2, T3);
       lds_add(x0 + f, T0);
       lds_add(x1 + f, T1);
       lds_add(x2 + f, T2);
       lds_add(x3 + f, T3);
     }
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < min(count[2], qcap); i += blockDim.x) closed_unit(queue[i] / nf, queue[i] % nf);
   // ---- pass 3: degenerate tetrahedra (calc, m_tetrahedron_20170325.F:231-300)
-  for (int jj = threadIdx.x; jj < 2 * nother; jj += blockDim.x) {
-    const int j = jj >> 1;  // entry j, both of its items
+  // Work units (entry, chunk of frequencies) as in the sweep, both items of
+  // the entry (dsgn = +1 and -1) per unit.
+  const int fsplit3 = max(1, min(nfb, (2 * (int)blockDim.x + max(nother, 1) - 1) / max(nother, 1)));
+  const int fchunk3 = (nfb + fsplit3 - 1) / fsplit3;
+  for (int u3 = threadIdx.x; u3 < nother * fsplit3; u3 += blockDim.x) {
+    const int j = u3 / fsplit3, fa = (u3 % fsplit3) * fchunk3, fb = min(nfb, fa + fchunk3);
     const int2 ent = list[ntet - 1 - j];
-    const Item m = load_item(ent.x + (jj & 1) * ntet, ntet, imt, e1, e2, delt);
+    const Item m = load_item(ent.x, ntet, imt, e1, e2, delt);  // dsgn = +1
     const int code = ent.y, kind = code & 15;
     const int b0 = vtx(code, 0, 0), b1 = vtx(code, 0, 1), b2 = vtx(code, 0, 2), b3 = vtx(code, 0, 3);
     const int o0 = vtx(code, 1, 0), o1 = vtx(code, 1, 1);
@@ -494,7 +545,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     double2* y33 = x + (size_t)sel4(b3, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
     double2* y40 = x + (size_t)sel4(o0, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
     double2* y41 = x + (size_t)sel4(o1, m.c[0], m.c[1], m.c[2], m.c[3]) * nf;
-    const double ds = m.dsgn;
     // the class's nodes (near-equal values replaced by their mean) and the
     // corners they belong to, for the series; kAll's closed form is exact
     zd n2 = G40, n3 = G41;
@@ -512,77 +562,75 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
       z2 = y32;
       z3 = y33;
     }
-    if (kind != kAll) {
-      // frequencies where the series converges; the closed forms take the rest
-      for (int f = 0; f < nfb; ++f) {
-        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
-        zd w0, w1, w2, w3;
-        if (series_w(m1, m1, n2, n3, e, w0, w1, w2, w3)) {
-          lds_add(y30 + f, w0 * ds);
-          lds_add(y31 + f, w1 * ds);
-          lds_add(z2 + f, w2 * ds);
-          lds_add(z3 + f, w3 * ds);
+    // series_w's nodes, mean and offsets for dsgn = +1; the series where the
+    // ratio is below kSeriesR (both signs at once where both are)
+    const zd sc = (m1 + m1 + n2 + n3) * 0.25;
+    const double sd0 = m1.x - sc.x, sd2 = n2.x - sc.x, sd3 = n3.x - sc.x;
+    const double sdm = fmax(fabs(sd0), fmax(fabs(sd2), fabs(sd3)));
+    const double lim = kind == kAll ? INFINITY : sdm * sdm * (1.0 / (kSeriesR * kSeriesR));
+#pragma unroll 1
+    for (int f = fa; f < fb; ++f) {
+      const double2 ev = em[f0 + f];
+      const zd e = mkz(ev.x, ev.y);
+      const zd sp = mkz(ev.x + sc.x, ev.y + sc.y), sm = mkz(ev.x - sc.x, ev.y - sc.y);
+      const double ap = sp.x * sp.x + sp.y * sp.y, am = sm.x * sm.x + sm.y * sm.y;
+      if (lim < ap && lim < am) {
+        zd T0, T1, T2, T3;
+        joint_series(sd0, sd0, sd2, sd3, sdm, sp, sm, ap, am, T0, T1, T2, T3);
+        lds_add(y30 + f, T0);
+        lds_add(y31 + f, T1);
+        lds_add(z2 + f, T2);
+        lds_add(z3 + f, T3);
+        continue;
+      }
+#pragma unroll 1
+      for (int sg = 0; sg < 2; ++sg) {
+        const double ds = sg ? -1.0 : 1.0;  // the item's dsgn: its nodes are ds times those above
+        if (kind != kAll) {
+          zd w0, w1, w2, w3;
+          if (series_w(m1 * ds, m1 * ds, n2 * ds, n3 * ds, e, w0, w1, w2, w3)) {
+            lds_add(y30 + f, w0 * ds);
+            lds_add(y31 + f, w1 * ds);
+            lds_add(z2 + f, w2 * ds);
+            lds_add(z3 + f, w3 * ds);
+            continue;
+          }
         }
-      }
-    }
-    // true where the series was taken above (the same test, no arithmetic)
-    auto by_series = [&](zd e) {
-      const zd c = (m1 + m1 + n2 + n3) * 0.25;
-      const zd s = e + c;
-      const double dm = fmax(fmax(fabs(m1.x - c.x), fabs(m1.x - c.x)),
-                             fmax(fabs(n2.x - c.x), fabs(n3.x - c.x)));
-      return kind != kAll && dm < kSeriesR * absz(s);
-    };
-    if (kind == kPair) {
-      const zd q1 = (G30 + G31) / 2.0;
-#pragma unroll 1
-      for (int f = 0; f < nfb; ++f) {
-        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
-        if (by_series(e)) continue;
-        const zd c1 = f_sub2g10(q1, G40, G41, e);
-        const zd c2 = f_sub2g30(q1, G40, G41, e);
-        const zd c3 = f_sub1g30(q1, G41, G40, e);
-        lds_add(y30 + f, c1 * ds);
-        lds_add(y31 + f, c1 * ds);
-        lds_add(y40 + f, c2 * ds);
-        lds_add(y41 + f, (c3 - c2 - c1 - c1) * ds);
-      }
-    } else if (kind == kTriple) {
-      const zd q1 = (G30 + G31 + G32) / 3.0;
-#pragma unroll 1
-      for (int f = 0; f < nfb; ++f) {
-        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
-        if (by_series(e)) continue;
-        const zd c1 = f_sub2g30g10(q1, G40, e);
-        const zd c2 = f_sub1g30g20(q1, G40, e);
-        lds_add(y30 + f, c1 * ds);
-        lds_add(y31 + f, c1 * ds);
-        lds_add(y32 + f, c1 * ds);
-        lds_add(y40 + f, (c2 - c1 - c1 - c1) * ds);
-      }
-    } else if (kind == kTwoPairs) {
-      const zd q1 = (G30 + G31) / 2.0, q2 = (G32 + G33) / 2.0;
-#pragma unroll 1
-      for (int f = 0; f < nfb; ++f) {
-        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
-        if (by_series(e)) continue;
-        const zd c1 = f_sub2g30g20(q1, q2, e);
-        const zd c2 = f_sub1g30g21(q1, q2, e);
-        lds_add(y30 + f, c1 * ds);
-        lds_add(y31 + f, c1 * ds);
-        lds_add(y32 + f, (c2 - c1 - c1) * ds * 0.5);
-        lds_add(y33 + f, (c2 - c1 - c1) * ds * 0.5);
-      }
-    } else {  // kAll
-      const zd q1 = (G30 + G31 + G32 + G33) / 4.0;
-#pragma unroll 1
-      for (int f = 0; f < nfb; ++f) {
-        const zd e = mkz(em[f0 + f].x, em[f0 + f].y);
-        const zd c1 = f_sub2simple(q1, e);
-        lds_add(y30 + f, c1 * ds);
-        lds_add(y31 + f, c1 * ds);
-        lds_add(y32 + f, c1 * ds);
-        lds_add(y33 + f, c1 * ds);
+        const zd H30 = G30 * ds, H31 = G31 * ds, H32 = G32 * ds, H33 = G33 * ds;
+        const zd H40 = G40 * ds, H41 = G41 * ds;
+        if (kind == kPair) {
+          const zd q1 = (H30 + H31) / 2.0;
+          const zd c1 = f_sub2g10(q1, H40, H41, e);
+          const zd c2 = f_sub2g30(q1, H40, H41, e);
+          const zd c3 = f_sub1g30(q1, H41, H40, e);
+          lds_add(y30 + f, c1 * ds);
+          lds_add(y31 + f, c1 * ds);
+          lds_add(y40 + f, c2 * ds);
+          lds_add(y41 + f, (c3 - c2 - c1 - c1) * ds);
+        } else if (kind == kTriple) {
+          const zd q1 = (H30 + H31 + H32) / 3.0;
+          const zd c1 = f_sub2g30g10(q1, H40, e);
+          const zd c2 = f_sub1g30g20(q1, H40, e);
+          lds_add(y30 + f, c1 * ds);
+          lds_add(y31 + f, c1 * ds);
+          lds_add(y32 + f, c1 * ds);
+          lds_add(y40 + f, (c2 - c1 - c1 - c1) * ds);
+        } else if (kind == kTwoPairs) {
+          const zd q1 = (H30 + H31) / 2.0, q2 = (H32 + H33) / 2.0;
+          const zd c1 = f_sub2g30g20(q1, q2, e);
+          const zd c2 = f_sub1g30g21(q1, q2, e);
+          lds_add(y30 + f, c1 * ds);
+          lds_add(y31 + f, c1 * ds);
+          lds_add(y32 + f, (c2 - c1 - c1) * ds * 0.5);
+          lds_add(y33 + f, (c2 - c1 - c1) * ds * 0.5);
+        } else {  // kAll
+          const zd q1 = (H30 + H31 + H32 + H33) / 4.0;
+          const zd c1 = f_sub2simple(q1, e);
+          lds_add(y30 + f, c1 * ds);
+          lds_add(y31 + f, c1 * ds);
+          lds_add(y32 + f, c1 * ds);
+          lds_add(y33 + f, c1 * ds);
+        }
       }
     }
   }
@@ -1042,15 +1090,19 @@ __global__ void k_drude_finish(const double2* __restrict__ part, int nb, double 
   wd[d] = sqrt(8.0 * re / volume / (double)ntk);
 }
 
+// Closed-form queue of the sweep: a quarter of the (tetrahedron, frequency)
+// pairs of a workgroup (the rest, rarely needed, are evaluated inline).
+int tetra_queue_len(int nk3, int nf) { return (6 * nk3 * nf) / 4 + 64; }
+
 size_t tetra_lds_bytes(int nk3, int nf) {
   return 3 * sizeof(double2) * (size_t)nk3 * nf + 2 * sizeof(double) * (size_t)nk3 +
-         sizeof(int2) * 6 * (size_t)nk3;
+         sizeof(int2) * 6 * (size_t)nk3 + sizeof(int) * (size_t)tetra_queue_len(nk3, nf);
 }
 
 // Frequencies per workgroup: as many as keep two workgroups per CU (the
 // register budget allows two waves per SIMD), evened out over the slices.
 int tetra_freqs_per_block(int nk3, int nen) {
-  const size_t fixed = tetra_lds_bytes(nk3, 0), per = 3 * sizeof(double2) * (size_t)nk3;
+  const size_t fixed = tetra_lds_bytes(nk3, 0), per = tetra_lds_bytes(nk3, 1) - fixed;
   const size_t target = 80 * 1024;
   int nf = target > fixed ? (int)((target - fixed) / per) : 1;
   nf = std::max(1, std::min(nf, nen));
@@ -1073,7 +1125,7 @@ void launch_tetra_weights(int ntk, int n1, int n2, int n3, const int* imt, const
   const dim3 grid(npair, (nen + nf - 1) / nf);
   hipLaunchKernelGGL(k_tetra_weights, grid, dim3(256), lds, s, ntk, n1, n2, n3, imt, index_kpt,
                      pocc, pvir, ikq, eband, pband, ef, delt, dmna, dmnr, nen, nf, em, ldd,
-                     cuts ? 1 : 0, status, dout);
+                     cuts ? 1 : 0, tetra_queue_len(nk3, nf), status, dout);
   RP_HIP(hipGetLastError());
   if (!cuts) return;
   // x[nk3][nf], e1, e2, the cut list (<= lds of the uncut kernel)
