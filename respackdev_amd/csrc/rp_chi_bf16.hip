@@ -42,6 +42,7 @@
 #include <hip/hip_bfloat16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "rp_common.h"
 #include "rp_kernels.h"
@@ -128,8 +129,8 @@ __global__ void k_weights_to_f32(const double2* __restrict__ dw, size_t n, float
 // per SIMD).  Item (split z, frequency w, tile) accumulates K chunks
 // [z*chunks_per_split, ...) into out, the FP64 partial [z][w][ng][ng] (element
 // (G,G') at G' * ng + G); every element of it is stored by exactly one item.
-// Columns of A beyond ng are zero, so edge tiles run the full MFMA schedule
-// (no branches between MFMAs).
+// Columns of A beyond ng are zero; in edge tiles the waves whose rows or
+// columns all lie beyond ng skip their MFMAs.
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ df, size_t ldd,
              size_t chunks_per_split, size_t nchunks, int ng, int ntile, int nen, int nsplit,
@@ -139,7 +140,11 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
   constexpr int kBuf = (kTM + kTN) * kRowBytes;
   const int t = threadIdx.x, lane = t & 63;
   const int wave = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform (SGPR)
-  const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves: 64 G x 32 G' each
+  // 2 x 4 waves of 64 G x 32 G' each.  Wave w runs on SIMD w % 4; the two
+  // waves of a SIMD take different row halves and column blocks two apart, so
+  // that in an edge tile (rows 64-127 or columns 64-127 past ng) each SIMD
+  // keeps one wave with MFMAs.
+  const int wm = (wave >> 2) & 1, wn = (wave + 2 * (wave >> 2)) & 3;
   const int li = lane & 15, q = lane >> 4;
   // this XCD's share of the (frequency, tile) pairs, walked split-major
   const int ntiles = ntile * ntile, npairs = ntiles * nen;
@@ -161,6 +166,7 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
   const int z = item / mypairs, u = (item % mypairs) * kXcd + xcd;
   const int w = u / ntiles, tile = u % ntiles;
   const int g0 = (tile % ntile) * kTM, h0 = (tile / ntile) * kTN;
+  const bool active = g0 + wm * 64 < ng && h0 + wn * 32 < ng;  // wave-uniform
   const float2* dww = df + (size_t)w * ldd;
   const size_t c0 = (size_t)z * chunks_per_split;
   size_t c1 = c0 + chunks_per_split;
@@ -243,13 +249,20 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
   }
   __syncthreads();
   int since = 0;
-  for (size_t c = c0; c < c1; ++c) {
+  // The chunk loop, in two forms: a wave whose 64 rows or 32 columns lie
+  // wholly past ng (edge tiles) only stages (its accumulators would stay zero
+  // and its part of the tile is never stored), the others also read their
+  // operands and run the MFMAs.  Both meet the same barriers.
+  auto chunks = [&](auto act) {
+    constexpr bool kAct = decltype(act)::value;
+    for (size_t c = c0; c < c1; ++c) {
     const int cur = (int)((c - c0) & 1);
     const unsigned char* bcur = smem + cur * kBuf;
     // next chunk's operands in flight during the MFMAs (the last chunk loads
     // itself again: no branch, so that loads, MFMAs and staging form one
     // scheduling region)
     load_chunk(c + 1 < c1 ? c + 1 : c);
+    if constexpr (kAct) {
     // lane group q supplies the 8 K of octet q of the chunk (k-slots 8q..8q+7)
     bf8v Brh[2], Brl[2], Bih[2], Bil[2], Lh[4], Ll[4];
 #pragma unroll
@@ -316,25 +329,28 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
         acc_re[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Bih[b], acc_re[a][b], 0, 0, 0);
         acc_in[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ll[a], Brh[b], acc_in[a][b], 0, 0, 0);
       }
-    stage(smem + (cur ^ 1) * kBuf);  // into the idle buffer (unused after the last chunk)
-    // Issue order: the eight global loads, the sixteen operand reads of the
-    // real phase, its 48 MFMAs in pairs, the imaginary phase's reads, then its
-    // 48 MFMAs each followed by up to five VALU instructions of the staging
-    // (measured 2-4% faster than leaving the order to the scheduler).
-    __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);   // global reads
-    __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);  // LDS reads
-#pragma unroll
-    for (int s_ = 0; s_ < 24; ++s_) __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
-    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
-#pragma unroll
-    for (int s_ = 0; s_ < 48; ++s_) {
-      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-      __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // VALU
     }
-    __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);  // LDS writes
-    if (++since == flush_chunks) {
-      since = 0;
-      flush();
+    stage(smem + (cur ^ 1) * kBuf);  // into the idle buffer (unused after the last chunk)
+    if constexpr (kAct) {
+      // Issue order: the eight global loads, the sixteen operand reads of the
+      // real phase, its 48 MFMAs in pairs, the imaginary phase's reads, then
+      // its 48 MFMAs each followed by up to five VALU instructions of the
+      // staging (measured 2-4% faster than leaving the order to the scheduler).
+      __builtin_amdgcn_sched_group_barrier(0x020, 8, 0);   // global reads
+      __builtin_amdgcn_sched_group_barrier(0x100, 16, 0);  // LDS reads
+#pragma unroll
+      for (int s_ = 0; s_ < 24; ++s_) __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+      for (int s_ = 0; s_ < 48; ++s_) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);  // VALU
+      }
+      __builtin_amdgcn_sched_group_barrier(0x200, 8, 0);  // LDS writes
+      if (++since == flush_chunks) {
+        since = 0;
+        flush();
+      }
     }
     if (t == 0) {
       if (waiting) {
@@ -354,8 +370,11 @@ k_chi_bf16x3(const u4v* __restrict__ Ao, size_t lda, const float2* __restrict__ 
       }
     }
     __syncthreads();
-  }
-  if (since || first) flush();
+    }
+  };
+  if (active) chunks(std::true_type{});
+  else chunks(std::false_type{});
+  if (active && (since || first)) flush();
   }  // item
   // A workgroup out of items releases every later rendezvous of its XCD (its
   // peers would otherwise wait out the spin bound at each one); 2^22 exceeds
