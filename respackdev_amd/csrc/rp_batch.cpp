@@ -71,6 +71,13 @@ void swap_scratch(rp_ctx* h, Scratch& s) {
 
 namespace {
 
+// The handle's two streams exchanged for a scope (restored however it ends).
+struct OnStream2 {
+  rp_ctx* h;
+  explicit OnStream2(rp_ctx* h_) : h(h_) { std::swap(h->stream, h->stream2); }
+  ~OnStream2() { std::swap(h->stream, h->stream2); }
+};
+
 // Restores the handle's own per-q state and work buffers however the plan
 // ends, and ends a capture left open by an error.
 struct BatchScope {
@@ -187,18 +194,35 @@ int rp_chiqw_batch_plan(rp_handle h, int32_t nq, const double* qs, const double*
         }
         scope.leave();
       }
-      // 3. capture every q's launches into one graph
+      // 3. capture every q's launches into one graph.  The pair densities,
+      //    weights and contraction operands of q go on the second stream, after
+      //    q-1's contraction (the last reader of their buffers), so that they
+      //    run beside q-1's Dyson step, which leaves most compute units idle;
+      //    the contraction of q waits for them.
       h->capturing = true;
       RP_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeRelaxed));
       launch_zero(h->d_status.p, sizeof(int), h->stream);
+      RP_HIP(hipEventRecord(h->bev[0], h->stream));
+      RP_HIP(hipStreamWaitEvent(h->stream2, h->bev[0], 0));
       for (int i = 0; i < nq; ++i) {
         scope.enter(i);
-        issue_pairdens(h);
-        issue_weights(h);
-        issue_accumulate(h, precision, b->slots[i].launch, false);
+        {
+          OnStream2 on2(h);  // the stages issue on h->stream
+          issue_pairdens(h);
+          issue_weights(h);
+          issue_operands(h, precision);
+          RP_HIP(hipEventRecord(h->bev[1], h->stream));
+        }
+        RP_HIP(hipStreamWaitEvent(h->stream, h->bev[1], 0));
+        issue_accumulate(h, precision, b->slots[i].launch, false, true);
+        RP_HIP(hipEventRecord(h->bev[2], h->stream));
+        RP_HIP(hipStreamWaitEvent(h->stream2, h->bev[2], 0));
         issue_dyson(h);
         scope.leave();
       }
+      // the second stream joins back before the capture ends
+      RP_HIP(hipEventRecord(h->bev[1], h->stream2));
+      RP_HIP(hipStreamWaitEvent(h->stream, h->bev[1], 0));
       hipGraph_t g = nullptr;
       h->capturing = false;
       RP_HIP(hipStreamEndCapture(h->stream, &g));

@@ -40,8 +40,10 @@ int rp_init(int32_t device, rp_handle* out) {
     h->device = device;
     h->ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     RP_HIP(hipStreamCreate(&h->stream));
+    RP_HIP(hipStreamCreate(&h->stream2));
     for (auto& e : h->ev) RP_HIP(hipEventCreate(&e));
     for (auto& e : h->gev) RP_HIP(hipEventCreate(&e));
+    for (auto& e : h->bev) RP_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     h->d_status.reserve(1);
     *out = h;
   });
@@ -608,21 +610,30 @@ ChiLaunch chi_launch_heuristic(const rp_ctx* h, int prec) {
   return c;
 }
 
-void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events) {
+// The split-BF16 contraction's operands of direction 0: the single-precision
+// weights and the presplit octets of the first (or only) pair-density set.
+void issue_operands(rp_ctx* h, int prec) {
+  if (prec != RP_PREC_BF16X3) return;
+  hipStream_t s = h->stream;
+  grow(h, h->d_Ao, h->kpad * h->lda * 8, "split operand buffer");
+  grow(h, h->d_wf, (size_t)h->nen * h->kpad, "single-precision weight buffer");
+  launch_weights_to_f32(h->d_w.p, (size_t)h->nen * h->kpad, h->d_wf.p, s);
+  launch_presplit(h->q0 ? h->d_A3.p : h->d_A.p, h->lda, h->kpad, h->d_Ao.p, s);
+}
+
+void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events, bool operands_ready) {
   hipStream_t s = h->stream;
   const int ng = h->ng, nen = h->nen, nc = h->q0 ? 3 : 1;
   const size_t per = (size_t)ng * ng * nen;
   grow(h, h->d_chi, (size_t)nc * per, "chi buffer");
   grow(h, h->d_part, (size_t)c.nsplit * per, "partial-sum buffer");
   if (prec == RP_PREC_BF16X3) {
-    grow(h, h->d_Ao, h->kpad * h->lda * 8, "split operand buffer");
-    grow(h, h->d_wf, (size_t)nen * h->kpad, "single-precision weight buffer");
     grow(h, h->d_sync, bf16x3_sync_words(), "rendezvous counters");
-    launch_weights_to_f32(h->d_w.p, (size_t)nen * h->kpad, h->d_wf.p, s);
+    if (!operands_ready) issue_operands(h, prec);
   }
   for (int ix = 0; ix < nc; ++ix) {
     const float2* A = h->q0 ? h->d_A3.p + (size_t)ix * h->kpad * h->lda : h->d_A.p;
-    if (prec == RP_PREC_BF16X3) launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
+    if (prec == RP_PREC_BF16X3 && ix > 0) launch_presplit(A, h->lda, h->kpad, h->d_Ao.p, s);
     if (events) RP_HIP(hipEventRecord(h->gev[2 * ix], s));
     if (prec == RP_PREC_BF16X3)
       launch_chi_bf16x3(h->d_Ao.p, h->lda, h->d_wf.p, h->kpad, h->kpad, ng, nen, c.nsplit, c.nwg,

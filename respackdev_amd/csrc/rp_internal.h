@@ -124,8 +124,10 @@ struct rp_ctx {
   int device = 0;
   int ncu = 256;  // compute units of the device
   hipStream_t stream = nullptr;
+  hipStream_t stream2 = nullptr;  // a batch graph's pair densities and weights (default priority)
   hipEvent_t ev[6] = {};
   hipEvent_t gev[6] = {};  // around each main contraction launch (<= 3 directions)
+  hipEvent_t bev[3] = {};  // the batch graph's fork and joins between the two streams
   double gemm_seconds = 0;
   int gemm_launches = 0;
   // system
@@ -188,7 +190,10 @@ struct rp_ctx {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : gev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : bev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (stream2) (void)hipStreamDestroy(stream2);
   }
   int nbatch() const { return q0 ? 3 * nen : nen; }
 };
@@ -203,7 +208,9 @@ void throw_status(int st, const std::string& what);
 void noalloc_init(int device);
 void issue_pairdens(rp_ctx* h);
 void issue_weights(rp_ctx* h);
-void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events);
+void issue_operands(rp_ctx* h, int prec);
+void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events,
+                      bool operands_ready = false);
 void issue_dyson(rp_ctx* h);
 ChiLaunch chi_launch_heuristic(const rp_ctx* h, int prec);
 // The launch for the current q: the tuned one for its shape, tuning it now
