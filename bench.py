@@ -26,6 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "chi(q,w) s per q-point: mixed/batched vs plain FP64 GPU at <=1e-5 rel err"
+ERR_BAR = 1e-5  # chi and eps^-1 relative Frobenius error of the mixed path against FP64
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}  # dense MFMA, MI355X
 KERNEL = {"fp64": "k_chi_gemm<0>", "fp32": "k_chi_gemm<1>", "bf16x3": "k_chi_bf16x3"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
@@ -214,6 +215,10 @@ def main() -> int:
         out["speedup_vs_fp64"] = results["fp64"]["wall"] / res["wall"]
         out["chi_rel_err_vs_fp64"] = rel(res["chi"], results["fp64"]["chi"])
         out["epsinv_rel_err_vs_fp64"] = rel(res["eps"], results["fp64"]["eps"])
+        out["within_error_bar"] = bool(max(out["chi_rel_err_vs_fp64"],
+                                           out["epsinv_rel_err_vs_fp64"]) <= ERR_BAR)
+        if not out["within_error_bar"]:
+            log(f"WARNING: mixed-precision error above the {ERR_BAR:g} bar")
     out["stage_s"] = {p: results[p]["stages"] for p in results}
     # dominant kernel: the main contraction launch, timed with HIP events on the
     # library's stream around each launch (one launch per q away from Gamma)
@@ -367,6 +372,7 @@ def c3_bench(args) -> int:
                         for p, ts in (("fp64", [t for t in tune if t["precision"] == 0]),
                                       (m, [t for t in tune if t["precision"] != 0]))},
            "chi_rel_err_vs_fp64_max": err_chi, "epsinv_rel_err_vs_fp64_max": err_eps,
+           "within_error_bar": bool(max(err_chi, err_eps) <= ERR_BAR),
            "eager_stage_s_per_q": {p: {k_: v_ / nq for k_, v_ in res[p]["stages"].items()}
                                    for p in res},
            "cpu_baseline": cpu,
