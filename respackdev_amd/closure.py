@@ -159,20 +159,25 @@ def truncation_energy(s, nvir: int) -> float:
     return float(np.max(s.eig[:, s.nocc + nvir - 1]) - s.ef)
 
 
-def training_samples(s, ecut_eps: float, nen: int, n_trunc: int, n_full: int, device: int = 0):
-    """(inputs, targets) of one system: every irreducible q, every direction and
-    frequency, the near-diagonal entries of chi at n_trunc against n_full
-    empty bands (FP64 chiqw on the GPU)."""
+def training_samples(s, ecut_eps: float, nen: int, n_trunc: int, n_full: int, device: int = 0,
+                     n_q: int | None = None, seed: int = 0):
+    """(inputs, targets) of one system: every irreducible q (or n_q of them drawn
+    with `seed`), every direction and frequency, the near-diagonal entries of chi
+    at n_trunc against n_full empty bands (FP64 chiqw on the GPU)."""
     from .chiqw import Chiqw, from_system
     from . import gen
     w = from_system(s)
     e_cut = truncation_energy(s, n_trunc)
+    ks = list(w.ski)
+    if n_q is not None and n_q < len(ks):
+        pick = np.random.default_rng(seed).choice(len(ks), n_q, replace=False)
+        ks = [ks[i] for i in sorted(pick)]
     xs, ts = [], []
     c = Chiqw(device)
     try:
         info = c.open(w, ecut_for_eps=ecut_eps, num_freq_grid=nen)
         em, vol = np.asarray(info["em"]), float(info["volume"])
-        for k in w.ski:
+        for k in ks:
             q = tuple(gen._fold(v) for v in k)
             c.set_window(0.0, n_full)
             full = c.run_q(q, "fp64", want_chi=True, want_eps=False)
@@ -256,17 +261,20 @@ def w_matrix(s, sqi, em, eps_list, ecut_eps: float, nwf: int, wan_seed: int, dev
 def run(n_train: int = 16, n_test: int = 4, nk: int = 3, nocc: int = 8, n_trunc: int = 50,
         n_full: int = 400, plain=(50, 100, 200), ecut_psi: float = 24.0, ecut_eps: float = 8.0,
         nen: int = 16, steps: int = 3000, batch: int = 4096, lr: float = 1e-3, nwf: int = 5,
-        seed0: int = 5000, device: int = 0, log=print) -> dict:
+        seed0: int = 5000, device: int = 0, log=print, q_per_system: int | None = None,
+        test_nen: int | None = None) -> dict:
     """Trains the closure on n_train family systems and reports, on n_test
     held-out ones, the error of the static W_ij (w = 0, every i, j and R; meV,
     against the full band set) of plain truncation at each count in `plain`
-    and of n_trunc + closure."""
+    and of n_trunc + closure.  q_per_system: training q drawn per system (all
+    irreducible q when None); test_nen: frequency points of the held-out runs
+    (w = 0 is the first point of every grid, so 1 gives the static W alone)."""
     import time
     t0 = time.perf_counter()
     xs, ts = [], []
     for i in range(n_train):
         s, a, gap = family_system(i, nk, nocc, n_full, ecut_psi, seed0)
-        x, t = training_samples(s, ecut_eps, nen, n_trunc, n_full, device)
+        x, t = training_samples(s, ecut_eps, nen, n_trunc, n_full, device, q_per_system, seed0 + i)
         xs.append(x)
         ts.append(t)
         log(f"train system {i}: a={a:.2f} gap={gap:.2f} eV, {len(x)} samples")
@@ -284,7 +292,8 @@ def run(n_train: int = 16, n_test: int = 4, nk: int = 3, nocc: int = 8, n_trunc:
         errs, wscale = {}, []
         for j in range(n_test):
             s, a, gap = family_system(n_train + j, nk, nocc, n_full, ecut_psi, seed0)
-            sqi, em, eps = eps_variants(s, ecut_eps, nen, n_trunc, n_full, plain, net, sc, device)
+            sqi, em, eps = eps_variants(s, ecut_eps, test_nen or nen, n_trunc, n_full, plain, net,
+                                        sc, device)
             # static W (w = 0, the calc_ifreq = 1 default of calc_int), every i, j and R
             wfull = w_matrix(s, sqi, em, eps["full"], ecut_eps, nwf, seed0 + j, device)[0]
             wscale.append(float(np.abs(wfull).max() * AU_EV))
@@ -297,7 +306,8 @@ def run(n_train: int = 16, n_test: int = 4, nk: int = 3, nocc: int = 8, n_trunc:
                 ", ".join(f"{k} {v[-1]:.2f} meV" for k, v in errs.items()))
     finally:
         net.close()
-    return {"samples": int(len(x)), "steps": steps, "loss_first": float(losses[:50].mean()),
+    return {"samples": int(len(x)), "steps": steps, "train_systems": n_train,
+            "test_systems": n_test, "loss_first": float(losses[:50].mean()),
             "loss_last": float(losses[-50:].mean()), "data_s": t_data, "train_s": t_train,
             "w_static_max_ev": float(np.mean(wscale)),
             "w_err_mev": {k: float(np.mean(v)) for k, v in errs.items()},
