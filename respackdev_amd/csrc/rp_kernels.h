@@ -121,8 +121,8 @@ void launch_w_func(const double2* rho, const long long* rho_off, const int4* qin
 void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int nwf, int nen, double2* funcw,
-                       hipStream_t s);
+                       const double2* shell, int ntq, int nwf, int nen, int max_nge,
+                       double2* funcw, hipStream_t s);
 // xg: scratch of j_contract_scratch(sum over q of NG_eps, npair) complex values
 void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,

@@ -606,7 +606,8 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     launch_w_func(w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.ntq, nwf, w.d_func.p, w.d_shell.p, s);
     RP_HIP(hipEventRecord(w.ev[2], s));
     launch_w_contract(w.d_eps.p, w.d_eps_off.p, w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.d_minv.p,
-                      w.d_pf1.p, w.d_pf2.p, w.d_map_off.p, w.d_shell.p, w.ntq, nwf, nen, w.d_funcw.p, s);
+                      w.d_pf1.p, w.d_pf2.p, w.d_map_off.p, w.d_shell.p, w.ntq, nwf, nen,
+                      *std::max_element(w.nge.begin(), w.nge.end()), w.d_funcw.p, s);
     RP_HIP(hipEventRecord(w.ev[3], s));
     if (func_out)
       RP_HIP(hipMemcpyAsync(func_out, w.d_func.p, sizeof(double2) * w.ntq * nwf * nwf,

@@ -67,10 +67,12 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // The vectors are the VB = nv <= 8 Wannier functions of calc_w3d and every
 // (i,j) is formed, out [q][w][j][i] (calc_j3d's diagonal pairs: k_j_contract).
 // A workgroup takes all the vectors of its (q, w), so E is read once per
-// (q, w): 8 B of HBM per 8 VB flops.  512 threads, rows a and a + 512 of E per thread (NG_eps
-// <= 1024); the loads of the next PD columns of E are in flight during a
+// (q, w): 8 B of HBM per 8 VB flops.  nt threads (a multiple of 64, at most 512,
+// at least half the largest NG_eps), rows a and a + nt of E per thread (NG_eps
+// <= 1024): every wave holds two rows per lane, so no wave waits on the others'
+// second row; the loads of the next PD columns of E are in flight during a
 // column's arithmetic.
-// grid: (nen, ntq).
+// grid: (nen, ntq), nt threads.
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
 constexpr int kCT = 512;       // threads of the contraction
 constexpr int kTB = 32;        // columns of E staged per pass
@@ -83,7 +85,8 @@ __global__ void __launch_bounds__(512) k_w_contract(
     const long long* __restrict__ map_off, const double2* __restrict__ shell, int nv, int nen,
     double2* __restrict__ funcw) {
   constexpr int kPairs = VB * VB;
-  constexpr int kWaves = kCT / 64;
+  constexpr int kWaves = kCT / 64;  // at most
+  const int nt = (int)blockDim.x, nwv = nt >> 6;
   const int w = blockIdx.x, q = blockIdx.y;
   const int4 qi = qinfo[q];
   const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
@@ -96,7 +99,7 @@ __global__ void __launch_bounds__(512) k_w_contract(
   __shared__ double2 X[VB][kTB];
   __shared__ double red[kPairs][kWaves][2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int a0 = t, a1 = t + kCT;
+  const int a0 = t, a1 = t + nt;
   const bool v0 = a0 < nge, v1 = a1 < nge;
   double y0r[VB], y0i[VB], y1r[VB], y1i[VB];
 #pragma unroll
@@ -104,7 +107,7 @@ __global__ void __launch_bounds__(512) k_w_contract(
   for (int b0 = 0; b0 < nge; b0 += kTB) {
     const int nb = min(kTB, nge - b0);
     __syncthreads();
-    for (int e = t; e < VB * kTB; e += kCT) {
+    for (int e = t; e < VB * kTB; e += nt) {
       const int c = e / kTB, bb = e % kTB;
       double2 v = make_double2(0.0, 0.0);  // vectors beyond nvb stay zero
       if (bb < nb && c < nvb) {
@@ -189,13 +192,13 @@ __global__ void __launch_bounds__(512) k_w_contract(
   }
   __syncthreads();
   const int npairs = nvb * nvb;
-  for (int e = t; e < 2 * npairs; e += kCT) {
+  for (int e = t; e < 2 * npairs; e += nt) {
     const int pr = e >> 1, part = e & 1;
     const int i = pr % nvb, j = pr / nvb;
     const int slot = i + VB * j;
     double sum = 0.0;
 #pragma unroll
-    for (int wv = 0; wv < kWaves; ++wv) sum += red[slot][wv][part];
+    for (int wv = 0; wv < nwv; ++wv) sum += red[slot][wv][part];
     const double2 sh = shell[((size_t)q * nv + j) * nv + i];
     double2* o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
     if (part == 0) o->x = sum + sh.x;
@@ -485,14 +488,16 @@ void launch_w_func(const double2* rho, const long long* rho_off, const int4* qin
 void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int nwf, int nen, double2* funcw,
-                       hipStream_t s) {
+                       const double2* shell, int ntq, int nwf, int nen, int max_nge,
+                       double2* funcw, hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
+  RP_CHECK(max_nge >= 1 && max_nge <= 2 * kCT, "W contraction: eps sphere too large");
+  const int nt = std::max(64, (max_nge + 127) / 128 * 64);  // two rows per thread
   // as many accumulators as Wannier functions
   switch (nwf) {
 #define RP_WC(NV)                                                                              \
   case NV:                                                                                     \
-    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ntq), dim3(kCT), 0, s, eps,        \
+    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ntq), dim3(nt), 0, s, eps,         \
                        eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, \
                        funcw);                                                                 \
     break;
