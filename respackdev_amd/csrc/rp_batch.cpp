@@ -71,13 +71,6 @@ void swap_scratch(rp_ctx* h, Scratch& s) {
 
 namespace {
 
-// The handle's two streams exchanged for a scope (restored however it ends).
-struct OnStream2 {
-  rp_ctx* h;
-  explicit OnStream2(rp_ctx* h_) : h(h_) { std::swap(h->stream, h->stream2); }
-  ~OnStream2() { std::swap(h->stream, h->stream2); }
-};
-
 // Restores the handle's own per-q state and work buffers however the plan
 // ends, and ends a capture left open by an error.
 struct BatchScope {

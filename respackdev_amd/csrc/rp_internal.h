@@ -6,6 +6,7 @@
 
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "rp_common.h"
@@ -208,6 +209,13 @@ void throw_status(int st, const std::string& what);
 void noalloc_init(int device);
 void issue_pairdens(rp_ctx* h);
 void issue_weights(rp_ctx* h);
+// The handle's two streams exchanged for a scope (restored however it ends):
+// the stage bodies issue on h->stream.
+struct OnStream2 {
+  rp_ctx* h;
+  explicit OnStream2(rp_ctx* h_) : h(h_) { std::swap(h->stream, h->stream2); }
+  ~OnStream2() { std::swap(h->stream, h->stream2); }
+};
 void issue_operands(rp_ctx* h, int prec);
 void issue_accumulate(rp_ctx* h, int prec, const ChiLaunch& c, bool events,
                       bool operands_ready = false);
