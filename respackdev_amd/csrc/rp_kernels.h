@@ -38,10 +38,23 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
                                   const double2* twtab, float2* A, hipStream_t s);
 // calc_int: P[q][p][r] = sum_k wk[k] conj(u_iw,k+q(r) e^{-i 2pi r.dG/N}) u_jw,k(r)
 // (FP64) for the q of ikq[nq][ntk] / shift[nq][ntk][3]; full: every pair
-// p = iw + nwf*jw, else the diagonal p = w.  wk may be null (1).
+// p = iw + nwf*jw, else the diagonal p = w.  wk may be null (1).  kqs (may be
+// null) packs the same per (q, k): pair_rsum_pack.
 void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int ny, int nz,
-                      const int* ikq, const int* shift, const double2* wk, int nq, double2* P,
-                      hipStream_t s);
+                      const int* ikq, const int* shift, const unsigned* kqs, const double2* wk,
+                      int nq, double2* P, hipStream_t s);
+// k+q index (< 2^20) and umklapp vector (components -8..7) in one word, or
+// false when they do not fit
+inline bool pair_rsum_pack(int ikq, const int* d, unsigned* out) {
+  if (ikq < 0 || ikq >= (1 << 20)) return false;
+  unsigned v = (unsigned)ikq;
+  for (int c = 0; c < 3; ++c) {
+    if (d[c] < -8 || d[c] > 7) return false;
+    v |= (unsigned)(d[c] + 8) << (20 + 4 * c);
+  }
+  *out = v;
+  return true;
+}
 // rho[rho_off[q] + p*ngpsi[q] + i] from P of q = q_offset .. q_offset+nq-1:
 // pruned forward transform read at -G of the psi sphere (plans at plan_off),
 // / |q+G| (qg at qg_off), / NTK, zero at g0s[q].
@@ -118,11 +131,19 @@ int w3d_max_wannier();
 int w3d_max_eps_g();
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s);
+// q per group of launch_w_contract (q sharing one irreducible eps^-1), and
+// the vectors of a group (nwf per q)
+int w_contract_members(int nwf);
+int w_contract_vectors(int nwf);
+// groups g = 0 .. ngroups-1 of the q gmem[goff[g] .. goff[g+1]), each sharing
+// one irreducible eps^-1 and at most w_contract_members(nwf) long; xw: scratch
+// of w_contract_vectors(nwf) * (sum over q of NG_eps) complex values
 void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int nwf, int nen, int max_nge,
-                       double2* funcw, hipStream_t s);
+                       const double2* shell, const int* goff, const int* gmem, int ngroups,
+                       int nwf, int nen, int max_nge, double2* xw, double2* funcw,
+                       hipStream_t s);
 // xg: scratch of j_contract_scratch(sum over q of NG_eps, npair) complex values
 void launch_j_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,

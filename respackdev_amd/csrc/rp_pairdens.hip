@@ -20,6 +20,7 @@
 // over HBM (k_line_dft).
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "rp_common.h"
@@ -570,6 +571,151 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
   }
 }
 
+// The same sums with the band grids of a tile of TR grid points (every
+// Wannier function, every k) held in LDS: each grid value is then read from
+// HBM once per tile and workgroup instead of once per q.  Arithmetic and
+// summation order per (q, pair, r) are those of k_pair_rsum, except that the
+// full mode multiplies the umklapp phase into the jw factor (the same product,
+// rounded differently: about 1e-16 relative).
+// grid (ceil(nr/TR), ngroups, nqsplit), 512 threads; thread (rl = t % TR,
+// ql = t / TR) takes the point r0 + rl and the q = qbeg + ql, qbeg + ql + QL, ...
+// of its workgroup's range of qper q.  Dynamic LDS: the phase tables, w_k
+// (ntk) and the grids [w][k][TR] (single precision, as in HBM).
+constexpr int kRsumTileThreads = 512;
+constexpr int kRsumTileJB = 3;  // jw per group in the full mode
+__device__ __forceinline__ int rsum_wrap(int m, int n) {
+  if (m >= n || m < -n) m %= n;  // only for umklapp vectors beyond +-1
+  return m < 0 ? m + n : m;
+}
+constexpr int kRsumPF = 8;  // k of packed indices loaded ahead
+template <bool FULL, int TR>
+__global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
+    const float2* __restrict__ ur, int ntk, int nwf, int nx, int ny, int nz,
+    const unsigned* __restrict__ kqs, const double2* __restrict__ wk, int nq, int qper,
+    double2* __restrict__ P) {
+  extern __shared__ double2 rs_lds[];
+  double2* tab = rs_lds;               // 3 * kTwMax: e^{-i 2pi m/n} per axis
+  double2* swk = tab + 3 * kTwMax;     // ntk
+  float2* su = (float2*)(swk + ntk);   // [w][k][TR]
+  constexpr int QL = kRsumTileThreads / TR;
+  const double tpi = 2.0 * 3.141592653589793238462643383279502884;
+  const int t = threadIdx.x;
+  for (int i = t; i < nx + ny + nz; i += kRsumTileThreads) {
+    const int ax = i < nx ? 0 : (i < nx + ny ? 1 : 2);
+    const int m = ax == 0 ? i : (ax == 1 ? i - nx : i - nx - ny);
+    const int n = ax == 0 ? nx : (ax == 1 ? ny : nz);
+    double sn, cs;
+    sincos(tpi * ((double)m / (double)n), &sn, &cs);
+    tab[(ax == 0 ? 0 : (ax == 1 ? nx : nx + ny)) + m] = make_double2(cs, -sn);
+  }
+  for (int k = t; k < ntk; k += kRsumTileThreads) swk[k] = wk ? wk[k] : make_double2(1.0, 0.0);
+  const int nr = nx * ny * nz;
+  const int r0 = blockIdx.x * TR;
+  for (int e = t; e < nwf * ntk * TR; e += kRsumTileThreads) {
+    const int row = e / TR, rr = r0 + e % TR;  // row = w * ntk + k
+    su[e] = rr < nr ? ur[(size_t)row * nr + rr] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const int rl = t % TR, ql = t / TR, z = blockIdx.y;
+  const int r = r0 + rl;
+  if (r >= nr) return;  // after the only barrier
+  const int x = r % nx, y = (r / nx) % ny, zz = r / (nx * ny);
+  const int npair = FULL ? nwf * nwf : nwf;
+  constexpr int NA = FULL ? kRsumTileJB : 8;
+  const int qend = min(nq, (int)(blockIdx.z + 1) * qper);
+  for (int q = blockIdx.z * qper + ql; q < qend; q += QL) {
+    double2 acc[NA][FULL ? 8 : 1];
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+      for (int b = 0; b < (FULL ? 8 : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
+    // packed (k+q, umklapp) of every k, kRsumPF k ahead of their use
+    const unsigned* kq_row = kqs + (size_t)q * ntk;
+    unsigned nxt[kRsumPF];
+#pragma unroll
+    for (int j = 0; j < kRsumPF; ++j) nxt[j] = j < ntk ? kq_row[j] : 0u;
+    for (int k0 = 0; k0 < ntk; k0 += kRsumPF) {
+      unsigned cur[kRsumPF];
+#pragma unroll
+      for (int j = 0; j < kRsumPF; ++j) {
+        cur[j] = nxt[j];
+        nxt[j] = k0 + kRsumPF + j < ntk ? kq_row[k0 + kRsumPF + j] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < kRsumPF; ++j) {
+        const int k = k0 + j;
+        if (k >= ntk) break;  // uniform
+        const int kq = (int)(cur[j] & 0xfffffu);
+        const int d1 = (int)((cur[j] >> 20) & 15u) - 8, d2 = (int)((cur[j] >> 24) & 15u) - 8,
+                  d3 = (int)(cur[j] >> 28) - 8;
+        double2 ph = make_double2(1.0, 0.0);
+        if (d1 != 0 || d2 != 0 || d3 != 0) {
+          const double2 px = tab[rsum_wrap(d1 * x, nx)];
+          const double2 py = tab[nx + rsum_wrap(d2 * y, ny)];
+          const double2 pz = tab[nx + ny + rsum_wrap(d3 * zz, nz)];
+          const double2 tt = make_double2(px.x * py.x - px.y * py.y, px.x * py.y + px.y * py.x);
+          ph = make_double2(tt.x * pz.x - tt.y * pz.y, tt.x * pz.y + tt.y * pz.x);
+        }
+        const double2 wv = swk[k];
+        if constexpr (FULL) {
+          // conj(u_iw,k+q ph) (w_k u_jw,k) = conj(u_iw,k+q) (conj(ph) w_k u_jw,k): the
+          // phase goes with the jw factor, once per jw instead of once per iw
+          const double2 cw = make_double2(ph.x * wv.x + ph.y * wv.y, ph.x * wv.y - ph.y * wv.x);
+          double2 bq[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            bq[i] = make_double2(0.0, 0.0);
+            if (i < nwf) {
+              const float2 v = su[(i * ntk + kq) * TR + rl];
+              bq[i] = make_double2((double)v.x, (double)v.y);
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            const int jw = z * kRsumTileJB + a;
+            if (jw >= nwf) continue;
+            const float2 v = su[(jw * ntk + k) * TR + rl];
+            const double2 aw = make_double2((double)v.x * cw.x - (double)v.y * cw.y,
+                                            (double)v.x * cw.y + (double)v.y * cw.x);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+              acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
+              acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
+            }
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < NA; ++a) {
+            if (a >= nwf) continue;
+            // b = u_w,k+q(r) e^{-i 2pi r.dG/N}, conj(b) (w_k u_w,k)
+            const float2 u = su[(a * ntk + kq) * TR + rl];
+            const double2 b = make_double2((double)u.x * ph.x - (double)u.y * ph.y,
+                                           (double)u.x * ph.y + (double)u.y * ph.x);
+            const float2 v = su[(a * ntk + k) * TR + rl];
+            const double2 aw = make_double2((double)v.x * wv.x - (double)v.y * wv.y,
+                                            (double)v.x * wv.y + (double)v.y * wv.x);
+            acc[a][0].x += b.x * aw.x + b.y * aw.y;
+            acc[a][0].y += b.x * aw.y - b.y * aw.x;
+          }
+        }
+      }
+    }
+    double2* out = P + (size_t)q * npair * nr + r;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      const int jw = FULL ? z * kRsumTileJB + a : a;
+      if (jw >= nwf) continue;
+      if constexpr (FULL) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (i < nwf) out[(size_t)(i + nwf * jw) * nr] = acc[a][i];
+      } else {
+        out[(size_t)jw * nr] = acc[a][0];
+      }
+    }
+  }
+}
+
 template <int N>
 __device__ __forceinline__ void pd_zaxis_grid(const double2* __restrict__ src, int nx, int ny,
                                               unsigned zmask, const double2* tw, double2* g) {
@@ -775,11 +921,64 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
 }
 
 void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int ny, int nz,
-                      const int* ikq, const int* shift, const double2* wk, int nq, double2* P,
-                      hipStream_t s) {
+                      const int* ikq, const int* shift, const unsigned* kqs, const double2* wk,
+                      int nq, double2* P, hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= 8, "Wannier pair sums take at most 8 Wannier functions");
   RP_CHECK(nx <= kTwMax && ny <= kTwMax && nz <= kTwMax, "FFT axis too long for the phase tables");
   const int nr = nx * ny * nz;
+  if (nq <= 0) return;
+  // The LDS-tiled form when the packed indices exist and the grids of one
+  // point fit (NWF * NTK <= 20000).  Tile size and q split: least (rounds of
+  // workgroups) x (q per thread), with the workgroups a compute unit holds by
+  // LDS and by registers (two of the diagonal form, one of the full form).
+  const size_t fixed = sizeof(double2) * (3 * kTwMax + (size_t)ntk);
+  const size_t per_point = sizeof(float2) * (size_t)nwf * ntk;
+  int tr = 0, qper = nq;
+  if (kqs) {
+    int dev = 0, ncu = 256;
+    RP_HIP(hipGetDevice(&dev));
+    RP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int groups = full ? (nwf + kRsumTileJB - 1) / kRsumTileJB : 1;
+    long long best = -1;
+    for (int c : {16, 8, 4, 2, 1}) {
+      const size_t lds = fixed + per_point * c;
+      if (lds > kLdsBytes) continue;
+      const int per_cu = std::min<int>((int)(kLdsBytes / lds), full ? 1 : 2);
+      const int tiles = (nr + c - 1) / c, ql = kRsumTileThreads / c;
+      for (int split = 1; split <= 8; ++split) {
+        const int qp = (nq + split - 1) / split;
+        const int ns = (nq + qp - 1) / qp;
+        const long long rounds = ((long long)tiles * groups * ns + (long long)ncu * per_cu - 1) /
+                                 ((long long)ncu * per_cu);
+        const long long cost = rounds * ((qp + ql - 1) / ql);
+        if (best < 0 || cost < best) { best = cost; tr = c; qper = qp; }
+      }
+    }
+  }
+  if (tr) {
+    const int groups = full ? (nwf + kRsumTileJB - 1) / kRsumTileJB : 1;
+    const int tiles = (nr + tr - 1) / tr;
+    const int nsplit = (nq + qper - 1) / qper;
+    const size_t lds = fixed + per_point * tr;
+    dim3 grid(tiles, groups, nsplit);
+    auto go = [&](auto fn) {
+      RP_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      hipLaunchKernelGGL(fn, grid, dim3(kRsumTileThreads), lds, s, ur, ntk, nwf, nx, ny, nz, kqs,
+                         wk, nq, qper, P);
+    };
+    auto pick = [&](auto full_tag) {
+      constexpr bool F = decltype(full_tag)::value;
+      if (tr == 16) go(k_pair_rsum_tile<F, 16>);
+      else if (tr == 8) go(k_pair_rsum_tile<F, 8>);
+      else if (tr == 4) go(k_pair_rsum_tile<F, 4>);
+      else if (tr == 2) go(k_pair_rsum_tile<F, 2>);
+      else go(k_pair_rsum_tile<F, 1>);
+    };
+    if (full) pick(std::true_type{});
+    else pick(std::false_type{});
+    RP_HIP(hipGetLastError());
+    return;
+  }
   const int groups = full ? (nwf + kRsumJB - 1) / kRsumJB : 1;
   dim3 grid((nr + 255) / 256, nq, groups);
   if (full)

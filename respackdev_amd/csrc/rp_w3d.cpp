@@ -41,6 +41,8 @@ struct W3d {
   std::vector<float2> eps_head;
   std::vector<long long> rho_off, map_off, eps_off;
   std::vector<int4> qinfo;
+  // the q grouped by irreducible eps^-1 for the W contraction
+  std::vector<int> wgoff, wgmem;
   std::vector<int> minv;
   std::vector<double2> pf1, pf2;
   double chead = 0;
@@ -51,7 +53,8 @@ struct W3d {
   DevBuf<float2> d_ur, d_eps;
   DevBuf<long long> d_eps_off, d_rho_off, d_map_off;
   DevBuf<int4> d_qinfo;
-  DevBuf<int> d_minv;
+  DevBuf<int> d_minv, d_wgoff, d_wgmem;
+  DevBuf<double2> d_wx;  // X vectors of the W contraction's q groups
   DevBuf<double> d_sq;
   // per-q geometry of the pair densities, every q at once (built on first use
   // after rp_w3d_set_q): |q+G|, k+q index and umklapp per k, pruned-transform plan
@@ -61,6 +64,7 @@ struct W3d {
   int geo_max_nzz = 0;
   DevBuf<double> d_geo_qg;
   DevBuf<int> d_geo_ikq, d_geo_shift, d_geo_plan, d_geo_g0, d_ngpsi, d_geo_lg;
+  DevBuf<unsigned> d_geo_kqs;  // ikq and shift packed (pair_rsum_pack; empty if they do not fit)
   DevBuf<long long> d_geo_qg_off, d_geo_plan_off;
   // summed pair-density grids P[q][p][r] of a chunk of q (launch_pair_rsum)
   DevBuf<double2> d_P;
@@ -147,6 +151,20 @@ void upload_eps_maps(W3d& w, hipStream_t s) {
   w.d_map_off.upload(w.map_off, s);
   w.d_qinfo.upload(w.qinfo, s);
   w.d_minv.upload(w.minv, s);
+  // groups of at most w_contract_members q sharing one irreducible eps^-1, so
+  // that the W contraction reads each eps^-1 once per group
+  const size_t mx = (size_t)w_contract_members(std::max(1, w.nwf));
+  std::vector<std::vector<int>> by(w.nq_irr);
+  for (int q = 0; q < w.ntq; ++q) by[w.qinfo[q].z].push_back(q);
+  w.wgoff.assign(1, 0);
+  w.wgmem.clear();
+  for (const auto& v : by)
+    for (size_t i = 0; i < v.size(); i += mx) {
+      for (size_t j = i; j < std::min(v.size(), i + mx); ++j) w.wgmem.push_back(v[j]);
+      w.wgoff.push_back((int)w.wgmem.size());
+    }
+  w.d_wgoff.upload(w.wgoff, s);
+  w.d_wgmem.upload(w.wgmem, s);
   w.d_pf1.upload(w.pf1, s);
   w.d_pf2.upload(w.pf2, s);
 }
@@ -220,6 +238,11 @@ void ensure_geometry(W3d& w, hipStream_t s) {
   w.d_geo_lg.upload(lgs, s);
   w.d_geo_ikq.upload(ikq, s);
   w.d_geo_shift.upload(shift, s);
+  std::vector<unsigned> kqs(ikq.size());
+  bool packed = true;
+  for (size_t i = 0; i < ikq.size() && packed; ++i) packed = pair_rsum_pack(ikq[i], &shift[3 * i], &kqs[i]);
+  if (packed) w.d_geo_kqs.upload(kqs, s);
+  else w.d_geo_kqs.release();
   w.d_geo_plan.upload(plan, s);
   w.d_geo_qg_off.upload(w.geo_qg_off, s);
   w.d_geo_plan_off.upload(w.geo_plan_off, s);
@@ -246,7 +269,8 @@ void rho_all_q(W3d& w, bool full, const double2* wk, const long long* d_rho_off,
     const int nq = std::min(qchunk, w.ntq - q0);
     launch_pair_rsum(w.d_ur.p, w.ntk, w.nwf, full, w.nw[0], w.nw[1], w.nw[2],
                      w.d_geo_ikq.p + (size_t)q0 * w.ntk, w.d_geo_shift.p + (size_t)3 * q0 * w.ntk,
-                     wk, nq, w.d_P.p, s);
+                     w.d_geo_kqs.n ? w.d_geo_kqs.p + (size_t)q0 * w.ntk : nullptr, wk, nq,
+                     w.d_P.p, s);
     if (pruned_transform_lds_bytes(w.nw[0], w.nw[1], w.nw[2], w.geo_max_nzz) <= lds_budget_bytes())
       launch_rho_from_grid(w.d_P.p, npair, w.nw[0], w.nw[1], w.nw[2], w.d_geo_plan.p,
                            w.d_geo_plan_off.p, w.geo_max_nzz, w.d_geo_qg.p, w.d_geo_qg_off.p,
@@ -605,9 +629,11 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     w.d_funcw.reserve((size_t)w.ntq * nen * nwf * nwf);
     launch_w_func(w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.ntq, nwf, w.d_func.p, w.d_shell.p, s);
     RP_HIP(hipEventRecord(w.ev[2], s));
+    w.d_wx.reserve((size_t)w_contract_vectors(nwf) * (size_t)w.minv.size());
     launch_w_contract(w.d_eps.p, w.d_eps_off.p, w.d_rho.p, w.d_rho_off.p, w.d_qinfo.p, w.d_minv.p,
-                      w.d_pf1.p, w.d_pf2.p, w.d_map_off.p, w.d_shell.p, w.ntq, nwf, nen,
-                      *std::max_element(w.nge.begin(), w.nge.end()), w.d_funcw.p, s);
+                      w.d_pf1.p, w.d_pf2.p, w.d_map_off.p, w.d_shell.p, w.d_wgoff.p, w.d_wgmem.p,
+                      (int)w.wgoff.size() - 1, nwf, nen,
+                      *std::max_element(w.nge.begin(), w.nge.end()), w.d_wx.p, w.d_funcw.p, s);
     RP_HIP(hipEventRecord(w.ev[3], s));
     if (func_out)
       RP_HIP(hipMemcpyAsync(func_out, w.d_func.p, sizeof(double2) * w.ntq * nwf * nwf,

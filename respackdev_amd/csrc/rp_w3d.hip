@@ -60,147 +60,176 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
   }
 }
 
-// funcw for one (q, w): Y(a,c) = sum_b E(a,b) X_c(b) over the irreducible
-// eps^-1 E (complex64, column-major), then funcw = sum_a Z(a) Y(a,.) + shell.
+// funcw for one w and a group of q that share one irreducible eps^-1 E
+// (complex64, column-major): per member q, Y(a,c) = sum_b E(a,b) X_c(b), then
+// funcw = sum_a Z(a) Y(a,.) + shell, where
 //   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
-// The vectors are the VB = nv <= 8 Wannier functions of calc_w3d and every
-// (i,j) is formed, out [q][w][j][i] (calc_j3d's diagonal pairs: k_j_contract).
-// A workgroup takes all the vectors of its (q, w), so E is read once per
-// (q, w): 8 B of HBM per 8 VB flops.  nt threads (a multiple of 64, at most 512,
-// at least half the largest NG_eps), rows a and a + nt of E per thread (NG_eps
-// <= 1024): every wave holds two rows per lane, so no wave waits on the others'
-// second row; the loads of the next PD columns of E are in flight during a
-// column's arithmetic.
-// grid: (nen, ntq), nt threads.
+// with the member's own map m and phases.  The vectors are the NV <= 8 Wannier
+// functions of calc_w3d per member, up to 4, 4, 3, 2, 2, 1, 1, 1 members per group (NV = 1..8),
+// and every (i,j) of a member is formed, out [q][w][j][i] (calc_j3d's diagonal
+// pairs: k_j_contract).  A workgroup takes all the vectors of its group, so E
+// is read once per (group, w): 8 B of HBM per 8 NV (members) flops.  X of
+// every group is formed once (k_w_xvec, independent of w) and staged kTB
+// columns at a time.  One row a of E per thread (nt = NG_eps rounded up to 64,
+// NG_eps <= 1024), and the loads of E run PD columns ahead of the arithmetic
+// across the staging barriers.
+// grid: (nen, ngroups), nt threads; group g has the q gmem[goff[g] .. goff[g+1]).
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
-constexpr int kCT = 512;       // threads of the contraction
-constexpr int kTB = 32;        // columns of E staged per pass
-template <int VB, int PD = 8>
-__global__ void __launch_bounds__(512) k_w_contract(
+constexpr int kCT = 1024;      // threads of the contraction (one row each)
+constexpr int kTB = 128;       // columns of X staged per pass
+constexpr int wc_members_of(int nv) { return nv <= 2 ? 4 : (nv == 3 ? 3 : (nv <= 5 ? 2 : 1)); }
+template <int NV>
+constexpr int wc_members() { return wc_members_of(NV); }
+int w_contract_members(int nwf) { return wc_members_of(nwf); }
+int w_contract_vectors(int nwf) { return nwf * w_contract_members(nwf); }
+
+// X_c(b) of every group, [VB][nge] at xw + VB * map_off[first member] (zero
+// past the group's vectors).  grid: ngroups, 256 threads.
+__global__ void __launch_bounds__(256) k_w_xvec(const double2* __restrict__ rho,
+                                                const long long* __restrict__ rho_off,
+                                                const int4* __restrict__ qinfo,
+                                                const int* __restrict__ minv,
+                                                const double2* __restrict__ pf1,
+                                                const double2* __restrict__ pf2,
+                                                const long long* __restrict__ map_off,
+                                                const int* __restrict__ goff,
+                                                const int* __restrict__ gmem, int nv, int vb,
+                                                double2* __restrict__ xw) {
+  const int grp = blockIdx.x;
+  const int mbeg = goff[grp], nvt = (goff[grp + 1] - mbeg) * nv;
+  const int q0 = gmem[mbeg];
+  const int nge = qinfo[q0].x;
+  double2* out = xw + map_off[q0] * vb;
+  for (int e = threadIdx.x; e < vb * nge; e += blockDim.x) {
+    const int c = e / nge, b = e % nge;
+    double2 v = make_double2(0.0, 0.0);
+    if (c < nvt) {
+      const int q = gmem[mbeg + c / nv];
+      const int4 qi = qinfo[q];
+      const long long mo = map_off[q];
+      const int g = minv[mo + b];
+      const double2 rv = rho[rho_off[q] + (size_t)(c % nv) * qi.y + g];
+      const double2 p = qi.w == 1 ? pf2[mo + g] : pf1[mo + g];
+      // trs = +1: conj(rho) pf2; trs = -1: rho pf1
+      const double xr = rv.x, xi = qi.w == 1 ? -rv.y : rv.y;
+      v = make_double2(xr * p.x - xi * p.y, xr * p.y + xi * p.x);
+    }
+    out[e] = v;
+  }
+}
+
+template <int NV, int PD = 16>
+__global__ void __launch_bounds__(kCT) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
     const int4* __restrict__ qinfo, const int* __restrict__ minv,
     const double2* __restrict__ pf1, const double2* __restrict__ pf2,
-    const long long* __restrict__ map_off, const double2* __restrict__ shell, int nv, int nen,
-    double2* __restrict__ funcw) {
-  constexpr int kPairs = VB * VB;
+    const long long* __restrict__ map_off, const double2* __restrict__ shell,
+    const int* __restrict__ goff, const int* __restrict__ gmem,
+    const double2* __restrict__ xw, int nen, double2* __restrict__ funcw) {
+  constexpr int M = wc_members<NV>();
+  constexpr int VB = M * NV;
   constexpr int kWaves = kCT / 64;  // at most
+  static_assert(kTB % PD == 0, "E is loaded PD columns ahead across blocks");
   const int nt = (int)blockDim.x, nwv = nt >> 6;
-  const int w = blockIdx.x, q = blockIdx.y;
-  const int4 qi = qinfo[q];
-  const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
+  const int w = blockIdx.x, grp = blockIdx.y;
+  const int mbeg = goff[grp], nm = goff[grp + 1] - mbeg;  // 1 <= nm <= M
+  const int q0m = gmem[mbeg];
+  const int4 qi0 = qinfo[q0m];
+  const int nge = qi0.x, iqir = qi0.z;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
-  const int nvb = nv;  // vectors (Wannier functions)
-  const double2* r = rho + rho_off[q];
-  const int* mi = minv + map_off[q];
-  const double2* p1 = pf1 + map_off[q];
-  const double2* p2 = pf2 + map_off[q];
+  const double2* xg = xw + map_off[q0m] * VB;
   __shared__ double2 X[VB][kTB];
-  __shared__ double red[kPairs][kWaves][2];
+  __shared__ double red[M * NV * NV][kWaves][2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int a0 = t, a1 = t + nt;
-  const bool v0 = a0 < nge, v1 = a1 < nge;
-  double y0r[VB], y0i[VB], y1r[VB], y1i[VB];
+  const bool v0 = t < nge;
+  double yr[VB], yi[VB];
 #pragma unroll
-  for (int c = 0; c < VB; ++c) y0r[c] = y0i[c] = y1r[c] = y1i[c] = 0.0;
+  for (int c = 0; c < VB; ++c) yr[c] = yi[c] = 0.0;
+  // rows past nge read row 0 (unconditional loads, several in flight); their
+  // sums are never used (Z is zero there)
+  const float2* ep = E + (v0 ? t : 0);
+  float2 q0[PD];
+#pragma unroll
+  for (int j = 0; j < PD; ++j) q0[j] = ep[(size_t)(j < nge ? j : nge - 1) * nge];
   for (int b0 = 0; b0 < nge; b0 += kTB) {
     const int nb = min(kTB, nge - b0);
     __syncthreads();
     for (int e = t; e < VB * kTB; e += nt) {
       const int c = e / kTB, bb = e % kTB;
-      double2 v = make_double2(0.0, 0.0);  // vectors beyond nvb stay zero
-      if (bb < nb && c < nvb) {
-        const int b = b0 + bb, g = mi[b];
-        const double2 rv = r[(size_t)c * ngpsi + g];
-        const double2 p = trs == 1 ? p2[g] : p1[g];
-        // trs = +1: conj(rho) pf2; trs = -1: rho pf1
-        const double xr = rv.x, xi = trs == 1 ? -rv.y : rv.y;
-        v = make_double2(xr * p.x - xi * p.y, xr * p.y + xi * p.x);
-      }
-      X[c][bb] = v;
+      X[c][bb] = bb < nb ? xg[(size_t)c * nge + b0 + bb] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    // rows past nge read row 0 (unconditional loads, several in flight); their
-    // sums are never used (Z is zero there)
-    const float2* c0p = E + (size_t)b0 * nge + (v0 ? a0 : 0);
-    const float2* c1p = E + (size_t)b0 * nge + (v1 ? a1 : 0);
-    // the loads of the next PD columns are in flight during a column's arithmetic
-    float2 q0[PD], q1[PD];
-#pragma unroll
-    for (int j = 0; j < PD; ++j) {
-      const int cj = j < nb ? j : nb - 1;
-      q0[j] = c0p[(size_t)cj * nge];
-      q1[j] = c1p[(size_t)cj * nge];
-    }
 #pragma unroll 1
     for (int bq = 0; bq < nb; bq += PD) {
 #pragma unroll
       for (int j = 0; j < PD; ++j) {
         const int bb = bq + j;
         if (bb >= nb) break;  // uniform
-        const float2 e0 = q0[j], e1 = q1[j];
-        const int nx = bb + PD < nb ? bb + PD : nb - 1;
-        q0[j] = c0p[(size_t)nx * nge];
-        q1[j] = c1p[(size_t)nx * nge];
-        const double e0r = e0.x, e0i = e0.y, e1r = e1.x, e1i = e1.y;
+        const float2 e0 = q0[j];
+        const int col = b0 + bb + PD;  // the column this slot holds next
+        q0[j] = ep[(size_t)(col < nge ? col : nge - 1) * nge];
+        const double er = e0.x, ei = e0.y;
 #pragma unroll
-        for (int c = 0; c < VB; ++c) {
+        for (int c = 0; c < VB; ++c) {  // X is zero past the group's vectors
           const double2 x = X[c][bb];
-          y0r[c] += e0r * x.x - e0i * x.y;
-          y0i[c] += e0r * x.y + e0i * x.x;
-          y1r[c] += e1r * x.x - e1i * x.y;
-          y1i[c] += e1r * x.y + e1i * x.x;
+          yr[c] += er * x.x - ei * x.y;
+          yi[c] += er * x.y + ei * x.x;
         }
       }
     }
   }
-  // funcw_ij = sum_a Z(a) Y(a, .): per thread over its two rows, then over
-  // the wave (shuffles) and the waves (LDS)
-  const int g0 = v0 ? mi[a0] : 0, g1 = v1 ? mi[a1] : 0;
+  // funcw_ij = sum_a Z(a) Y(a, .) per member: per thread over its row, then
+  // over the wave (shuffles) and the waves (LDS)
 #pragma unroll
-  for (int zi = 0; zi < VB; ++zi) {
-    if (zi >= nvb) continue;
-    // Z of vector zi at both rows
-    double z0r = 0.0, z0i = 0.0, z1r = 0.0, z1i = 0.0;
-    if (v0) {
-      const double2 rv = r[(size_t)zi * ngpsi + g0];
-      if (trs == 1) { const double2 p = p1[g0]; z0r = rv.x * p.x - rv.y * p.y; z0i = rv.x * p.y + rv.y * p.x; }
-      else { const double2 p = p2[g0]; z0r = rv.x * p.x + rv.y * p.y; z0i = rv.x * p.y - rv.y * p.x; }
-    }
-    if (v1) {
-      const double2 rv = r[(size_t)zi * ngpsi + g1];
-      if (trs == 1) { const double2 p = p1[g1]; z1r = rv.x * p.x - rv.y * p.y; z1i = rv.x * p.y + rv.y * p.x; }
-      else { const double2 p = p2[g1]; z1r = rv.x * p.x + rv.y * p.y; z1i = rv.x * p.y - rv.y * p.x; }
-    }
+  for (int m = 0; m < M; ++m) {
+    if (m >= nm) break;  // uniform
+    const int q = gmem[mbeg + m];
+    const int4 qi = qinfo[q];
+    const long long mo = map_off[q];
+    const int trs = qi.w, ngpsi = qi.y;
+    const double2* r = rho + rho_off[q];
+    const int g0 = v0 ? minv[mo + t] : 0;
 #pragma unroll
-    for (int yc = 0; yc < VB; ++yc) {
-      if (yc >= nvb) continue;
-      double sr = (z0r * y0r[yc] - z0i * y0i[yc]) + (z1r * y1r[yc] - z1i * y1i[yc]);
-      double si = (z0r * y0i[yc] + z0i * y0r[yc]) + (z1r * y1i[yc] + z1i * y1r[yc]);
-      for (int off = 32; off > 0; off >>= 1) {
-        sr += __shfl_xor(sr, off);
-        si += __shfl_xor(si, off);
+    for (int zi = 0; zi < NV; ++zi) {
+      // Z of vector zi at the thread's row
+      double zr = 0.0, zim = 0.0;
+      if (v0) {
+        const double2 rv = r[(size_t)zi * ngpsi + g0];
+        if (trs == 1) { const double2 p = pf1[mo + g0]; zr = rv.x * p.x - rv.y * p.y; zim = rv.x * p.y + rv.y * p.x; }
+        else { const double2 p = pf2[mo + g0]; zr = rv.x * p.x + rv.y * p.y; zim = rv.x * p.y - rv.y * p.x; }
       }
-      // pair slot: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
-      const int slot = trs == 1 ? zi + VB * yc : yc + VB * zi;
-      if (lane == 0) {
-        red[slot][wave][0] = sr;
-        red[slot][wave][1] = si;
+#pragma unroll
+      for (int yc = 0; yc < NV; ++yc) {
+        const int c = m * NV + yc;
+        double sr = zr * yr[c] - zim * yi[c];
+        double si = zr * yi[c] + zim * yr[c];
+        for (int off = 32; off > 0; off >>= 1) {
+          sr += __shfl_xor(sr, off);
+          si += __shfl_xor(si, off);
+        }
+        // pair slot (i + NV j) of the member: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
+        const int slot = m * NV * NV + (trs == 1 ? zi + NV * yc : yc + NV * zi);
+        if (lane == 0) {
+          red[slot][wave][0] = sr;
+          red[slot][wave][1] = si;
+        }
       }
     }
   }
   __syncthreads();
-  const int npairs = nvb * nvb;
-  for (int e = t; e < 2 * npairs; e += nt) {
-    const int pr = e >> 1, part = e & 1;
-    const int i = pr % nvb, j = pr / nvb;
-    const int slot = i + VB * j;
+  for (int e = t; e < 2 * nm * NV * NV; e += nt) {
+    const int slot = e >> 1, part = e & 1;
+    const int m = slot / (NV * NV), pr = slot % (NV * NV);
+    const int i = pr % NV, j = pr / NV;
+    const int q = gmem[mbeg + m];
     double sum = 0.0;
 #pragma unroll
-    for (int wv = 0; wv < nwv; ++wv) sum += red[slot][wv][part];
-    const double2 sh = shell[((size_t)q * nv + j) * nv + i];
-    double2* o = funcw + (((size_t)q * nen + w) * nv + j) * nv + i;
+    for (int wv = 0; wv < kWaves; ++wv)
+      if (wv < nwv) sum += red[slot][wv][part];
+    const double2 sh = shell[((size_t)q * NV + j) * NV + i];
+    double2* o = funcw + (((size_t)q * nen + w) * NV + j) * NV + i;
     if (part == 0) o->x = sum + sh.x;
     else o->y = sum + sh.y;
   }
@@ -477,7 +506,7 @@ __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ fu
 }
 
 int w3d_max_wannier() { return kMaxWf; }
-int w3d_max_eps_g() { return 2 * kCT; }
+int w3d_max_eps_g() { return kCT; }
 
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s) {
@@ -488,18 +517,21 @@ void launch_w_func(const double2* rho, const long long* rho_off, const int4* qin
 void launch_w_contract(const float2* eps, const long long* eps_off, const double2* rho,
                        const long long* rho_off, const int4* qinfo, const int* minv,
                        const double2* pf1, const double2* pf2, const long long* map_off,
-                       const double2* shell, int ntq, int nwf, int nen, int max_nge,
-                       double2* funcw, hipStream_t s) {
+                       const double2* shell, const int* goff, const int* gmem, int ngroups,
+                       int nwf, int nen, int max_nge, double2* xw, double2* funcw,
+                       hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
-  RP_CHECK(max_nge >= 1 && max_nge <= 2 * kCT, "W contraction: eps sphere too large");
-  const int nt = std::max(64, (max_nge + 127) / 128 * 64);  // two rows per thread
-  // as many accumulators as Wannier functions
+  RP_CHECK(max_nge >= 1 && max_nge <= kCT, "W contraction: eps sphere too large");
+  if (ngroups <= 0) return;
+  const int nt = (max_nge + 63) / 64 * 64;  // one row per thread
+  hipLaunchKernelGGL(k_w_xvec, dim3(ngroups), dim3(256), 0, s, rho, rho_off, qinfo, minv, pf1,
+                     pf2, map_off, goff, gmem, nwf, w_contract_vectors(nwf), xw);
   switch (nwf) {
-#define RP_WC(NV)                                                                              \
-  case NV:                                                                                     \
-    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ntq), dim3(nt), 0, s, eps,         \
-                       eps_off, rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, nwf, nen, \
-                       funcw);                                                                 \
+#define RP_WC(NV)                                                                             \
+  case NV:                                                                                    \
+    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ngroups), dim3(nt), 0, s, eps, eps_off, \
+                       rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, goff, gmem, xw,   \
+                       nen, funcw);                                                           \
     break;
     RP_WC(1) RP_WC(2) RP_WC(3) RP_WC(4) RP_WC(5) RP_WC(6) RP_WC(7) RP_WC(8)
 #undef RP_WC
