@@ -70,21 +70,20 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // and every (i,j) of a member is formed, out [q][w][j][i] (calc_j3d's diagonal
 // pairs: k_j_contract).  A workgroup takes all the vectors of its group, so E
 // is read once per (group, w): 8 B of HBM per 8 NV (members) flops.  X of
-// every group is formed once (k_w_xvec, independent of w) and staged kTB
-// columns at a time.  One row a of E per thread (nt = NG_eps rounded up to 64,
-// NG_eps <= 1024), and the loads of E run PD columns ahead of the arithmetic
-// across the staging barriers.
+// every group is formed once (k_w_xvec, independent of w); X(b, .) is the same
+// for every row, so it is read with scalar loads (no LDS, no barriers).  One
+// row a of E per thread (nt = NG_eps rounded up to 64, NG_eps <= 1024), the
+// loads of E PD columns ahead of the arithmetic.
 // grid: (nen, ngroups), nt threads; group g has the q gmem[goff[g] .. goff[g+1]).
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
 constexpr int kCT = 1024;      // threads of the contraction (one row each)
-constexpr int kTB = 128;       // columns of X staged per pass
 constexpr int wc_members_of(int nv) { return nv <= 2 ? 4 : (nv == 3 ? 3 : (nv <= 5 ? 2 : 1)); }
 template <int NV>
 constexpr int wc_members() { return wc_members_of(NV); }
 int w_contract_members(int nwf) { return wc_members_of(nwf); }
 int w_contract_vectors(int nwf) { return nwf * w_contract_members(nwf); }
 
-// X_c(b) of every group, [VB][nge] at xw + VB * map_off[first member] (zero
+// X_c(b) of every group, [nge][VB] at xw + VB * map_off[first member] (zero
 // past the group's vectors).  grid: ngroups, 256 threads.
 __global__ void __launch_bounds__(256) k_w_xvec(const double2* __restrict__ rho,
                                                 const long long* __restrict__ rho_off,
@@ -102,7 +101,7 @@ __global__ void __launch_bounds__(256) k_w_xvec(const double2* __restrict__ rho,
   const int nge = qinfo[q0].x;
   double2* out = xw + map_off[q0] * vb;
   for (int e = threadIdx.x; e < vb * nge; e += blockDim.x) {
-    const int c = e / nge, b = e % nge;
+    const int b = e / vb, c = e % vb;
     double2 v = make_double2(0.0, 0.0);
     if (c < nvt) {
       const int q = gmem[mbeg + c / nv];
@@ -131,7 +130,6 @@ __global__ void __launch_bounds__(kCT) k_w_contract(
   constexpr int M = wc_members<NV>();
   constexpr int VB = M * NV;
   constexpr int kWaves = kCT / 64;  // at most
-  static_assert(kTB % PD == 0, "E is loaded PD columns ahead across blocks");
   const int nt = (int)blockDim.x, nwv = nt >> 6;
   const int w = blockIdx.x, grp = blockIdx.y;
   const int mbeg = goff[grp], nm = goff[grp + 1] - mbeg;  // 1 <= nm <= M
@@ -140,7 +138,6 @@ __global__ void __launch_bounds__(kCT) k_w_contract(
   const int nge = qi0.x, iqir = qi0.z;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
   const double2* xg = xw + map_off[q0m] * VB;
-  __shared__ double2 X[VB][kTB];
   __shared__ double red[M * NV * NV][kWaves][2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const bool v0 = t < nge;
@@ -153,30 +150,25 @@ __global__ void __launch_bounds__(kCT) k_w_contract(
   float2 q0[PD];
 #pragma unroll
   for (int j = 0; j < PD; ++j) q0[j] = ep[(size_t)(j < nge ? j : nge - 1) * nge];
-  for (int b0 = 0; b0 < nge; b0 += kTB) {
-    const int nb = min(kTB, nge - b0);
-    __syncthreads();
-    for (int e = t; e < VB * kTB; e += nt) {
-      const int c = e / kTB, bb = e % kTB;
-      X[c][bb] = bb < nb ? xg[(size_t)c * nge + b0 + bb] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
+  // X of column b is uniform over the workgroup: scalar loads, no LDS
 #pragma unroll 1
-    for (int bq = 0; bq < nb; bq += PD) {
+  for (int bq = 0; bq < nge; bq += PD) {
 #pragma unroll
-      for (int j = 0; j < PD; ++j) {
-        const int bb = bq + j;
-        if (bb >= nb) break;  // uniform
-        const float2 e0 = q0[j];
-        const int col = b0 + bb + PD;  // the column this slot holds next
-        q0[j] = ep[(size_t)(col < nge ? col : nge - 1) * nge];
-        const double er = e0.x, ei = e0.y;
+    for (int j = 0; j < PD; ++j) {
+      const int bb = bq + j;
+      if (bb >= nge) break;  // uniform
+      const float2 e0 = q0[j];
+      const int col = bb + PD;  // the column this slot holds next
+      q0[j] = ep[(size_t)(col < nge ? col : nge - 1) * nge];
+      const double er = e0.x, ei = e0.y;
+      const double2* xb = xg + (size_t)bb * VB;
 #pragma unroll
-        for (int c = 0; c < VB; ++c) {  // X is zero past the group's vectors
-          const double2 x = X[c][bb];
-          yr[c] += er * x.x - ei * x.y;
-          yi[c] += er * x.y + ei * x.x;
-        }
+      for (int c = 0; c < VB; ++c) {  // X is zero past the group's vectors
+        const double2 x = xb[c];
+        yr[c] = fma(er, x.x, yr[c]);
+        yr[c] = fma(-ei, x.y, yr[c]);
+        yi[c] = fma(er, x.y, yi[c]);
+        yi[c] = fma(ei, x.x, yi[c]);
       }
     }
   }
