@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
 // of its workgroup's range of qper q.  Dynamic LDS: the phase tables, w_k
 // (ntk) and the grids [w][k][TR] (single precision, as in HBM).
 constexpr int kRsumTileThreads = 512;
-constexpr int kRsumTileJB = 3;  // jw per group in the full mode
+constexpr int kRsumTileJB = 5;  // jw per group in the full mode
 __device__ __forceinline__ int rsum_wrap(int m, int n) {
   if (m >= n || m < -n) m %= n;  // only for umklapp vectors beyond +-1
   return m < 0 ? m + n : m;
