@@ -66,24 +66,28 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 //   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
 // with the member's own map m and phases.  The vectors are the NV <= 8 Wannier
-// functions of calc_w3d per member, up to 4, 4, 3, 2, 2, 1, 1, 1 members per group (NV = 1..8),
-// and every (i,j) of a member is formed, out [q][w][j][i] (calc_j3d's diagonal
+// functions of calc_w3d per member, 16 / NV members per group at most, and
+// every (i,j) of a member is formed, out [q][w][j][i] (calc_j3d's diagonal
 // pairs: k_j_contract).  A workgroup takes all the vectors of its group, so E
-// is read once per (group, w): 8 B of HBM per 8 NV (members) flops.  X of
-// every group is formed once (k_w_xvec, independent of w); X(b, .) is the same
-// for every row, so it is read with scalar loads (no LDS, no barriers).  One
-// row a of E per thread (nt = NG_eps rounded up to 64, NG_eps <= 1024), the
-// loads of E PD columns ahead of the arithmetic.
-// grid: (nen, ngroups), nt threads; group g has the q gmem[goff[g] .. goff[g+1]).
+// is read once per (group, w).  Y = E X runs on the FP64 matrix cores as in
+// k_j_contract (Re Y = Er Xr - Ei Xi, Im Y = Er Xi + Ei Xr, v_mfma_f64_16x16x4,
+// the group's vectors as one 16-column block): lane l supplies A(l % 16, l / 16)
+// and B(l / 16, l % 16) and holds D(l / 16 + 4 v, l % 16).  X of every group
+// is formed once (k_w_xvec, independent of w).  A wave takes one 16-row block
+// of E per pass over all columns; then each lane multiplies its four Y(a, c)
+// by the Z_i(a) of c's member and keeps NV sums, reduced over the lanes of
+// c and the waves at the end.
+// grid: (nen, ngroups), 512 threads; group g has the q gmem[goff[g] .. goff[g+1]).
+typedef double jd4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
-constexpr int kCT = 1024;      // threads of the contraction (one row each)
-constexpr int wc_members_of(int nv) { return nv <= 2 ? 4 : (nv == 3 ? 3 : (nv <= 5 ? 2 : 1)); }
+constexpr int kCT = 1024;      // rows of E the W contraction takes (NG_eps)
+constexpr int kWcVec = 16;     // vectors of a group: one MFMA column block
 template <int NV>
-constexpr int wc_members() { return wc_members_of(NV); }
-int w_contract_members(int nwf) { return wc_members_of(nwf); }
-int w_contract_vectors(int nwf) { return nwf * w_contract_members(nwf); }
+constexpr int wc_members() { return kWcVec / NV; }
+int w_contract_members(int nwf) { return kWcVec / nwf; }
+int w_contract_vectors(int) { return kWcVec; }
 
-// X_c(b) of every group, [nge][VB] at xw + VB * map_off[first member] (zero
+// X_c(b) of every group, [nge][vb] at xw + vb * map_off[first member] (zero
 // past the group's vectors).  grid: ngroups, 256 threads.
 __global__ void __launch_bounds__(256) k_w_xvec(const double2* __restrict__ rho,
                                                 const long long* __restrict__ rho_off,
@@ -118,8 +122,8 @@ __global__ void __launch_bounds__(256) k_w_xvec(const double2* __restrict__ rho,
   }
 }
 
-template <int NV, int PD = 16>
-__global__ void __launch_bounds__(kCT) k_w_contract(
+template <int NV>
+__global__ void __launch_bounds__(512) k_w_contract(
     const float2* __restrict__ eps, const long long* __restrict__ eps_off,
     const double2* __restrict__ rho, const long long* __restrict__ rho_off,
     const int4* __restrict__ qinfo, const int* __restrict__ minv,
@@ -127,99 +131,97 @@ __global__ void __launch_bounds__(kCT) k_w_contract(
     const long long* __restrict__ map_off, const double2* __restrict__ shell,
     const int* __restrict__ goff, const int* __restrict__ gmem,
     const double2* __restrict__ xw, int nen, double2* __restrict__ funcw) {
-  constexpr int M = wc_members<NV>();
-  constexpr int VB = M * NV;
-  constexpr int kWaves = kCT / 64;  // at most
-  const int nt = (int)blockDim.x, nwv = nt >> 6;
   const int w = blockIdx.x, grp = blockIdx.y;
-  const int mbeg = goff[grp], nm = goff[grp + 1] - mbeg;  // 1 <= nm <= M
+  const int mbeg = goff[grp], nm = goff[grp + 1] - mbeg;  // 1 <= nm <= 16 / NV
+  const int nvt = nm * NV;
   const int q0m = gmem[mbeg];
   const int4 qi0 = qinfo[q0m];
   const int nge = qi0.x, iqir = qi0.z;
   const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
-  const double2* xg = xw + map_off[q0m] * VB;
-  __shared__ double red[M * NV * NV][kWaves][2];
+  const double2* Xg = xw + map_off[q0m] * kWcVec;  // (b, c) at Xg[b * 16 + c]
+  __shared__ double red[kWcVec][NV][8][2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const bool v0 = t < nge;
-  double yr[VB], yi[VB];
+  const int li = lane & 15, lk = lane >> 4;
+  const int nab = (nge + 15) / 16;
+  // this lane's vector c = li: member li / NV, its NV sums Z_i Y_c
+  const bool cv = li < nvt;
+  const int qm = gmem[mbeg + (cv ? li / NV : 0)];
+  const int4 qim = qinfo[qm];
+  const long long mom = map_off[qm];
+  const double2* rm = rho + rho_off[qm];
+  double sre[NV], sim[NV];
 #pragma unroll
-  for (int c = 0; c < VB; ++c) yr[c] = yi[c] = 0.0;
-  // rows past nge read row 0 (unconditional loads, several in flight); their
-  // sums are never used (Z is zero there)
-  const float2* ep = E + (v0 ? t : 0);
-  float2 q0[PD];
+  for (int i = 0; i < NV; ++i) sre[i] = sim[i] = 0.0;
+  for (int ab = wave; ab < nab; ab += 8) {
+    jd4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
+    // rows past nge read row nge - 1; their sums are never used
+    const int a = ab * 16 + li;
+    const int arow = a < nge ? a : nge - 1;
+    for (int b0 = 0; b0 < nge; b0 += 4) {
+      const int b = b0 + lk;
+      const bool bv = b < nge;  // columns past nge contribute zero
+      const int bc = bv ? b : nge - 1;
+      const double2 xv = Xg[(size_t)bc * kWcVec + li];
+      const double xr = bv ? xv.x : 0.0, xi = bv ? xv.y : 0.0, nxi = -xi;
+      const float2 ev = E[(size_t)bc * nge + arow];
+      const double er = ev.x, ei = ev.y;
+      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr, accr, 0, 0, 0);
+      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi, accr, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xi, acci, 0, 0, 0);
+      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, xr, acci, 0, 0, 0);
+    }
+    // Z_i(a) Y(a, c) for this lane's rows a = 16 ab + lk + 4 v and vector c = li
+    if (cv) {
 #pragma unroll
-  for (int j = 0; j < PD; ++j) q0[j] = ep[(size_t)(j < nge ? j : nge - 1) * nge];
-  // X of column b is uniform over the workgroup: scalar loads, no LDS
-#pragma unroll 1
-  for (int bq = 0; bq < nge; bq += PD) {
+      for (int v = 0; v < 4; ++v) {
+        const int ar = ab * 16 + lk + 4 * v;
+        if (ar >= nge) continue;
+        const int g = minv[mom + ar];
+        const double yr = accr[v], yi = acci[v];
+        const double2 pp = qim.w == 1 ? pf1[mom + g] : pf2[mom + g];
 #pragma unroll
-    for (int j = 0; j < PD; ++j) {
-      const int bb = bq + j;
-      if (bb >= nge) break;  // uniform
-      const float2 e0 = q0[j];
-      const int col = bb + PD;  // the column this slot holds next
-      q0[j] = ep[(size_t)(col < nge ? col : nge - 1) * nge];
-      const double er = e0.x, ei = e0.y;
-      const double2* xb = xg + (size_t)bb * VB;
-#pragma unroll
-      for (int c = 0; c < VB; ++c) {  // X is zero past the group's vectors
-        const double2 x = xb[c];
-        yr[c] = fma(er, x.x, yr[c]);
-        yr[c] = fma(-ei, x.y, yr[c]);
-        yi[c] = fma(er, x.y, yi[c]);
-        yi[c] = fma(ei, x.x, yi[c]);
+        for (int i = 0; i < NV; ++i) {
+          const double2 rv = rm[(size_t)i * qim.y + g];
+          double zr, zi;
+          if (qim.w == 1) {  // Z = rho pf1
+            zr = rv.x * pp.x - rv.y * pp.y;
+            zi = rv.x * pp.y + rv.y * pp.x;
+          } else {           // Z = conj(rho) pf2
+            zr = rv.x * pp.x + rv.y * pp.y;
+            zi = rv.x * pp.y - rv.y * pp.x;
+          }
+          sre[i] += zr * yr - zi * yi;
+          sim[i] += zr * yi + zi * yr;
+        }
       }
     }
   }
-  // funcw_ij = sum_a Z(a) Y(a, .) per member: per thread over its row, then
-  // over the wave (shuffles) and the waves (LDS)
+  // lanes of one vector (li) hold their rows' sums: over lk, then the waves
 #pragma unroll
-  for (int m = 0; m < M; ++m) {
-    if (m >= nm) break;  // uniform
-    const int q = gmem[mbeg + m];
-    const int4 qi = qinfo[q];
-    const long long mo = map_off[q];
-    const int trs = qi.w, ngpsi = qi.y;
-    const double2* r = rho + rho_off[q];
-    const int g0 = v0 ? minv[mo + t] : 0;
+  for (int i = 0; i < NV; ++i)
+    for (int off = 16; off < 64; off <<= 1) {
+      sre[i] += __shfl_xor(sre[i], off);
+      sim[i] += __shfl_xor(sim[i], off);
+    }
+  if (lk == 0) {
 #pragma unroll
-    for (int zi = 0; zi < NV; ++zi) {
-      // Z of vector zi at the thread's row
-      double zr = 0.0, zim = 0.0;
-      if (v0) {
-        const double2 rv = r[(size_t)zi * ngpsi + g0];
-        if (trs == 1) { const double2 p = pf1[mo + g0]; zr = rv.x * p.x - rv.y * p.y; zim = rv.x * p.y + rv.y * p.x; }
-        else { const double2 p = pf2[mo + g0]; zr = rv.x * p.x + rv.y * p.y; zim = rv.x * p.y - rv.y * p.x; }
-      }
-#pragma unroll
-      for (int yc = 0; yc < NV; ++yc) {
-        const int c = m * NV + yc;
-        double sr = zr * yr[c] - zim * yi[c];
-        double si = zr * yi[c] + zim * yr[c];
-        for (int off = 32; off > 0; off >>= 1) {
-          sr += __shfl_xor(sr, off);
-          si += __shfl_xor(si, off);
-        }
-        // pair slot (i + NV j) of the member: trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
-        const int slot = m * NV * NV + (trs == 1 ? zi + NV * yc : yc + NV * zi);
-        if (lane == 0) {
-          red[slot][wave][0] = sr;
-          red[slot][wave][1] = si;
-        }
-      }
+    for (int i = 0; i < NV; ++i) {
+      red[li][i][wave][0] = sre[i];
+      red[li][i][wave][1] = sim[i];
     }
   }
   __syncthreads();
-  for (int e = t; e < 2 * nm * NV * NV; e += nt) {
-    const int slot = e >> 1, part = e & 1;
-    const int m = slot / (NV * NV), pr = slot % (NV * NV);
-    const int i = pr % NV, j = pr / NV;
+  for (int e = t; e < 2 * nvt * NV; e += blockDim.x) {
+    const int part = e & 1, idx = e >> 1;
+    const int c = idx / NV, zi = idx % NV;
+    const int m = c / NV, yc = c % NV;
     const int q = gmem[mbeg + m];
+    // trs = +1 pairs Z_i Y_j, trs = -1 pairs Z_j Y_i
+    const bool tp = qinfo[q].w == 1;
+    const int i = tp ? zi : yc, j = tp ? yc : zi;
     double sum = 0.0;
 #pragma unroll
-    for (int wv = 0; wv < kWaves; ++wv)
-      if (wv < nwv) sum += red[slot][wv][part];
+    for (int wv = 0; wv < 8; ++wv) sum += red[c][zi][wv][part];
     const double2 sh = shell[((size_t)q * NV + j) * NV + i];
     double2* o = funcw + (((size_t)q * nen + w) * NV + j) * NV + i;
     if (part == 0) o->x = sum + sh.x;
@@ -238,7 +240,6 @@ __global__ void __launch_bounds__(kCT) k_w_contract(
 // l supplies A(l % 16, l / 16) and B(l / 16, l % 16) and holds
 // D(l / 16 + 4 v, l % 16), v < 4 (checked on the GPU with row- and column-numbered
 // operands).  grid: (nen, ntq, groups), 512 threads.
-typedef double jd4 __attribute__((ext_vector_type(4)));
 constexpr int kJA = 1;
 __global__ void __launch_bounds__(256) k_j_xvec(const double2* __restrict__ rho,
                                                 const long long* __restrict__ rho_off,
@@ -515,13 +516,12 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
   RP_CHECK(max_nge >= 1 && max_nge <= kCT, "W contraction: eps sphere too large");
   if (ngroups <= 0) return;
-  const int nt = (max_nge + 63) / 64 * 64;  // one row per thread
   hipLaunchKernelGGL(k_w_xvec, dim3(ngroups), dim3(256), 0, s, rho, rho_off, qinfo, minv, pf1,
                      pf2, map_off, goff, gmem, nwf, w_contract_vectors(nwf), xw);
   switch (nwf) {
 #define RP_WC(NV)                                                                             \
   case NV:                                                                                    \
-    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ngroups), dim3(nt), 0, s, eps, eps_off, \
+    hipLaunchKernelGGL((k_w_contract<NV>), dim3(nen, ngroups), dim3(512), 0, s, eps, eps_off, \
                        rho, rho_off, qinfo, minv, pf1, pf2, map_off, shell, goff, gmem, xw,   \
                        nen, funcw);                                                           \
     break;
