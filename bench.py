@@ -425,8 +425,10 @@ def calc_int_bench(args) -> int:
     rp_j3d_run for R = 0 (X and J of one lattice vector).
 
     Also reported: device time per stage (events on the library's stream), the
-    HBM rooflines of the W and J contractions (bytes per (q, w) = 8 NG_eps^2
-    + 16 NG_psi NV, NV the vectors: NWF for W, NWF^2 pair vectors for J), V
+    HBM rooflines of the W and J contractions (bytes per w: 8 NG_eps^2 per
+    eps^-1 read, once per group of q sharing an irreducible eps^-1 for W and
+    once per q for J, + 16 NG_psi NV per q, NV the vectors: NWF for W, NWF^2
+    pair vectors for J), V
     and W against the oracle over every q in meV (calc_intW.F:942-985),
     X_ii(0) - V_ii(0) and J_ii(0, w) - W_ii(0, w) in meV, and the oracle's
     calc_w3d timed on the host as the CPU baseline."""
@@ -495,15 +497,19 @@ def calc_int_bench(args) -> int:
                                   "functions, NG_eps ~600, nen 32",
                       "ntk": info["ntk"], "ntq": ntq, "nwf": nwf, "nen": nen,
                       "nq_irr": int(len(ng)), "ng_eps": [int(ng.min()), int(ng.max())]}}
-    # HBM rooflines of the two contractions: eps^-1 of each full-mesh q read
-    # once per frequency (NG_eps of a full-mesh q is its irreducible one's),
-    # plus the vectors
+    # HBM rooflines of the two contractions: per frequency, eps^-1 of every
+    # full-mesh q (J) or of every group of up to 16 // NWF q sharing one
+    # irreducible eps^-1 (W), read once, plus the vectors
     from oracle import pipeline_w3d
     cset = pipeline_w3d.setup_w3d(w, W, sqi, lambda iq: eps[iq], em, wl["ecut_eps"])
     nge = np.array([cset["nge"][iq] for iq in range(ntq)], np.float64)
     ngp = np.array([cset["ngp"][iq] for iq in range(ntq)], np.float64)
-    for name, nv, key in (("k_w_contract", nwf, "w_contract"), ("k_j_contract", nwf * nwf, "j_contract")):
-        byt = float(np.sum(nen * (8.0 * nge ** 2 + 16.0 * ngp * nv)))
+    star = np.bincount(np.asarray(cset["numirrq"][:ntq]), minlength=len(ng))
+    ngroups = np.ceil(star / (16 // nwf))
+    e_w = float(np.sum(ngroups * 8.0 * np.asarray(ng, np.float64) ** 2))
+    for name, nv, key, e_bytes in (("k_w_contract", nwf, "w_contract", e_w),
+                                   ("k_j_contract", nwf * nwf, "j_contract", float(np.sum(8.0 * nge ** 2)))):
+        byt = float(nen * (e_bytes + np.sum(16.0 * ngp * nv)))
         out.setdefault("rooflines", {})[name] = {
             "bound": "hbm", "achieved": byt / stage[key] / 1e9, "peak": 8000.0, "unit": "GB/s",
             "frac": byt / stage[key] / 8.0e12, "algorithmic_bytes_per_launch": byt,
