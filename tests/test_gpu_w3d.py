@@ -102,3 +102,49 @@ def test_six_wannier_functions():
     assert relf(funcw, o["funcw"]) <= 1e-6 and relf(wm, o["wmat"]) <= 1e-6
     assert relf(x, oj["X"]) <= 1e-6 and relf(j, jo) <= 1e-6
     assert np.abs(j - jo).max() * AU <= 1e-3
+
+
+def test_w3d_wannier_subset_on_a_large_mesh():
+    """The pair sums, W and J of two Wannier functions do not depend on which
+    others are present.  On an 8^3 mesh (NTK 512) eight Wannier functions leave
+    room in LDS for the band grids of at most four grid points per workgroup of
+    the pair sums, J's sums run in two groups of jw, and the W contraction takes
+    two q per group; the first two alone fit sixteen points and take up to
+    eight q per group.  Same arithmetic per (q, pair, point) and per vector
+    column, so the two runs agree to rounding."""
+    wl = dict(gen.WORKLOADS["w3d_tiny"])
+    wl["nk"] = 8
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    from respackdev_amd.chiqw import from_system
+    w = from_system(s)
+    wan, _ = gen.make_wannier(s, 8, 5)
+    ntg = max(c.shape[1] for c in wan)
+    W8 = np.zeros((len(wan), 8, ntg), np.complex128)
+    for k, c in enumerate(wan):
+        W8[k, :, :c.shape[1]] = c
+    em = np.linspace(0.0, 1.0, 4) + 0.01j
+    sqi = np.array([[gen._fold(x) for x in k] for k in w.ski])
+    rng = np.random.default_rng(3)
+    out = {}
+    for nwf in (8, 2):
+        g = W3d(0)
+        try:
+            g.open(w, wl["ecut_eps"], em)
+            g.set_wannier(np.ascontiguousarray(W8[:, :nwf]))
+            ng = g.set_q(sqi)
+            if not out:
+                eps = [np.eye(n)[None] + 0.05 * (rng.standard_normal((len(em), n, n)) +
+                                                1j * rng.standard_normal((len(em), n, n)))
+                       for n in ng]
+            for iq, e in enumerate(eps):
+                g.set_epsinv(iq, e)
+            assert g.info()["ntq"] == 512
+            func, funcw = g.funcs()
+            x, j, _, _ = g.j3d([(0, 0, 0), (1, 0, -1)])
+        finally:
+            g.close()
+        out[nwf] = (func[:, :2, :2], funcw[:, :, :2, :2], x[:, :2, :2], j[:, :, :2, :2])
+    for a, b, name in zip(out[8], out[2], ("func", "funcw", "X", "J")):
+        err = relf(a, b)
+        print(f"{name}: {err:.2e}", flush=True)
+        assert err <= 1e-12, name
