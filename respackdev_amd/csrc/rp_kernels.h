@@ -128,6 +128,9 @@ void launch_gauss_jordan(double2* a, int n, int nbatch, int* ipiv, int* work, in
 
 // rp_w3d.hip
 int w3d_max_wannier();
+// rows of a column of eps^-1 in HBM: NG_eps rounded up to 16, so that every
+// 16-row block of the contractions starts on a 128-byte line
+__host__ __device__ inline int w3d_eps_ld(int nge) { return (nge + 15) / 16 * 16; }
 int w3d_max_eps_g();
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s);

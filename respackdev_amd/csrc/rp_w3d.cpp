@@ -35,7 +35,8 @@ struct W3d {
   std::vector<int> numirrq, numrotq, trsq, rwq, nge, ngpsi;
   std::vector<std::vector<int>> lg0;
   // eps^-1 of every irreducible q lives in HBM only (d_eps, complex(4) as the
-  // reference's epsirr, (ig,jg,ie) at eps_off[iq]); the host keeps which q
+  // reference's epsirr, (ig,jg,ie) at eps_off[iq], columns padded to
+  // w3d_eps_ld(NG_eps) rows); the host keeps which q
   // are set and the Gamma head element eps^-1_00(w) for the Hybertsen-Louie term
   std::vector<char> eps_have;
   std::vector<float2> eps_head;
@@ -569,7 +570,7 @@ int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps
     size_t tot = 0;
     for (int iq = 0; iq < nq_irr; ++iq) {
       w.eps_off[iq] = (long long)tot;
-      tot += (size_t)w.nge[iq] * w.nge[iq] * w.nen;
+      tot += (size_t)w3d_eps_ld(w.nge[iq]) * w.nge[iq] * w.nen;
     }
     RP_HIP(hipSetDevice(h->device));
     w.d_eps.reserve(tot);
@@ -587,13 +588,18 @@ int rp_w3d_set_epsinv(rp_handle h, int32_t iq, const double* epsinv) {
     W3d& w = w3d_of(h, "rp_w3d_set_epsinv");
     RP_CHECK(w.have_q, "rp_w3d_set_epsinv: q list not set (rp_w3d_set_q)");
     RP_CHECK(iq >= 0 && iq < w.nq_irr && epsinv, "rp_w3d_set_epsinv: bad arguments");
-    const size_t ng = w.nge[iq], n = ng * ng * w.nen;
-    // epsirr is complex(4) in the reference (calc_intW.F:586-630)
-    std::vector<float2> e(n);
-    for (size_t i = 0; i < n; ++i) e[i] = make_float2((float)epsinv[2 * i], (float)epsinv[2 * i + 1]);
+    const size_t ng = w.nge[iq], ld = (size_t)w3d_eps_ld((int)ng), n = ld * ng * w.nen;
+    // epsirr is complex(4) in the reference (calc_intW.F:586-630); columns
+    // padded with zeros to ld rows
+    std::vector<float2> e(n, make_float2(0.f, 0.f));
+    for (size_t c = 0; c < ng * w.nen; ++c)
+      for (size_t r = 0; r < ng; ++r) {
+        const size_t i = r + ng * c;
+        e[r + ld * c] = make_float2((float)epsinv[2 * i], (float)epsinv[2 * i + 1]);
+      }
     if (iq == w.iqgm)
       for (int ie = 0; ie < w.nen; ++ie)
-        w.eps_head[ie] = e[(size_t)w.nog0 + ng * ((size_t)w.nog0 + ng * ie)];
+        w.eps_head[ie] = e[(size_t)w.nog0 + ld * ((size_t)w.nog0 + ng * ie)];
     RP_HIP(hipSetDevice(h->device));
     RP_HIP(hipMemcpy(w.d_eps.p + w.eps_off[iq], e.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
     w.eps_have[iq] = 1;

@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(512) k_w_contract(
   const int q0m = gmem[mbeg];
   const int4 qi0 = qinfo[q0m];
   const int nge = qi0.x, iqir = qi0.z;
-  const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
+  const int ld = w3d_eps_ld(nge);
+  const float2* E = eps + eps_off[iqir] + (size_t)w * ld * nge;
   const double2* Xg = xw + map_off[q0m] * kWcVec;  // (b, c) at Xg[b * 16 + c]
   __shared__ double red[kWcVec][NV][8][2];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -154,16 +155,15 @@ __global__ void __launch_bounds__(512) k_w_contract(
   for (int i = 0; i < NV; ++i) sre[i] = sim[i] = 0.0;
   for (int ab = wave; ab < nab; ab += 8) {
     jd4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    // rows past nge read row nge - 1; their sums are never used
-    const int a = ab * 16 + li;
-    const int arow = a < nge ? a : nge - 1;
+    // rows nge .. ld-1 are the zero padding; their sums are never used
+    const int arow = ab * 16 + li;
     for (int b0 = 0; b0 < nge; b0 += 4) {
       const int b = b0 + lk;
       const bool bv = b < nge;  // columns past nge contribute zero
       const int bc = bv ? b : nge - 1;
       const double2 xv = Xg[(size_t)bc * kWcVec + li];
       const double xr = bv ? xv.x : 0.0, xi = bv ? xv.y : 0.0, nxi = -xi;
-      const float2 ev = E[(size_t)bc * nge + arow];
+      const float2 ev = E[(size_t)bc * ld + arow];
       const double er = ev.x, ei = ev.y;
       accr = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr, accr, 0, 0, 0);
       accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi, accr, 0, 0, 0);
@@ -282,7 +282,8 @@ __global__ void __launch_bounds__(512) k_j_contract(
   const int w = blockIdx.x, q = blockIdx.y, grp = blockIdx.z;
   const int4 qi = qinfo[q];
   const int nge = qi.x, ngpsi = qi.y, iqir = qi.z, trs = qi.w;
-  const float2* E = eps + eps_off[iqir] + (size_t)w * nge * nge;
+  const int ld = w3d_eps_ld(nge);
+  const float2* E = eps + eps_off[iqir] + (size_t)w * ld * nge;
   const int c0 = grp * 32, nvb = min(32, nv - c0);
   const double2* Xq = xg + map_off[q] * g32 + c0;  // (b, c) at Xq[b * g32 + c]
   const double2* r = rho + rho_off[q] + (size_t)c0 * ngpsi;
@@ -299,9 +300,10 @@ __global__ void __launch_bounds__(512) k_j_contract(
     int arow[kJA];
 #pragma unroll
     for (int j = 0; j < kJA; ++j) {
-      // rows past nge read row nge - 1; their sums are never used
+      // rows nge .. ld-1 are the zero padding, rows past ld (row blocks past
+      // the last) read row 0; their sums are never used
       const int a = (a0 + wave + 8 * j) * 16 + li;
-      arow[j] = a < nge ? a : nge - 1;
+      arow[j] = a < ld ? a : 0;
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) accr[j][cb] = acci[j][cb] = jd4{0.0, 0.0, 0.0, 0.0};
     }
@@ -319,7 +321,7 @@ __global__ void __launch_bounds__(512) k_j_contract(
       }
 #pragma unroll
       for (int j = 0; j < kJA; ++j) {
-        const float2 ev = E[(size_t)bc * nge + arow[j]];
+        const float2 ev = E[(size_t)bc * ld + arow[j]];
         const double er = ev.x, ei = ev.y;
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
