@@ -131,7 +131,6 @@ int w3d_max_wannier();
 // rows of a column of eps^-1 in HBM: NG_eps rounded up to 16, so that every
 // 16-row block of the contractions starts on a 128-byte line
 __host__ __device__ inline int w3d_eps_ld(int nge) { return (nge + 15) / 16 * 16; }
-int w3d_max_eps_g();
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s);
 // q per group of launch_w_contract (q sharing one irreducible eps^-1), and

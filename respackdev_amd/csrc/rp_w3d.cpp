@@ -614,13 +614,6 @@ int rp_w3d_funcs(rp_handle h, double* func_out, double* funcw_out) {
     RP_HIP(hipSetDevice(h->device));
     hipStream_t s = h->stream;
     const int ntk = w.ntk, nwf = w.nwf, nen = w.nen;
-    // the W contraction holds two rows of eps^-1 per thread (the J contraction
-    // has no such limit)
-    for (int iq = 0; iq < w.ntq; ++iq)
-      RP_CHECK(w.nge[iq] <= w3d_max_eps_g(), "rp_w3d_funcs: eps sphere of " +
-                                                 std::to_string(w.nge[iq]) + " G exceeds the " +
-                                                 std::to_string(w3d_max_eps_g()) +
-                                                 " of the W contraction kernel");
     upload_eps_maps(w, s);
     w.d_rho_off.upload(w.rho_off, s);
     w.d_rho.reserve((size_t)(w.rho_off.back() + (long long)w.ngpsi.back() * w3d_max_wannier()));

@@ -80,7 +80,6 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // grid: (nen, ngroups), 512 threads; group g has the q gmem[goff[g] .. goff[g+1]).
 typedef double jd4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
-constexpr int kCT = 1024;      // rows of E the W contraction takes (NG_eps)
 constexpr int kWcVec = 16;     // vectors of a group: one MFMA column block
 template <int NV>
 constexpr int wc_members() { return kWcVec / NV; }
@@ -501,7 +500,6 @@ __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ fu
 }
 
 int w3d_max_wannier() { return kMaxWf; }
-int w3d_max_eps_g() { return kCT; }
 
 void launch_w_func(const double2* rho, const long long* rho_off, const int4* qinfo, int ntq,
                    int nwf, double2* func, double2* shell, hipStream_t s) {
@@ -516,7 +514,7 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        int nwf, int nen, int max_nge, double2* xw, double2* funcw,
                        hipStream_t s) {
   RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
-  RP_CHECK(max_nge >= 1 && max_nge <= kCT, "W contraction: eps sphere too large");
+  RP_CHECK(max_nge >= 1, "W contraction: empty eps sphere");
   if (ngroups <= 0) return;
   hipLaunchKernelGGL(k_w_xvec, dim3(ngroups), dim3(256), 0, s, rho, rho_off, qinfo, minv, pf1,
                      pf2, map_off, goff, gmem, nwf, w_contract_vectors(nwf), xw);

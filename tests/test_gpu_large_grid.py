@@ -11,16 +11,15 @@ Reduced band set and k mesh (2^3 k, 2 occupied + 3 empty bands, ~4700 plane
 waves): FP64 chi and eps^-1 against the oracle within 1e-6; calc_w3d's func
 and funcw (two model Wannier functions, random eps^-1) within 1e-6.
 
-The same cell with an eps cutoff of 32 Ry (NG_eps 1166-1189): calc_j3d's X and
-J against the oracle within 1e-6 (the J contraction has no row limit), and
-calc_w3d's contraction, which holds at most 1024 rows, refuses it.
+The same cell with an eps cutoff of 32 Ry (NG_eps 1088-1189): calc_j3d's X and
+J and calc_w3d's func and funcw against the oracle within 1e-6 (neither
+contraction has a row limit: both take 16-row blocks of eps^-1 per pass).
 """
 import numpy as np
 import pytest
 
 import oracle
 from respackdev_amd import gen
-from respackdev_amd._lib import RpError
 from respackdev_amd.chiqw import Chiqw, from_system
 from respackdev_amd.w3d import AU, W3d
 
@@ -101,13 +100,14 @@ def test_w3d_on_a_24_cubed_grid(srvo3_sized):
     assert ef <= 1e-6 and ew <= 1e-6
 
 
-def test_j3d_with_more_than_1024_eps_rows(srvo3_sized):
+def test_w3d_and_j3d_with_more_than_1024_eps_rows(srvo3_sized):
     from oracle import pipeline_w3d
     nwf, nen, ecut_eps = 2, 2, 32.0
     rlist = [(0, 0, 0), (1, 0, 0)]
     w, W, em, sqi, eps = _w3d_inputs(srvo3_sized, nwf, nen, ecut_eps, 5)
     assert min(e.shape[-1] for e in eps) > 1024
     oj = pipeline_w3d.run_j3d(w, W, sqi, eps, em, ecut_eps, rlist)
+    ow = pipeline_w3d.run_w3d(w, W, sqi, eps, em, ecut_eps)
     g = W3d(0)
     try:
         g.open(w, ecut_eps, em)
@@ -116,12 +116,13 @@ def test_j3d_with_more_than_1024_eps_rows(srvo3_sized):
         for iq, e in enumerate(eps):
             g.set_epsinv(iq, e)
         x, j, _, _ = g.j3d(rlist)
-        with pytest.raises(RpError, match="exceeds the 1024"):
-            g.funcs()
+        func, funcw = g.funcs()
     finally:
         g.close()
     jo = np.moveaxis(oj["J"], 1, 0)  # [ie][r][j][i]
     ex, ej = relf(x, oj["X"]), relf(j, jo)
-    print(f"j3d at NG_eps {max(e.shape[-1] for e in eps)}: X {ex:.3e} J {ej:.3e}", flush=True)
-    assert ex <= 1e-6 and ej <= 1e-6
+    ef, ew = relf(func, ow["func"]), relf(funcw, ow["funcw"])
+    print(f"NG_eps {max(e.shape[-1] for e in eps)}: X {ex:.3e} J {ej:.3e} func {ef:.3e} "
+          f"funcw {ew:.3e}", flush=True)
+    assert ex <= 1e-6 and ej <= 1e-6 and ef <= 1e-6 and ew <= 1e-6
     assert np.abs(j - jo).max() * AU <= 1e-3
