@@ -476,12 +476,13 @@ __global__ void __launch_bounds__(256) k_pair_densities_pruned(
 //
 // k_pair_rsum, grid (ceil(nr/256), nq, ngroups), 256 threads, one grid point r
 // per thread:
-//   FULL = false: pairs p = w (iw = jw = w), w < nwf <= 8, one group;
+//   FULL = false: pairs p = w (iw = jw = w), w < nwf <= NI, one group;
 //   FULL = true:  pairs p = iw + nwf*jw, group z takes jw in {2z, 2z + 1}.
+// NI (8 or 16) bounds the Wannier functions held per thread.
 // P[q][p][r] complex FP64.  ikq[q][k], shift[q][k][3] (the umklapp vectors),
 // wk[k] (may be null: 1).
 constexpr int kRsumJB = 2;
-template <bool FULL>
+template <bool FULL, int NI>
 __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur, int ntk, int nwf,
                                                    int nx, int ny, int nz, const int* __restrict__ ikq,
                                                    const int* __restrict__ shift,
@@ -504,12 +505,12 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
   const int npair = FULL ? nwf * nwf : nwf;
   if (r >= nr) return;
   const int x = r % nx, y = (r / nx) % ny, zz = r / (nx * ny);
-  constexpr int NA = FULL ? kRsumJB : 8;  // accumulator rows (jw, or w on the diagonal)
-  double2 acc[NA][FULL ? 8 : 1];
+  constexpr int NA = FULL ? kRsumJB : NI;  // accumulator rows (jw, or w on the diagonal)
+  double2 acc[NA][FULL ? NI : 1];
 #pragma unroll
   for (int a = 0; a < NA; ++a)
 #pragma unroll
-    for (int b = 0; b < (FULL ? 8 : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
+    for (int b = 0; b < (FULL ? NI : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
   const int* kq_of = ikq + (size_t)q * ntk;
   const int* sh_of = shift + (size_t)3 * q * ntk;
   for (int k = 0; k < ntk; ++k) {
@@ -525,9 +526,9 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
     }
     const double2 wv = wk ? wk[k] : make_double2(1.0, 0.0);
     // b_iw = u_iw,k+q(r) e^{-i 2pi r.dG/N} (FP64 product of the single-precision grid)
-    double2 bq[8];
+    double2 bq[NI];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NI; ++i) {
       bq[i] = make_double2(0.0, 0.0);
       if (i < nwf) {
         const float2 v = ur[((size_t)i * ntk + kq) * nr + r];
@@ -545,7 +546,7 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
                                       (double)v.x * wv.y + (double)v.y * wv.x);
       if constexpr (FULL) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {  // conj(b_iw) (w_k a_jw)
+        for (int i = 0; i < NI; ++i) {  // conj(b_iw) (w_k a_jw)
           acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
           acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
         }
@@ -563,7 +564,7 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
     if (jw >= nwf) continue;
     if constexpr (FULL) {
 #pragma unroll
-      for (int i = 0; i < 8; ++i)
+      for (int i = 0; i < NI; ++i)
         if (i < nwf) out[(size_t)(i + nwf * jw) * nr] = acc[a][i];
     } else {
       out[(size_t)jw * nr] = acc[a][0];
@@ -582,13 +583,15 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
 // of its workgroup's range of qper q.  Dynamic LDS: the phase tables, w_k
 // (ntk) and the grids [w][k][TR] (single precision, as in HBM).
 constexpr int kRsumTileThreads = 512;
-constexpr int kRsumTileJB = 5;  // jw per group in the full mode
+// jw per group in the full mode: the accumulators of JB jw x NI iw
+template <int NI>
+constexpr int rsum_tile_jb() { return NI <= 8 ? 5 : 2; }
 __device__ __forceinline__ int rsum_wrap(int m, int n) {
   if (m >= n || m < -n) m %= n;  // only for umklapp vectors beyond +-1
   return m < 0 ? m + n : m;
 }
 constexpr int kRsumPF = 8;  // k of packed indices loaded ahead
-template <bool FULL, int TR>
+template <bool FULL, int TR, int NI>
 __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
     const float2* __restrict__ ur, int ntk, int nwf, int nx, int ny, int nz,
     const unsigned* __restrict__ kqs, const double2* __restrict__ wk, int nq, int qper,
@@ -621,14 +624,15 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
   if (r >= nr) return;  // after the only barrier
   const int x = r % nx, y = (r / nx) % ny, zz = r / (nx * ny);
   const int npair = FULL ? nwf * nwf : nwf;
-  constexpr int NA = FULL ? kRsumTileJB : 8;
+  constexpr int JB = rsum_tile_jb<NI>();
+  constexpr int NA = FULL ? JB : NI;
   const int qend = min(nq, (int)(blockIdx.z + 1) * qper);
   for (int q = blockIdx.z * qper + ql; q < qend; q += QL) {
-    double2 acc[NA][FULL ? 8 : 1];
+    double2 acc[NA][FULL ? NI : 1];
 #pragma unroll
     for (int a = 0; a < NA; ++a)
 #pragma unroll
-      for (int b = 0; b < (FULL ? 8 : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
+      for (int b = 0; b < (FULL ? NI : 1); ++b) acc[a][b] = make_double2(0.0, 0.0);
     // packed (k+q, umklapp) of every k, kRsumPF k ahead of their use
     const unsigned* kq_row = kqs + (size_t)q * ntk;
     unsigned nxt[kRsumPF];
@@ -661,9 +665,9 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
           // conj(u_iw,k+q ph) (w_k u_jw,k) = conj(u_iw,k+q) (conj(ph) w_k u_jw,k): the
           // phase goes with the jw factor, once per jw instead of once per iw
           const double2 cw = make_double2(ph.x * wv.x + ph.y * wv.y, ph.x * wv.y - ph.y * wv.x);
-          double2 bq[8];
+          double2 bq[NI];
 #pragma unroll
-          for (int i = 0; i < 8; ++i) {
+          for (int i = 0; i < NI; ++i) {
             bq[i] = make_double2(0.0, 0.0);
             if (i < nwf) {
               const float2 v = su[(i * ntk + kq) * TR + rl];
@@ -672,13 +676,13 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
           }
 #pragma unroll
           for (int a = 0; a < NA; ++a) {
-            const int jw = z * kRsumTileJB + a;
+            const int jw = z * JB + a;
             if (jw >= nwf) continue;
             const float2 v = su[(jw * ntk + k) * TR + rl];
             const double2 aw = make_double2((double)v.x * cw.x - (double)v.y * cw.y,
                                             (double)v.x * cw.y + (double)v.y * cw.x);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < NI; ++i) {
               acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
               acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
             }
@@ -703,11 +707,11 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
     double2* out = P + (size_t)q * npair * nr + r;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
-      const int jw = FULL ? z * kRsumTileJB + a : a;
+      const int jw = FULL ? z * JB + a : a;
       if (jw >= nwf) continue;
       if constexpr (FULL) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
+        for (int i = 0; i < NI; ++i)
           if (i < nwf) out[(size_t)(i + nwf * jw) * nr] = acc[a][i];
       } else {
         out[(size_t)jw * nr] = acc[a][0];
@@ -923,14 +927,16 @@ void launch_pair_densities_pruned(const float2* ur, int ntk, const int* pocc, co
 void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int ny, int nz,
                       const int* ikq, const int* shift, const unsigned* kqs, const double2* wk,
                       int nq, double2* P, hipStream_t s) {
-  RP_CHECK(nwf >= 1 && nwf <= 8, "Wannier pair sums take at most 8 Wannier functions");
+  RP_CHECK(nwf >= 1 && nwf <= 16, "Wannier pair sums take at most 16 Wannier functions");
   RP_CHECK(nx <= kTwMax && ny <= kTwMax && nz <= kTwMax, "FFT axis too long for the phase tables");
   const int nr = nx * ny * nz;
   if (nq <= 0) return;
+  const bool wide = nwf > 8;  // NI = 16 forms
+  const int jb = wide ? rsum_tile_jb<16>() : rsum_tile_jb<8>();
   // The LDS-tiled form when the packed indices exist and the grids of one
   // point fit (NWF * NTK <= 20000).  Tile size and q split: least (rounds of
   // workgroups) x (q per thread), with the workgroups a compute unit holds by
-  // LDS and by registers (two of the diagonal form, one of the full form).
+  // LDS and by registers (two of the diagonal form with NI = 8, else one).
   const size_t fixed = sizeof(double2) * (3 * kTwMax + (size_t)ntk);
   const size_t per_point = sizeof(float2) * (size_t)nwf * ntk;
   int tr = 0, qper = nq;
@@ -938,12 +944,12 @@ void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int
     int dev = 0, ncu = 256;
     RP_HIP(hipGetDevice(&dev));
     RP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int groups = full ? (nwf + kRsumTileJB - 1) / kRsumTileJB : 1;
+    const int groups = full ? (nwf + jb - 1) / jb : 1;
     long long best = -1;
     for (int c : {16, 8, 4, 2, 1}) {
       const size_t lds = fixed + per_point * c;
       if (lds > kLdsBytes) continue;
-      const int per_cu = std::min<int>((int)(kLdsBytes / lds), full ? 1 : 2);
+      const int per_cu = std::min<int>((int)(kLdsBytes / lds), (full || wide) ? 1 : 2);
       const int tiles = (nr + c - 1) / c, ql = kRsumTileThreads / c;
       for (int split = 1; split <= 8; ++split) {
         const int qp = (nq + split - 1) / split;
@@ -956,7 +962,7 @@ void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int
     }
   }
   if (tr) {
-    const int groups = full ? (nwf + kRsumTileJB - 1) / kRsumTileJB : 1;
+    const int groups = full ? (nwf + jb - 1) / jb : 1;
     const int tiles = (nr + tr - 1) / tr;
     const int nsplit = (nq + qper - 1) / qper;
     const size_t lds = fixed + per_point * tr;
@@ -966,27 +972,39 @@ void launch_pair_rsum(const float2* ur, int ntk, int nwf, bool full, int nx, int
       hipLaunchKernelGGL(fn, grid, dim3(kRsumTileThreads), lds, s, ur, ntk, nwf, nx, ny, nz, kqs,
                          wk, nq, qper, P);
     };
-    auto pick = [&](auto full_tag) {
+    auto pick = [&](auto full_tag, auto ni_tag) {
       constexpr bool F = decltype(full_tag)::value;
-      if (tr == 16) go(k_pair_rsum_tile<F, 16>);
-      else if (tr == 8) go(k_pair_rsum_tile<F, 8>);
-      else if (tr == 4) go(k_pair_rsum_tile<F, 4>);
-      else if (tr == 2) go(k_pair_rsum_tile<F, 2>);
-      else go(k_pair_rsum_tile<F, 1>);
+      constexpr int NI = decltype(ni_tag)::value;
+      if (tr == 16) go(k_pair_rsum_tile<F, 16, NI>);
+      else if (tr == 8) go(k_pair_rsum_tile<F, 8, NI>);
+      else if (tr == 4) go(k_pair_rsum_tile<F, 4, NI>);
+      else if (tr == 2) go(k_pair_rsum_tile<F, 2, NI>);
+      else go(k_pair_rsum_tile<F, 1, NI>);
     };
-    if (full) pick(std::true_type{});
-    else pick(std::false_type{});
+    using N8 = std::integral_constant<int, 8>;
+    using N16 = std::integral_constant<int, 16>;
+    if (full) {
+      if (wide) pick(std::true_type{}, N16{});
+      else pick(std::true_type{}, N8{});
+    } else {
+      if (wide) pick(std::false_type{}, N16{});
+      else pick(std::false_type{}, N8{});
+    }
     RP_HIP(hipGetLastError());
     return;
   }
   const int groups = full ? (nwf + kRsumJB - 1) / kRsumJB : 1;
   dim3 grid((nr + 255) / 256, nq, groups);
-  if (full)
-    hipLaunchKernelGGL(k_pair_rsum<true>, grid, dim3(256), 0, s, ur, ntk, nwf, nx, ny, nz, ikq,
-                       shift, wk, P);
-  else
-    hipLaunchKernelGGL(k_pair_rsum<false>, grid, dim3(256), 0, s, ur, ntk, nwf, nx, ny, nz, ikq,
-                       shift, wk, P);
+  auto direct = [&](auto fn) {
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, ur, ntk, nwf, nx, ny, nz, ikq, shift, wk, P);
+  };
+  if (full) {
+    if (wide) direct(k_pair_rsum<true, 16>);
+    else direct(k_pair_rsum<true, 8>);
+  } else {
+    if (wide) direct(k_pair_rsum<false, 16>);
+    else direct(k_pair_rsum<false, 8>);
+  }
   RP_HIP(hipGetLastError());
 }
 

@@ -65,7 +65,7 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // funcw = sum_a Z(a) Y(a,.) + shell, where
 //   trs = +1: X_j(b) = conj(rho_j(m^-1 b)) pf2, Z = rho_i(m^-1 a) pf1
 //   trs = -1: X_i(b) = rho_i(m^-1 b) pf1,       Z = conj(rho_j(m^-1 a)) pf2
-// with the member's own map m and phases.  The vectors are the NV <= 8 Wannier
+// with the member's own map m and phases.  The vectors are the NV <= 16 Wannier
 // functions of calc_w3d per member, 16 / NV members per group at most, and
 // every (i,j) of a member is formed, out [q][w][j][i] (calc_j3d's diagonal
 // pairs: k_j_contract).  A workgroup takes all the vectors of its group, so E
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 // c and the waves at the end.
 // grid: (nen, ngroups), 512 threads; group g has the q gmem[goff[g] .. goff[g+1]).
 typedef double jd4 __attribute__((ext_vector_type(4)));
-constexpr int kMaxWf = 8;      // Wannier functions of calc_w3d
+constexpr int kMaxWf = 16;     // Wannier functions of calc_w3d and calc_j3d
 constexpr int kWcVec = 16;     // vectors of a group: one MFMA column block
 template <int NV>
 constexpr int wc_members() { return kWcVec / NV; }
@@ -513,7 +513,7 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        const double2* shell, const int* goff, const int* gmem, int ngroups,
                        int nwf, int nen, int max_nge, double2* xw, double2* funcw,
                        hipStream_t s) {
-  RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..8 Wannier functions");
+  RP_CHECK(nwf >= 1 && nwf <= kMaxWf, "W contraction supports 1..16 Wannier functions");
   RP_CHECK(max_nge >= 1, "W contraction: empty eps sphere");
   if (ngroups <= 0) return;
   hipLaunchKernelGGL(k_w_xvec, dim3(ngroups), dim3(256), 0, s, rho, rho_off, qinfo, minv, pf1,
@@ -526,6 +526,7 @@ void launch_w_contract(const float2* eps, const long long* eps_off, const double
                        nen, funcw);                                                           \
     break;
     RP_WC(1) RP_WC(2) RP_WC(3) RP_WC(4) RP_WC(5) RP_WC(6) RP_WC(7) RP_WC(8)
+    RP_WC(9) RP_WC(10) RP_WC(11) RP_WC(12) RP_WC(13) RP_WC(14) RP_WC(15) RP_WC(16)
 #undef RP_WC
   }
   RP_HIP(hipGetLastError());

@@ -104,6 +104,42 @@ def test_six_wannier_functions():
     assert np.abs(j - jo).max() * AU <= 1e-3
 
 
+
+def test_twelve_wannier_functions():
+    """Twelve Wannier functions (two sets of six model functions with different
+    centres): the pair sums with sixteen iw per thread, J's 144 pair vectors
+    in five groups of 32 (the last partial), and W's contraction with one q
+    per group of twelve vectors."""
+    from oracle import pipeline_w3d
+    wl, s, w, W6, em, sqi, eps = _w3d_case("random", nwf=6, seed=13)
+    wan2, _ = gen.make_wannier(s, 6, 29)
+    W = np.zeros((W6.shape[0], 12, W6.shape[2]), np.complex128)
+    W[:, :6] = W6
+    for k, c in enumerate(wan2):
+        W[k, 6:, :c.shape[1]] = c
+    o = pipeline_w3d.run_w3d(w, W, sqi, eps, em, wl["ecut_eps"])
+    rlist = [(0, 0, 0)]
+    oj = pipeline_w3d.run_j3d(w, W, sqi, eps, em, wl["ecut_eps"], rlist)
+    g = W3d(0)
+    try:
+        g.open(w, wl["ecut_eps"], em)
+        g.set_wannier(W)
+        g.set_q(sqi)
+        for iq, e in enumerate(eps):
+            g.set_epsinv(iq, e)
+        func, funcw = g.funcs()
+        v, wm, _, _ = g.vw()
+        x, j, _, _ = g.j3d(rlist)
+    finally:
+        g.close()
+    jo = np.moveaxis(oj["J"], 1, 0)
+    print(f"nwf 12: func {relf(func, o['func']):.3e} funcw {relf(funcw, o['funcw']):.3e} "
+          f"W {relf(wm, o['wmat']):.3e} X {relf(x, oj['X']):.3e} J {relf(j, jo):.3e}", flush=True)
+    assert relf(func, o["func"]) <= 1e-6 and relf(funcw, o["funcw"]) <= 1e-6
+    assert relf(v, o["vmat"]) <= 1e-6 and relf(wm, o["wmat"]) <= 1e-6
+    assert relf(x, oj["X"]) <= 1e-6 and relf(j, jo) <= 1e-6
+    assert np.abs(wm - o["wmat"]).max() * AU <= 1e-3 and np.abs(j - jo).max() * AU <= 1e-3
+
 def test_w3d_wannier_subset_on_a_large_mesh():
     """The pair sums, W and J of two Wannier functions do not depend on which
     others are present.  On an 8^3 mesh (NTK 512) eight Wannier functions leave
