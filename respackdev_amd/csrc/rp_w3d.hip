@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 typedef double jd4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxWf = 16;     // Wannier functions of calc_w3d and calc_j3d
 constexpr int kWcVec = 16;     // vectors of a group: one MFMA column block
+constexpr int kWA = 4;         // row blocks per wave and pass of the W contraction
 template <int NV>
 constexpr int wc_members() { return kWcVec / NV; }
 int w_contract_members(int nwf) { return kWcVec / nwf; }
@@ -152,45 +153,62 @@ __global__ void __launch_bounds__(512) k_w_contract(
   double sre[NV], sim[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) sre[i] = sim[i] = 0.0;
-  for (int ab = wave; ab < nab; ab += 8) {
-    jd4 accr = {0.0, 0.0, 0.0, 0.0}, acci = {0.0, 0.0, 0.0, 0.0};
-    // rows nge .. ld-1 are the zero padding; their sums are never used
-    const int arow = ab * 16 + li;
+  // up to kWA row blocks per wave and pass (ab = a0 + wave + 8 j) share each
+  // load of X; blocks past the last are skipped (uniform per wave)
+  for (int a0 = 0; a0 < nab; a0 += 8 * kWA) {
+    jd4 accr[kWA], acci[kWA];
+    int arow[kWA];
+#pragma unroll
+    for (int j = 0; j < kWA; ++j) {
+      accr[j] = acci[j] = jd4{0.0, 0.0, 0.0, 0.0};
+      // rows nge .. ld-1 are the zero padding; their sums are never used
+      arow[j] = (a0 + wave + 8 * j) * 16 + li;
+    }
+    const int nj = min(kWA, (nab - a0 - wave + 7) / 8);  // live blocks of this wave
     for (int b0 = 0; b0 < nge; b0 += 4) {
       const int b = b0 + lk;
       const bool bv = b < nge;  // columns past nge contribute zero
       const int bc = bv ? b : nge - 1;
       const double2 xv = Xg[(size_t)bc * kWcVec + li];
       const double xr = bv ? xv.x : 0.0, xi = bv ? xv.y : 0.0, nxi = -xi;
-      const float2 ev = E[(size_t)bc * ld + arow];
-      const double er = ev.x, ei = ev.y;
-      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr, accr, 0, 0, 0);
-      accr = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi, accr, 0, 0, 0);
-      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xi, acci, 0, 0, 0);
-      acci = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, xr, acci, 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < kWA; ++j) {
+        if (j >= nj) break;  // uniform
+        const float2 ev = E[(size_t)bc * ld + arow[j]];
+        const double er = ev.x, ei = ev.y;
+        accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr, accr[j], 0, 0, 0);
+        accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi, accr[j], 0, 0, 0);
+        acci[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xi, acci[j], 0, 0, 0);
+        acci[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, xr, acci[j], 0, 0, 0);
+      }
     }
     // Z_i(a) Y(a, c) for this lane's rows a = 16 ab + lk + 4 v and vector c = li
     if (cv) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int ar = ab * 16 + lk + 4 * v;
-        if (ar >= nge) continue;
-        const int g = minv[mom + ar];
-        const double yr = accr[v], yi = acci[v];
-        const double2 pp = qim.w == 1 ? pf1[mom + g] : pf2[mom + g];
+      for (int j = 0; j < kWA; ++j) {
+        if (j >= nj) break;  // uniform
+        const int ab = a0 + wave + 8 * j;
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-          const double2 rv = rm[(size_t)i * qim.y + g];
-          double zr, zi;
-          if (qim.w == 1) {  // Z = rho pf1
-            zr = rv.x * pp.x - rv.y * pp.y;
-            zi = rv.x * pp.y + rv.y * pp.x;
-          } else {           // Z = conj(rho) pf2
-            zr = rv.x * pp.x + rv.y * pp.y;
-            zi = rv.x * pp.y - rv.y * pp.x;
+        for (int v = 0; v < 4; ++v) {
+          const int ar = ab * 16 + lk + 4 * v;
+          if (ar >= nge) continue;
+          const int g = minv[mom + ar];
+          const double yr = accr[j][v], yi = acci[j][v];
+          const double2 pp = qim.w == 1 ? pf1[mom + g] : pf2[mom + g];
+#pragma unroll
+          for (int i = 0; i < NV; ++i) {
+            const double2 rv = rm[(size_t)i * qim.y + g];
+            double zr, zi;
+            if (qim.w == 1) {  // Z = rho pf1
+              zr = rv.x * pp.x - rv.y * pp.y;
+              zi = rv.x * pp.y + rv.y * pp.x;
+            } else {           // Z = conj(rho) pf2
+              zr = rv.x * pp.x + rv.y * pp.y;
+              zi = rv.x * pp.y - rv.y * pp.x;
+            }
+            sre[i] += zr * yr - zi * yi;
+            sim[i] += zr * yi + zi * yr;
           }
-          sre[i] += zr * yr - zi * yi;
-          sim[i] += zr * yi + zi * yr;
         }
       }
     }
