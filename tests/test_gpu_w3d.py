@@ -184,3 +184,49 @@ def test_w3d_wannier_subset_on_a_large_mesh():
         err = relf(a, b)
         print(f"{name}: {err:.2e}", flush=True)
         assert err <= 1e-12, name
+
+
+def test_sixteen_wannier_functions_on_an_11_cubed_mesh():
+    """Sixteen Wannier functions on an 11^3 mesh (NTK 1331): the band grids of
+    one grid point (16 x 1331 complex64) exceed the LDS, so the pair sums take
+    the per-point form with sixteen iw per thread.  The first two Wannier
+    functions alone fit the LDS-tiled form; their func, funcw, X and J agree
+    with the sixteen-function run to rounding (the full mode multiplies the
+    umklapp phase at a different place)."""
+    wl = dict(gen.WORKLOADS["w3d_tiny"])
+    wl["nk"] = 11
+    s = gen.make_system(**{k: wl[k] for k in ("a", "nk", "nocc", "nvir", "ecut_psi", "seed")})
+    from respackdev_amd.chiqw import from_system
+    w = from_system(s)
+    wans = [gen.make_wannier(s, 6, sd)[0] for sd in (5, 17, 29)]
+    ntg = max(c.shape[1] for c in wans[0])
+    W16 = np.zeros((len(wans[0]), 16, ntg), np.complex128)
+    for k in range(len(wans[0])):
+        c = np.concatenate([wn[k] for wn in wans])[:16]
+        W16[k, :, :c.shape[1]] = c
+    em = np.linspace(0.0, 1.0, 2) + 0.01j
+    sqi = np.array([[gen._fold(x) for x in k] for k in w.ski])
+    rng = np.random.default_rng(7)
+    out = {}
+    for nwf in (16, 2):
+        g = W3d(0)
+        try:
+            g.open(w, wl["ecut_eps"], em)
+            g.set_wannier(np.ascontiguousarray(W16[:, :nwf]))
+            ng = g.set_q(sqi)
+            if not out:
+                eps = [np.eye(n)[None] + 0.05 * (rng.standard_normal((len(em), n, n)) +
+                                                1j * rng.standard_normal((len(em), n, n)))
+                       for n in ng]
+            for iq, e in enumerate(eps):
+                g.set_epsinv(iq, e)
+            assert g.info()["ntq"] == 1331
+            func, funcw = g.funcs()
+            x, j, _, _ = g.j3d([(0, 1, 0)])
+        finally:
+            g.close()
+        out[nwf] = (func[:, :2, :2], funcw[:, :, :2, :2], x[:, :2, :2], j[:, :, :2, :2])
+    for a, b, name in zip(out[16], out[2], ("func", "funcw", "X", "J")):
+        err = relf(a, b)
+        print(f"{name}: {err:.2e}", flush=True)
+        assert err <= 1e-12, name
