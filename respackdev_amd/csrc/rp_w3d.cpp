@@ -44,6 +44,7 @@ struct W3d {
   std::vector<int4> qinfo;
   // the q grouped by irreducible eps^-1 for the W contraction
   std::vector<int> wgoff, wgmem;
+  int maps_nwf = 0;  // NWF the device copies of the maps and groups were made for (0: none)
   std::vector<int> minv;
   std::vector<double2> pf1, pf2;
   double chead = 0;
@@ -146,8 +147,10 @@ void check_ready(const W3d& w, const char* what) {
                                         std::to_string(iq + 1) + " missing");
 }
 
-// the per-q rotation maps (eps^-1 itself is already in HBM)
+// the per-q rotation maps (eps^-1 itself is already in HBM), uploaded once per
+// q list and NWF
 void upload_eps_maps(W3d& w, hipStream_t s) {
+  if (w.maps_nwf == w.nwf) return;
   w.d_eps_off.upload(w.eps_off, s);
   w.d_map_off.upload(w.map_off, s);
   w.d_qinfo.upload(w.qinfo, s);
@@ -168,6 +171,7 @@ void upload_eps_maps(W3d& w, hipStream_t s) {
   w.d_wgmem.upload(w.wgmem, s);
   w.d_pf1.upload(w.pf1, s);
   w.d_pf2.upload(w.pf2, s);
+  w.maps_nwf = w.nwf;
 }
 
 // Per q: k+q map, umklapp shifts, |q+G| and the G = 0 slot at q = 0.
@@ -454,6 +458,7 @@ int rp_w3d_set_wannier(rp_handle h, int32_t nwf, const double* wan) {
 int rp_w3d_set_q(rp_handle h, int32_t nq_irr, const double* sqi, int32_t* ng_eps) {
   return guarded([&] {
     W3d& w = w3d_of(h, "rp_w3d_set_q");
+    w.maps_nwf = 0;  // the maps change below
     RP_CHECK(sqi && nq_irr == w.nk_irr, "rp_w3d_set_q: one q per irreducible k (Nq_irr = Nk_irr)");
     w.nq_irr = nq_irr;
     w.sq.assign(sqi, sqi + 3 * (size_t)nq_irr);
