@@ -476,8 +476,9 @@ __global__ void __launch_bounds__(64) k_j_sum(const double2* __restrict__ func,
 }
 
 // V_ij(R) (slot w = nen) and W_ij(R,w): 2*2pi sum_q f_ij(q) e^{-i 2pi q.R} / V / NTQ
-// + head (calc_intW.F:942-985).  grid: (nR, nen + 1), threads over (i,j).
-// out: V [R][j][i], W [w][R][j][i].
+// + head (calc_intW.F:942-985).  grid: (nR, nen + 1), 64 threads, up to four
+// (i,j) each (NWF <= 16); the phases of a block's R are formed once per q in
+// LDS.  out: V [R][j][i], W [w][R][j][i].
 __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ func,
                                                   const double2* __restrict__ funcw,
                                                   const double* __restrict__ sq, int ntq, int nwf,
@@ -491,28 +492,52 @@ __global__ void __launch_bounds__(64) k_w_fourier(const double2* __restrict__ fu
   const int ia1 = r % n1 - na1, ia2 = (r / n1) % n2 - na2, ia3 = r / (n1 * n2) - na3;
   const double tpi = 2.0 * 3.141592653589793238462643383279502884;
   const bool isv = w == nen;
-  for (int ij = threadIdx.x; ij < nwf * nwf; ij += blockDim.x) {
-    const int i = ij % nwf, j = ij / nwf;
-    double sr = 0.0, si = 0.0;
-    for (int q = 0; q < ntq; ++q) {
+  const int np = nwf * nwf;  // <= 256: four (i, j) per thread
+  // e^{-i phase(q)} of kWfQ q at a time in LDS, shared by every (i, j)
+  constexpr int kWfQ = 256;
+  __shared__ double2 ph[kWfQ];
+  double sr[4] = {0.0, 0.0, 0.0, 0.0}, si[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int qb = 0; qb < ntq; qb += kWfQ) {
+    const int nq = min(kWfQ, ntq - qb);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nq; e += blockDim.x) {
+      const int q = qb + e;
       const double phase = tpi * (sq[3 * q] * (double)ia1 + sq[3 * q + 1] * (double)ia2 +
                                   sq[3 * q + 2] * (double)ia3);
-      double s, c;
-      sincos(phase, &s, &c);
-      const double2 f = isv ? func[((size_t)q * nwf + j) * nwf + i]
-                            : funcw[(((size_t)q * nen + w) * nwf + j) * nwf + i];
-      // f * exp(-i phase)
-      sr += f.x * c + f.y * s;
-      si += f.y * c - f.x * s;
+      double sn, cs;
+      sincos(phase, &sn, &cs);
+      ph[e] = make_double2(cs, sn);
     }
-    const double sc = 2.0 * tpi;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int ij = threadIdx.x + 64 * u;
+      if (ij >= np) break;
+      const int i = ij % nwf, j = ij / nwf;
+      for (int e = 0; e < nq; ++e) {
+        const int q = qb + e;
+        const double2 f = isv ? func[((size_t)q * nwf + j) * nwf + i]
+                              : funcw[(((size_t)q * nen + w) * nwf + j) * nwf + i];
+        const double c = ph[e].x, sn = ph[e].y;
+        // f * exp(-i phase)
+        sr[u] += f.x * c + f.y * sn;
+        si[u] += f.y * c - f.x * sn;
+      }
+    }
+  }
+  const double sc = 2.0 * tpi;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int ij = threadIdx.x + 64 * u;
+    if (ij >= np) break;
+    const int i = ij % nwf, j = ij / nwf;
     if (isv) {
-      vmat[((size_t)r * nwf + j) * nwf + i] =
-          make_double2(sc * sr / volume / (double)ntq + chead, sc * si / volume / (double)ntq);
+      vmat[((size_t)r * nwf + j) * nwf + i] = make_double2(
+          sc * sr[u] / volume / (double)ntq + chead, sc * si[u] / volume / (double)ntq);
     } else {
       const double2 h = cheadw[w];
       wmat[(((size_t)w * (n1 * n2 * (2 * na3 + 1)) + r) * nwf + j) * nwf + i] =
-          make_double2(sc * sr / volume / (double)ntq + h.x, sc * si / volume / (double)ntq + h.y);
+          make_double2(sc * sr[u] / volume / (double)ntq + h.x, sc * si[u] / volume / (double)ntq + h.y);
     }
   }
 }
