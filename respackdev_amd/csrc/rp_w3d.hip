@@ -165,17 +165,30 @@ __global__ void __launch_bounds__(512) k_w_contract(
       arow[j] = (a0 + wave + 8 * j) * 16 + li;
     }
     const int nj = min(kWA, (nab - a0 - wave + 7) / 8);  // live blocks of this wave
+    // the next step's X and E are loaded during this step's MFMAs (the last
+    // step loads itself again); columns past nge contribute zero
+    int bn = lk < nge ? lk : nge - 1;
+    double2 xn = Xg[(size_t)bn * kWcVec + li];
+    float2 en[kWA];
+#pragma unroll
+    for (int j = 0; j < kWA; ++j) en[j] = j < nj ? E[(size_t)bn * ld + arow[j]] : make_float2(0.f, 0.f);
     for (int b0 = 0; b0 < nge; b0 += 4) {
-      const int b = b0 + lk;
-      const bool bv = b < nge;  // columns past nge contribute zero
-      const int bc = bv ? b : nge - 1;
-      const double2 xv = Xg[(size_t)bc * kWcVec + li];
+      const bool bv = b0 + lk < nge;
+      const double2 xv = xn;
+      float2 ec[kWA];
+#pragma unroll
+      for (int j = 0; j < kWA; ++j) ec[j] = en[j];
+      const int b1 = (b0 + 4 < nge ? b0 + 4 : b0) + lk;
+      bn = b1 < nge ? b1 : nge - 1;
+      xn = Xg[(size_t)bn * kWcVec + li];
+#pragma unroll
+      for (int j = 0; j < kWA; ++j)
+        if (j < nj) en[j] = E[(size_t)bn * ld + arow[j]];
       const double xr = bv ? xv.x : 0.0, xi = bv ? xv.y : 0.0, nxi = -xi;
 #pragma unroll
       for (int j = 0; j < kWA; ++j) {
         if (j >= nj) break;  // uniform
-        const float2 ev = E[(size_t)bc * ld + arow[j]];
-        const double er = ev.x, ei = ev.y;
+        const double er = ec[j].x, ei = ec[j].y;
         accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr, accr[j], 0, 0, 0);
         accr[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(ei, nxi, accr[j], 0, 0, 0);
         acci[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xi, acci[j], 0, 0, 0);
