@@ -337,22 +337,36 @@ __global__ void __launch_bounds__(512) k_j_contract(
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) accr[j][cb] = acci[j][cb] = jd4{0.0, 0.0, 0.0, 0.0};
     }
+    // the next step's X and E are loaded during this step's MFMAs (the last
+    // step loads itself again); columns past nge contribute zero
+    int bn = lk < nge ? lk : nge - 1;
+    double2 xn[2];
+    float2 en[kJA];
+#pragma unroll
+    for (int cb = 0; cb < 2; ++cb) xn[cb] = Xq[(size_t)bn * g32 + cb * 16 + li];
+#pragma unroll
+    for (int j = 0; j < kJA; ++j) en[j] = E[(size_t)bn * ld + arow[j]];
     for (int b0 = 0; b0 < nge; b0 += 4) {
-      const int b = b0 + lk;
-      const bool bv = b < nge;  // columns past nge contribute zero
-      const int bc = bv ? b : nge - 1;
+      const bool bv = b0 + lk < nge;
       double xr[2], xi[2], nxi[2];
+      float2 ec[kJA];
 #pragma unroll
       for (int cb = 0; cb < 2; ++cb) {
-        const double2 xv = Xq[(size_t)bc * g32 + cb * 16 + li];
-        xr[cb] = bv ? xv.x : 0.0;
-        xi[cb] = bv ? xv.y : 0.0;
+        xr[cb] = bv ? xn[cb].x : 0.0;
+        xi[cb] = bv ? xn[cb].y : 0.0;
         nxi[cb] = -xi[cb];
       }
 #pragma unroll
+      for (int j = 0; j < kJA; ++j) ec[j] = en[j];
+      const int b1 = (b0 + 4 < nge ? b0 + 4 : b0) + lk;
+      bn = b1 < nge ? b1 : nge - 1;
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) xn[cb] = Xq[(size_t)bn * g32 + cb * 16 + li];
+#pragma unroll
+      for (int j = 0; j < kJA; ++j) en[j] = E[(size_t)bn * ld + arow[j]];
+#pragma unroll
       for (int j = 0; j < kJA; ++j) {
-        const float2 ev = E[(size_t)bc * ld + arow[j]];
-        const double er = ev.x, ei = ev.y;
+        const double er = ec[j].x, ei = ec[j].y;
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
           accr[j][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(er, xr[cb], accr[j][cb], 0, 0, 0);
