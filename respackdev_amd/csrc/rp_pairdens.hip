@@ -576,8 +576,8 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
 // Wannier function, every k) held in LDS: each grid value is then read from
 // HBM once per tile and workgroup instead of once per q.  Arithmetic and
 // summation order per (q, pair, r) are those of k_pair_rsum, except that the
-// full mode multiplies the umklapp phase into the jw factor (the same product,
-// rounded differently: about 1e-16 relative).
+// umklapp phase multiplies the jw factor (the same product, rounded
+// differently: about 1e-16 relative).
 // grid (ceil(nr/TR), ngroups, nqsplit), 512 threads; thread (rl = t % TR,
 // ql = t / TR) takes the point r0 + rl and the q = qbeg + ql, qbeg + ql + QL, ...
 // of its workgroup's range of qper q.  Dynamic LDS: the phase tables, w_k
@@ -688,16 +688,16 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
             }
           }
         } else {
+          // conj(u_w,k+q ph) (w_k u_w,k) = conj(u_w,k+q) (conj(ph) w_k u_w,k)
+          const double2 cw = make_double2(ph.x * wv.x + ph.y * wv.y, ph.x * wv.y - ph.y * wv.x);
 #pragma unroll
           for (int a = 0; a < NA; ++a) {
             if (a >= nwf) continue;
-            // b = u_w,k+q(r) e^{-i 2pi r.dG/N}, conj(b) (w_k u_w,k)
             const float2 u = su[(a * ntk + kq) * TR + rl];
-            const double2 b = make_double2((double)u.x * ph.x - (double)u.y * ph.y,
-                                           (double)u.x * ph.y + (double)u.y * ph.x);
+            const double2 b = make_double2((double)u.x, (double)u.y);
             const float2 v = su[(a * ntk + k) * TR + rl];
-            const double2 aw = make_double2((double)v.x * wv.x - (double)v.y * wv.y,
-                                            (double)v.x * wv.y + (double)v.y * wv.x);
+            const double2 aw = make_double2((double)v.x * cw.x - (double)v.y * cw.y,
+                                            (double)v.x * cw.y + (double)v.y * cw.x);
             acc[a][0].x += b.x * aw.x + b.y * aw.y;
             acc[a][0].y += b.x * aw.y - b.y * aw.x;
           }
