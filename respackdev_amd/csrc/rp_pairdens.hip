@@ -547,6 +547,7 @@ __global__ void __launch_bounds__(256) k_pair_rsum(const float2* __restrict__ ur
       if constexpr (FULL) {
 #pragma unroll
         for (int i = 0; i < NI; ++i) {  // conj(b_iw) (w_k a_jw)
+          if (i >= nwf) break;  // uniform
           acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
           acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
         }
@@ -683,6 +684,7 @@ __global__ void __launch_bounds__(kRsumTileThreads) k_pair_rsum_tile(
                                             (double)v.x * cw.y + (double)v.y * cw.x);
 #pragma unroll
             for (int i = 0; i < NI; ++i) {
+              if (i >= nwf) break;  // uniform
               acc[a][i].x += bq[i].x * aw.x + bq[i].y * aw.y;
               acc[a][i].y += bq[i].x * aw.y - bq[i].y * aw.x;
             }
