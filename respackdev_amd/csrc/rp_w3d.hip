@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) k_w_func(const double2* __restrict__ rho,
 typedef double jd4 __attribute__((ext_vector_type(4)));
 constexpr int kMaxWf = 16;     // Wannier functions of calc_w3d and calc_j3d
 constexpr int kWcVec = 16;     // vectors of a group: one MFMA column block
-constexpr int kWA = 4;         // row blocks per wave and pass of the W contraction
+constexpr int kWA = 3;         // row blocks per wave and pass of the W contraction
 template <int NV>
 constexpr int wc_members() { return kWcVec / NV; }
 int w_contract_members(int nwf) { return kWcVec / nwf; }
